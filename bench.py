@@ -1,0 +1,185 @@
+"""Benchmark: GUPS of the cone-beam operator pair at 512^3 / 512 views on one GPU.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 512] [--nphi 512]
+                    [--no-cpu-baseline]
+
+A step is one SIDDONVI sweep in FP32: a clean Siddon projection of the
+volume followed by a VI backprojection of the sinogram, device-resident
+(512^3 volume of 64 seeded random ellipsoids, 512 views over 360 deg,
+512x512 detector, FOD 1000 mm, FDD 2000 mm, 1 mm voxels, 2 mm pixels).
+GUPS = voxel-view updates / s = 2 N^3 Nphi / t_step.  The RS projector is
+timed alongside.  Kernel times come from HIP events recorded on the stream
+the kernels run on.  One JSON line is printed.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "GUPS proj & backproj at 512³/512 views, %HBM roofline; mixed vs FP32 @L2≤1e-3; p99 lat"
+HBM_PEAK = 8.0e12  # B/s, MI355X HBM3E
+
+
+def parse():
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=512)
+    ap.add_argument("--nphi", type=int, default=512)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-views", type=int, default=4, help="views in the CPU baseline sample")
+    return ap.parse_args()
+
+
+def timed(fn, reps, stream):
+    """Mean ms of `fn` over `reps` launches, HIP events on `stream`."""
+    import torch
+    starts = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    ends = [torch.cuda.Event(enable_timing=True) for _ in range(reps)]
+    for i in range(reps):
+        starts[i].record(stream)
+        fn()
+        ends[i].record(stream)
+    torch.cuda.synchronize()
+    ts = [s.elapsed_time(e) for s, e in zip(starts, ends)]
+    return sum(ts) / len(ts), ts
+
+
+def cpu_baseline(vol_host, n, nphi, views, threads):
+    """FP32 oracle (test-infrastructure C restatement) on the host cores:
+    VI backprojection and clean Siddon projection over the first `views`
+    views of the same acquisition."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    from oracle_ffi import Oracle, scaled_geometry, set_threads
+    set_threads(threads)
+    g = scaled_geometry(n, views)
+    g.num_stop, g.den_stop = 2.0 * views, float(nphi)   # same angles as views 0..views-1
+    o = Oracle(g)
+    t0 = time.perf_counter()
+    sino = o.siddon(vol_host)
+    t1 = time.perf_counter()
+    o.vi(sino)
+    t2 = time.perf_counter()
+    upd = float(n) ** 3 * views
+    return {
+        "value": round(2 * upd / (t2 - t0) / 1e9, 4),
+        "unit": "GUPS",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"clean Siddon projection + VI backprojection of the {n}^3 phantom over views 0..{views - 1} "
+                  f"of {nphi} (FP32 oracle, OpenMP): siddon {t1 - t0:.2f} s, vi {t2 - t1:.2f} s",
+        "siddon_gups": round(upd / (t1 - t0) / 1e9, 4),
+        "vi_gups": round(upd / (t2 - t1) / 1e9, 4),
+    }
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print(json.dumps({"error": f"bench.py runs on exactly one GPU; --gpus {a.gpus} is not supported "
+                                   "(no multi-GPU path in this build)"}))
+        sys.exit(2)
+    import torch
+    from tomogpi_amd import lib
+
+    n, nphi = a.n, a.nphi
+    ctx = lib.Context(0)
+    desc = lib.AcquisitionDesc.scaled(n, nphi)
+    G = lib.Geometry(ctx, desc)
+    stream = torch.cuda.current_stream()
+    vol = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+    G.ellipsoids(vol, count=64, seed=0, amin=5.0 * n / 512, amax=120.0 * n / 512)
+    sino = torch.empty(G.sino_shape, dtype=torch.float32, device="cuda")
+    back = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+    n_seg = G.project_work(lib.PROJ_SIDDON)
+    n_samp = G.project_work(lib.PROJ_RS)
+
+    def siddon():
+        G.project(vol, sino, lib.PROJ_SIDDON, stream=stream)
+
+    def vi():
+        G.backproject(sino, back, stream=stream)
+
+    def rs():
+        G.project(vol, sino, lib.PROJ_RS, stream=stream)
+
+    for _ in range(a.warmup):
+        siddon(); vi()
+    rs()
+    torch.cuda.synchronize()
+
+    # Timed region: K steps, synchronised on both sides; per-kernel events.
+    sid_ev, vi_ev = [], []
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        s0, s1, s2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
+        s0.record(stream); siddon(); s1.record(stream); vi(); s2.record(stream)
+        sid_ev.append((s0, s1)); vi_ev.append((s1, s2))
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    ms_step = wall * 1e3 / a.steps
+    sid_ms = sum(x.elapsed_time(y) for x, y in sid_ev) / a.steps
+    vi_ms = sum(x.elapsed_time(y) for x, y in vi_ev) / a.steps
+    rs_ms, _ = timed(rs, max(2, min(a.steps, 3)), stream)
+
+    upd = float(n) ** 3 * nphi
+    nsino = float(G.info["nu"]) * G.info["nv"] * nphi
+    b_vi = upd * 4 * 4 + float(n) ** 3 * 4
+    b_sid = n_seg * 4 + nsino * 4
+    b_rs = n_samp * 8 * 4 + nsino * 4
+    ops = {
+        "siddon": {"ms": round(sid_ms, 3), "gups": round(upd / sid_ms / 1e6, 2), "alg_bytes": b_sid,
+                   "hbm_frac": round(b_sid / (sid_ms * 1e-3) / HBM_PEAK, 4), "n_seg": n_seg},
+        "vi": {"ms": round(vi_ms, 3), "gups": round(upd / vi_ms / 1e6, 2), "alg_bytes": b_vi,
+               "hbm_frac": round(b_vi / (vi_ms * 1e-3) / HBM_PEAK, 4)},
+        "rs": {"ms": round(rs_ms, 3), "gups": round(upd / rs_ms / 1e6, 2), "alg_bytes": b_rs,
+               "hbm_frac": round(b_rs / (rs_ms * 1e-3) / HBM_PEAK, 4), "n_samp": n_samp},
+    }
+    dom = "vi" if vi_ms >= sid_ms else "siddon"
+    dom_bytes = b_vi if dom == "vi" else b_sid
+    dom_ms = vi_ms if dom == "vi" else sid_ms
+    achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "vi_kernel" if dom == "vi" else "siddon_kernel",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
+                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None}
+
+    base = None
+    if not a.no_cpu_baseline:
+        threads = min(16, os.cpu_count() or 1)
+        base = cpu_baseline(vol.cpu().numpy(), n, nphi, a.cpu_views, threads)
+
+    out = {
+        "metric": METRIC,
+        "value": round(2 * upd / (ms_step * 1e-3) / 1e9, 3),
+        "unit": "GUPS",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"SIDDONVI sweep (clean Siddon projection + VI backprojection), {n}^3 volume of "
+                               f"64 random ellipsoids (seed 0), {nphi} views over 360 deg, {n}x{n} detector, "
+                               f"FOD {desc.fod_mm:g} mm, FDD {desc.fdd_mm:g} mm, FP32 device-resident",
+                   "n": n, "nphi": nphi, "detector": [G.info["nu"], G.info["nv"]]},
+        "roofline": roofline,
+        "cpu_baseline": base,
+        "operators": ops,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

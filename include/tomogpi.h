@@ -1,0 +1,249 @@
+/*
+ * tomogpi.h - C ABI of the MI355X cone-beam projector / backprojector library
+ * (libtomogpi_mi355x.so).
+ *
+ * The library replaces the operator layer of TomoGPI (reference paths are
+ * relative to the reference repository root; C = src/TomoGPI_lib/src/CPP,
+ * I = src/TomoGPI_lib/inc):
+ *
+ *   Projector<V,S,T>::doProjection(S<T>* sino, V<T>* vol)      I/Projector.cuh:118
+ *   BackProjector<V,S,T>::doBackProjection(V<T>* vol, S<T>* s)  I/BackProjector.cuh:144
+ *   Projector<V,S,T>::updateVolume(out, in, lambda, positive)   C/Projector.cu:109-124
+ *   Iter3D::doSimpleGradient(...)                               C/Iter3D_Algo_Gradient.cu:25-223
+ *
+ * Conventions (identical to the reference containers):
+ *   volume   f[x + Nx*(y + Ny*z)]          x fastest   (C/Image3D.cu:140-233)
+ *   sinogram g[u + Nu*(v + Nv*p)]          u fastest   (C/Sinogram3D.cu:138-184)
+ *   Every operator OVERWRITES its output (reference kernels store, never
+ *   accumulate: K/BackProjector/backprojection3D_kernel_16reg_UM.cuh:129-145,
+ *   K/Projector/projection_ER_kernel_UM.cuh:143).
+ *
+ * Every entry point returns TOMO_OK (0) or a TOMO_ERR_* code and never calls
+ * exit(); tomo_last_error() returns a thread-local message for the last
+ * failure on the calling thread.  Pointers are device pointers unless the
+ * TOMO_HOST_PTR flag is given; `stream` is a hipStream_t (NULL = the null
+ * stream).  Device-pointer calls are stream-ordered and asynchronous;
+ * TOMO_HOST_PTR calls are synchronous.  A context may be used from several
+ * threads; calls that share a geometry and a stream are ordered by the stream.
+ */
+#ifndef TOMOGPI_H
+#define TOMOGPI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define TOMOGPI_ABI_VERSION 1
+
+/* ---- status codes ------------------------------------------------------ */
+#define TOMO_OK                0
+#define TOMO_ERR_INVALID_ARG   1   /* bad pointer, size, enum or geometry   */
+#define TOMO_ERR_HIP           2   /* a HIP runtime call failed             */
+#define TOMO_ERR_UNSUPPORTED   3   /* combination not implemented           */
+#define TOMO_ERR_OUT_OF_MEMORY 4   /* device allocation refused             */
+#define TOMO_ERR_IO            5   /* file missing, short or unwritable     */
+#define TOMO_ERR_CONFIG        6   /* config file or key missing            */
+#define TOMO_ERR_BUSY          7   /* serving queue full / not admitted     */
+
+/* ---- element types of volumes and sinograms ---------------------------- */
+#define TOMO_F32   0
+#define TOMO_F16   1   /* stored = value * scale, decoded as stored / scale */
+#define TOMO_BF16  2
+#define TOMO_F64   3   /* host-side I/O only */
+
+/* ---- operators ---------------------------------------------------------- */
+#define TOMO_PROJ_RS      0   /* regular-sampling ray-driven projector      */
+#define TOMO_PROJ_SIDDON  1   /* Siddon exact-chord projector                */
+#define TOMO_BP_VI        0   /* voxel-driven bilinear backprojector         */
+
+/* ---- flags --------------------------------------------------------------- */
+#define TOMO_FLAG_RS_FOV_CUBE   0x1  /* RS: circumscribed circle + cube clip
+                                        (projection_ER_kernel.cuh:56-312, the
+                                        reference's mem_CPU default); default
+                                        is the inscribed cylinder
+                                        (projection_ER_kernel_UM.cuh:25-153) */
+#define TOMO_FLAG_SIDDON_COMPAT 0x2  /* Siddon: reference C-path arithmetic and
+                                        quirks (SiddonProjector_compute_C_mem_CPU
+                                        .cpp:20-276): float traversal, first
+                                        voxel half a voxel behind, last chord
+                                        unclipped, indices wrapped modulo N, no
+                                        z clip. Default is the clean Siddon. */
+#define TOMO_FLAG_HOST_PTR      0x4  /* data pointers are host memory: stage,
+                                        compute, copy back, synchronise       */
+
+typedef struct tomo_ctx  tomo_ctx;
+typedef struct tomo_geom tomo_geom;
+
+/* Raw acquisition description, field for field the keys of the reference's
+ * configCT file (C/Config/ConfigCT.cpp:110-164, Data3D_0256/Projections_256/
+ * configCT).  Derived quantities follow the reference constructors:
+ *   step = pi*(stop-start)/Nphi, phi_p = (p+start)*step   (ConfigCT.cpp:119,
+ *                                                           Acquisition.cpp:90-97)
+ *   du = Su/Nu, cu = Nu/2-0.5 (float)                      (Detector.cpp:9-14)
+ *   delta = Sx/Nx, delta_z := delta, c = N/2-0.5 (float)   (Volume.cu:17-25)  */
+typedef struct tomo_geom_desc {
+    double  fod_mm;                 /* FOD: focus-object distance            */
+    double  fdd_mm;                 /* FDD: focus-detector distance          */
+    int32_t projection_nb;          /* projectionNb                          */
+    double  num_start_angle, den_start_angle;   /* start = num/den (x pi)    */
+    double  num_stop_angle,  den_stop_angle;    /* stop  = num/den (x pi)    */
+    float   u_detector_size_mm;     /* horizontalDetectorSize                */
+    float   v_detector_size_mm;     /* verticalDetectorSize                  */
+    int32_t u_pixel_nb;             /* horizontalDetectorPixelNb             */
+    int32_t v_pixel_nb;             /* verticalDetectorPixelNb               */
+    float   x_volume_size_mm, y_volume_size_mm, z_volume_size_mm;
+    int32_t x_pixel_nb, y_pixel_nb, z_pixel_nb;
+} tomo_geom_desc;
+
+/* Derived geometry as the kernels see it (read-only view). */
+typedef struct tomo_geom_info {
+    int32_t nx, ny, nz, nu, nv, nphi;
+    double  fod, fdd;
+    float   du, dv, cu, cv;         /* detector pixel size and centre pixel  */
+    float   delta, cx, cy, cz;      /* voxel size (all axes) and centre      */
+    double  step_angle, start_angle;
+} tomo_geom_info;
+
+/* ---- library / context --------------------------------------------------- */
+int         tomo_abi_version(void);
+const char* tomo_last_error(void);
+const char* tomo_status_string(int status);
+
+/* Binds device `device` (hipSetDevice) for every later call on this context. */
+int tomo_ctx_create(int device, tomo_ctx** out);
+int tomo_ctx_destroy(tomo_ctx* ctx);
+/* Free / total HBM of the context's device (bytes). */
+int tomo_ctx_mem_info(tomo_ctx* ctx, size_t* free_bytes, size_t* total_bytes);
+
+/* Device memory helpers for callers without their own allocator. */
+int tomo_alloc(tomo_ctx* ctx, size_t bytes, void** dptr);
+int tomo_free(tomo_ctx* ctx, void* dptr);
+int tomo_copy_to_device(tomo_ctx* ctx, void* dst, const void* src, size_t bytes);
+int tomo_copy_to_host(tomo_ctx* ctx, void* dst, const void* src, size_t bytes);
+int tomo_synchronize(tomo_ctx* ctx, void* stream);
+
+/* ---- geometry ------------------------------------------------------------ */
+/* Builds the per-view FP64 tables and uploads them to the context's device.  */
+int tomo_geom_create(tomo_ctx* ctx, const tomo_geom_desc* desc, tomo_geom** out);
+/* Reads <workdir>/configCT with the reference's token-scan semantics
+ * (C/Config/Config.cpp:34-180: first token equal to the key, value = next
+ * token; numbers through atoi/atof).  A missing file or key returns
+ * TOMO_ERR_CONFIG instead of exiting.                                        */
+int tomo_geom_desc_from_config(const char* workdir, tomo_geom_desc* out);
+int tomo_geom_destroy(tomo_geom* geom);
+int tomo_geom_get_info(const tomo_geom* geom, tomo_geom_info* out);
+/* phi_p in radians, FP64, p = 0..nphi-1 (Acquisition.cpp:90-97). */
+int tomo_geom_get_phi(const tomo_geom* geom, double* phi_out);
+
+/* ---- operators (replace doProjection / doBackProjection) ------------------ */
+/* sinogram <- P volume.  projector = TOMO_PROJ_RS | TOMO_PROJ_SIDDON.
+ * Replaces RegularSamplingProjector_compute_CUDA_mem_{GPU,CPU}::doProjection
+ * (C/RegularSamplingProjector_compute_CUDA_mem_GPU.cu:35-92, ..._mem_CPU.cu:26)
+ * and SiddonProjector_compute_C_mem_CPU::doProjection
+ * (C/SiddonProjector_compute_C_mem_CPU.cpp:20-276).                          */
+int tomo_project(tomo_ctx* ctx, const tomo_geom* geom, int projector, int flags,
+                 const void* volume, int volume_dtype, float volume_scale,
+                 void* sinogram, int sinogram_dtype, float sinogram_scale,
+                 void* stream);
+
+/* volume <- B sinogram.  backprojector = TOMO_BP_VI.
+ * Replaces VIBackProjector_compute_CUDA_mem_{GPU,CPU}::doBackProjection
+ * (C/VIBackProjector_compute_CUDA_mem_GPU.cu:26-111, ..._mem_CPU.cu:19-690). */
+int tomo_backproject(tomo_ctx* ctx, const tomo_geom* geom, int backprojector, int flags,
+                     const void* sinogram, int sinogram_dtype, float sinogram_scale,
+                     void* volume, int volume_dtype, float volume_scale,
+                     void* stream);
+
+/* Exact transposes, new (the reference has none; needed for <Ax,y>=<x,A^T y>).
+ * volume <- P^T sinogram and sinogram <- B^T volume.  Scatter with FP32
+ * atomics; output is zeroed first.                                           */
+int tomo_project_adjoint(tomo_ctx* ctx, const tomo_geom* geom, int projector, int flags,
+                         const void* sinogram, int sinogram_dtype, float sinogram_scale,
+                         void* volume, int volume_dtype, float volume_scale,
+                         void* stream);
+int tomo_backproject_adjoint(tomo_ctx* ctx, const tomo_geom* geom, int backprojector, int flags,
+                             const void* volume, int volume_dtype, float volume_scale,
+                             void* sinogram, int sinogram_dtype, float sinogram_scale,
+                             void* stream);
+
+/* Work units of one projection, for roofline accounting: the number of
+ * trilinear samples sum L_r (RS) or of positive-length chords (clean
+ * Siddon) over all rays.  Synchronous.                                       */
+int tomo_project_work(tomo_ctx* ctx, const tomo_geom* geom, int projector, int flags,
+                      uint64_t* units, void* stream);
+
+/* out <- in, element-type conversion with the storage scales.               */
+int tomo_convert(tomo_ctx* ctx, const void* in, int in_dtype, float in_scale,
+                 void* out, int out_dtype, float out_scale, size_t n, void* stream);
+
+/* ---- loop glue (Image3D/Sinogram3D vector ops, C/Image3D_CPU.cu:108-390,
+ *      C/Sinogram3D_CPU.cu:311-375) ------------------------------------------ */
+/* out = a - b  (diffSinogram: this = sinogram1 - sinogram2)                  */
+int tomo_diff(tomo_ctx* ctx, const float* a, const float* b, float* out, size_t n, void* stream);
+/* x *= s  (scalarVolume)                                                     */
+int tomo_scale(tomo_ctx* ctx, float* x, float s, size_t n, void* stream);
+/* y += lambda * x, then y = max(y,0) if positive (updateVolume)             */
+int tomo_axpy(tomo_ctx* ctx, float* y, const float* x, double lambda, int positive,
+              size_t n, void* stream);
+/* sum_i double(x_i*x_i)  ("L2Norm" of the reference: no square root).
+ * Synchronous: the result is on the host when the call returns.             */
+int tomo_sq_norm(tomo_ctx* ctx, const void* x, int dtype, float scale, size_t n,
+                 double* out, void* stream);
+/* <a,b> in FP64 (adjoint tests).  Synchronous.                               */
+int tomo_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, void* stream);
+/* y = L x, L = 7-point Laplacian 6x - sum of 6 neighbours, zero padding
+ * (separable {-1,2,-1} passes accumulated: C/Convolution3D_CPU_GPU.cu:146-148,
+ *  C/GeneralizedGaussianRegularizer_CPU.cu:158-226).                         */
+int tomo_laplacian(tomo_ctx* ctx, const tomo_geom* geom, const float* x, float* y, void* stream);
+
+/* ---- simple-gradient reconstruction (Iter3D::doSimpleGradient) ----------- */
+#define TOMO_PAIR_RSVI      0
+#define TOMO_PAIR_SIDDONVI  1
+
+typedef struct tomo_gd_params {
+    int32_t pair;                    /* TOMO_PAIR_*                          */
+    int32_t flags;                   /* operator flags (FOV mode, compat)    */
+    int32_t gradient_iteration_nb;   /* gradientIterationNb                   */
+    int32_t optimal_step_iteration_nb; /* optimalStepIterationNb             */
+    double  lambda;                  /* regularizationParameter (GG, beta=2) */
+    int32_t positivity;              /* positivity                           */
+    double  alpha_min_init;          /* -0.0347711 in the reference          */
+} tomo_gd_params;
+
+typedef struct tomo_gd_iter_stat {
+    double residual_sq;              /* sum (y - P x)^2 before the update    */
+    double norm_dj;                  /* sum dJ^2 (0 when the step is fixed)   */
+    double norm_dj_proj;             /* 2 sum (P dJ)^2 + lambda normdJProjReg */
+    double alpha;                    /* step used                            */
+} tomo_gd_iter_stat;
+
+/* Runs the loop on device-resident data (or host data with TOMO_HOST_PTR in
+ * params->flags).  `volume` holds the initial estimate on entry and the
+ * result on exit.  stats (may be NULL) receives gradient_iteration_nb
+ * records.  Synchronous (the step rule needs the norms on the host).        */
+int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* params,
+                const float* sinogram, float* volume, tomo_gd_iter_stat* stats,
+                void* stream);
+
+/* ---- phantoms (test and benchmark inputs) -------------------------------- */
+/* 3D Shepp-Logan, modified-SL geometry with densities 2,-.98,-.02,-.02,.01x6
+ * (Data3D_0256/phantom3d.m:175-200), voxel centres mapped to [-1,1].         */
+int tomo_phantom_shepp_logan(tomo_ctx* ctx, const tomo_geom* geom, float* volume, void* stream);
+/* Sum of `count` random ellipsoids (seeded): centres uniform in the inner 80%
+ * of the volume, semi-axes uniform in [amin,amax] voxels, uniformly random
+ * rotations, densities U[0,1).                                               */
+int tomo_phantom_ellipsoids(tomo_ctx* ctx, const tomo_geom* geom, int count, uint64_t seed,
+                            double amin, double amax, float* volume, void* stream);
+
+/* ---- raw file I/O (C/Image3D.cu:140-233, C/Sinogram3D.cu:138-184) ---------- */
+/* Headerless native-endian dumps of `count` elements of `dtype`.             */
+int tomo_file_read(const char* path, void* host_dst, int dtype, size_t count);
+int tomo_file_write(const char* path, const void* host_src, int dtype, size_t count);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* TOMOGPI_H */

@@ -1,0 +1,175 @@
+"""GPU parity: the HIP operators against the CPU oracle on the same inputs.
+
+The FP32 kernels follow the oracle's FP32 recipe operation for operation
+(FP64 per-ray / per-column geometry, explicit fmaf, sequential sums), so the
+forward operators are compared bit for bit (tolerance 0), with a relative
+L2/Linf fallback stated per test where summation order legitimately
+differs.  Transposes use FP32 atomics (order not fixed) and are compared
+with the FP64 oracle at rel-L2 <= 1e-5.
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle_ffi import Geometry, Oracle, ellipsoids, shepp_logan, scaled_geometry, laplacian as og_laplacian
+from tomogpi_amd import lib
+
+pytestmark = pytest.mark.gpu
+
+
+def to_desc(g: Geometry) -> lib.AcquisitionDesc:
+    return lib.AcquisitionDesc(
+        fod_mm=g.fod, fdd_mm=g.fdd, projection_nb=g.nphi, u_detector_size_mm=g.su, v_detector_size_mm=g.sv,
+        u_pixel_nb=g.nu, v_pixel_nb=g.nv, x_volume_size_mm=g.sx, y_volume_size_mm=g.sy, z_volume_size_mm=g.sz,
+        x_pixel_nb=g.nx, y_pixel_nb=g.ny, z_pixel_nb=g.nz, num_start_angle=g.num_start,
+        den_start_angle=g.den_start, num_stop_angle=g.num_stop, den_stop_angle=g.den_stop)
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.linalg.norm(a - b) / max(np.linalg.norm(b), 1e-300))
+
+
+def rel_linf(a, b):
+    a = np.asarray(a, dtype=np.float64)
+    b = np.asarray(b, dtype=np.float64)
+    return float(np.abs(a - b).max() / max(np.abs(b).max(), 1e-300))
+
+
+# Geometries: a cube case and a ragged one (odd sizes, non-square detector,
+# partial angular range, non-zero start angle).
+GEOMS = {
+    "cube48": scaled_geometry(48, 24),
+    "ragged": Geometry(fod=230.0, fdd=470.0, nphi=37, su=61 * 1.7, sv=29 * 2.3, nu=61, nv=29, sx=41 * 0.9,
+                       sy=35 * 0.9, sz=23 * 0.9, nx=41, ny=35, nz=23, num_start=1.0, den_start=3.0,
+                       num_stop=5.0, den_stop=4.0),
+}
+
+
+@pytest.fixture(scope="module", params=list(GEOMS))
+def setup(request, ctx):
+    g = GEOMS[request.param]
+    print(f"[case] geometry={request.param}", flush=True)
+    return g, Oracle(g), lib.Geometry(ctx, to_desc(g))
+
+
+def dev(a):
+    return torch.from_numpy(np.ascontiguousarray(a)).cuda()
+
+
+def test_geometry_info_matches_oracle(setup):
+    g, o, G = setup
+    oi = o.info
+    for k in ("du", "dv", "cu", "cv", "delta", "cx", "cy", "cz"):
+        assert G.info[k] == pytest.approx(oi[k], rel=0, abs=0), k
+    assert np.array_equal(G.phi(), o.phi)
+
+
+def test_vi_bit_exact(setup):
+    g, o, G = setup
+    rng = np.random.default_rng(11)
+    sino = rng.random(g.sino_shape, dtype=np.float32)
+    ref = o.vi(sino)
+    out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+    G.backproject(dev(sino), out)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+@pytest.mark.parametrize("cube", [False, True])
+def test_rs_bit_exact(setup, cube):
+    g, o, G = setup
+    vol = ellipsoids(g.nx, g.ny, g.nz, count=12, seed=3, amin=2.0, amax=12.0)
+    ref = o.rs(vol, cube=cube)
+    out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+    G.project(dev(vol), out, lib.PROJ_RS, lib.FLAG_RS_FOV_CUBE if cube else 0)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+def test_siddon_bit_exact(setup):
+    g, o, G = setup
+    rng = np.random.default_rng(5)
+    vol = rng.random(g.vol_shape, dtype=np.float32)
+    ref = o.siddon(vol)
+    out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+    G.project(dev(vol), out, lib.PROJ_SIDDON)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+@pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "vi"])
+def test_adjoint_dot(setup, ctx, op):
+    """<A x, y> = <x, A^T y> with the GPU operator and its GPU transpose (rel 1e-5)."""
+    g, o, G = setup
+    rng = np.random.default_rng(1)
+    x = dev(rng.random(g.vol_shape, dtype=np.float32))
+    y = dev(rng.random(g.sino_shape, dtype=np.float32))
+    if op == "vi":
+        bx = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.backproject(y, bx)
+        bty = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        G.backproject_adjoint(x, bty)
+        lhs, rhs = lib.dot(ctx, bx, x), lib.dot(ctx, y, bty)
+    else:
+        proj = lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS
+        flags = lib.FLAG_RS_FOV_CUBE if op == "rs_cube" else 0
+        ax = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        G.project(x, ax, proj, flags)
+        aty = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.project_adjoint(y, aty, proj, flags)
+        lhs, rhs = lib.dot(ctx, ax, y), lib.dot(ctx, x, aty)
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) <= 1e-5, (lhs, rhs)
+
+
+@pytest.mark.parametrize("op", ["rs", "siddon", "vi"])
+def test_transpose_vs_oracle(setup, op):
+    g, o, G = setup
+    rng = np.random.default_rng(2)
+    if op == "vi":
+        x = rng.random(g.vol_shape, dtype=np.float32)
+        ref = o.vi_T(x)
+        out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        G.backproject_adjoint(dev(x), out)
+    else:
+        y = rng.random(g.sino_shape, dtype=np.float32)
+        ref = o.rs_T(y) if op == "rs" else o.siddon_T(y)
+        out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.project_adjoint(dev(y), out, lib.PROJ_RS if op == "rs" else lib.PROJ_SIDDON)
+    assert rel_l2(out.cpu().numpy(), ref) <= 1e-5
+
+
+def test_work_counters(setup):
+    g, o, G = setup
+    assert G.project_work(lib.PROJ_RS) == o.rs_count()
+    assert G.project_work(lib.PROJ_RS, lib.FLAG_RS_FOV_CUBE) == o.rs_count(cube=True)
+    # FP32 traversal on the GPU vs FP64 in the oracle: zero-length ties may differ.
+    assert abs(G.project_work(lib.PROJ_SIDDON) - o.siddon_count()) <= 1e-4 * o.siddon_count()
+
+
+def test_phantoms_bit_exact(setup):
+    g, o, G = setup
+    out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+    G.shepp_logan(out)
+    assert np.array_equal(out.cpu().numpy(), shepp_logan(g.nx, g.ny, g.nz))
+    G.ellipsoids(out, count=9, seed=17, amin=2.0, amax=9.0)
+    assert np.array_equal(out.cpu().numpy(), ellipsoids(g.nx, g.ny, g.nz, count=9, seed=17, amin=2.0, amax=9.0))
+
+
+def test_laplacian_bit_exact(setup):
+    g, o, G = setup
+    x = np.random.default_rng(4).random(g.vol_shape, dtype=np.float32)
+    y = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+    G.laplacian(dev(x), y)
+    torch.cuda.synchronize()
+    assert np.array_equal(y.cpu().numpy(), og_laplacian(x))
+
+
+def test_host_pointer_path(setup):
+    """TOMO_FLAG_HOST_PTR (the reference's mem_CPU storage) gives the device result."""
+    g, o, G = setup
+    sino = np.random.default_rng(9).random(g.sino_shape, dtype=np.float32)
+    out = np.zeros(g.vol_shape, dtype=np.float32)
+    G.backproject(sino, out)
+    assert np.array_equal(out, o.vi(sino))

@@ -1,0 +1,60 @@
+// device_common.hpp - device helpers shared by the kernel files.
+//
+// Arithmetic contract: every file is compiled with -ffp-contract=off and
+// correctly rounded FP32 division / square root, so each fused multiply-add
+// below is an explicit fmaf()/fma() call.  The CPU oracle restates the same
+// operations in the same order, which makes the FP32 forms bit-comparable.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <hip/hip_bf16.h>
+
+#include <cstdint>
+
+#include "tomo_internal.hpp"
+
+namespace tomo {
+
+// ---- storage types: value = (float)stored * inv_scale; stored = value*scale
+template <typename T> struct Storage;
+template <> struct Storage<float> {
+    __device__ __forceinline__ static float load(const float* p, uint64_t i, float) { return p[i]; }
+    __device__ __forceinline__ static void store(float* p, uint64_t i, float v, float) { p[i] = v; }
+};
+template <> struct Storage<__half> {
+    __device__ __forceinline__ static float load(const __half* p, uint64_t i, float inv) {
+        return __half2float(p[i]) * inv;
+    }
+    __device__ __forceinline__ static void store(__half* p, uint64_t i, float v, float scale) {
+        p[i] = __float2half_rn(v * scale);
+    }
+};
+template <> struct Storage<__hip_bfloat16> {
+    __device__ __forceinline__ static float load(const __hip_bfloat16* p, uint64_t i, float inv) {
+        return __bfloat162float(p[i]) * inv;
+    }
+    __device__ __forceinline__ static void store(__hip_bfloat16* p, uint64_t i, float v, float scale) {
+        p[i] = __float2bfloat16(v * scale);
+    }
+};
+
+// Predicated read: clamped address (always in bounds) then select.
+template <typename T>
+__device__ __forceinline__ float tap(const T* p, int64_t idx, bool ok, float inv) {
+    const float v = Storage<T>::load(p, ok ? (uint64_t)idx : 0ull, inv);
+    return ok ? v : 0.0f;
+}
+
+inline unsigned ceil_div(size_t a, size_t b) { return (unsigned)((a + b - 1) / b); }
+
+// Dispatch a templated launcher on a TOMO_F32/F16/BF16 code.
+#define TOMO_DISPATCH_STORAGE(code, T, ...)                        \
+    switch (code) {                                                \
+        case TOMO_F32: { using T = float; __VA_ARGS__; break; }    \
+        case TOMO_F16: { using T = __half; __VA_ARGS__; break; }   \
+        case TOMO_BF16: { using T = __hip_bfloat16; __VA_ARGS__; break; } \
+        default: return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type"); \
+    }
+
+}  // namespace tomo

@@ -1,0 +1,287 @@
+// kernels_siddon.hip - Siddon projector (clean and reference-compatible
+// forms), the exact transpose of the clean form, and its segment counter.
+//
+// Clean form (default).  Exact chord lengths (mm) through the voxels the ray
+// crosses inside the FOV = inscribed cylinder of radius N delta/2 (the
+// reference Siddon's FOV, SiddonProjector_compute_C_mem_CPU.cpp:99-111) cut by
+// the z extent of the volume.  The source sits on the RS/VI side
+// (-FOD(cos,sin)), so RS, VI and the clean Siddon share one geometry.  Ray
+// setup (FOV entry, first voxel, first plane crossing on each axis, distance
+// between crossings) is FP64; the traversal runs in FP32 relative to the FOV
+// entry with the n-th crossing of an axis at fma(n, inc, first), so no
+// crossing is lost to accumulated rounding; the chord-weighted sum is FP32.
+//
+// Compat form (TOMO_FLAG_SIDDON_COMPAT).  The reference C path
+// (SiddonProjector_compute_C_mem_CPU.cpp:20-276) with its float/double
+// evaluation order and quirks: source on the +FOD side (mirror of RS/VI),
+// first voxel half a voxel behind the entry, last chord not clipped, indices
+// % 256 with unsigned wrap, no z clip.  Defined for 256^3 volumes only.
+#include "device_common.hpp"
+
+namespace tomo {
+namespace {
+
+constexpr int BX = 64, BY = 4;
+
+struct SidRay {
+    int idx[3], step[3];
+    float first[3], inc[3], tmax;
+    bool ok;
+};
+
+__device__ __forceinline__ SidRay sid_setup(const DevGeom& g, int p, int m, int n) {
+    SidRay r;
+    r.ok = false;
+    const double* vw = g.view + (size_t)p * kViewStride;
+    const double ca = vw[0], sa = vw[1], dl = (double)g.delta;
+    const double s = ((double)m - (double)g.cu) * (double)g.du;
+    const double t = ((double)n - (double)g.cv) * (double)g.dv;
+    const double rx = ca * g.fdd - s * sa, ry = sa * g.fdd + s * ca, rz = t;
+    const double L = sqrt((rx * rx + ry * ry) + rz * rz);
+    const double d[3] = {rx / L, ry / L, rz / L};
+    const double S[3] = {-g.fod * ca, -g.fod * sa, 0.0};
+    const double Rm = 0.5 * (double)g.nx * dl;
+    const double a = d[0] * d[0] + d[1] * d[1];
+    if (!(a > 0.0)) return r;
+    const double b = S[0] * d[0] + S[1] * d[1];
+    const double c = (S[0] * S[0] + S[1] * S[1]) - Rm * Rm;
+    const double disc = b * b - a * c;
+    if (!(disc > 0.0)) return r;
+    const double sq = sqrt(disc);
+    double l0 = (-b - sq) / a, l1 = (-b + sq) / a;
+    const double zh = 0.5 * (double)g.nz * dl;
+    if (d[2] != 0.0) {
+        const double ta = -zh / d[2], tb = zh / d[2];
+        l0 = fmax(l0, fmin(ta, tb));
+        l1 = fmin(l1, fmax(ta, tb));
+    }
+    if (!(l1 > l0)) return r;
+    const double cen[3] = {(double)g.cx, (double)g.cy, (double)g.cz};
+    const int nn[3] = {g.nx, g.ny, g.nz};
+#pragma unroll
+    for (int ax = 0; ax < 3; ++ax) {
+        const double pe = (S[ax] + l0 * d[ax]) / dl + cen[ax];
+        double fi = floor(pe + 0.5);
+        if (fi < 0.0) fi = 0.0;
+        if (fi > (double)(nn[ax] - 1)) fi = (double)(nn[ax] - 1);
+        r.idx[ax] = (int)fi;
+        if (d[ax] > 0.0) {
+            r.step[ax] = 1;
+            r.first[ax] = (float)fmax(((fi + 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0);
+            r.inc[ax] = (float)(dl / d[ax]);
+        } else if (d[ax] < 0.0) {
+            r.step[ax] = -1;
+            r.first[ax] = (float)fmax(((fi - 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0);
+            r.inc[ax] = (float)(-dl / d[ax]);
+        } else {
+            r.step[ax] = 0;
+            r.first[ax] = 3.0e38f;
+            r.inc[ax] = 0.0f;
+        }
+    }
+    r.tmax = (float)(l1 - l0);
+    r.ok = true;
+    return r;
+}
+
+// Traversal shared by the forward projector (mode 0), the transpose
+// (mode 1, scatter chord*gval) and the segment counter (mode 2).
+template <int MODE, typename TV>
+__device__ __forceinline__ float sid_trace(const DevGeom& g, const SidRay& r, const TV* __restrict__ vol, float inv,
+                                           float* __restrict__ scatter, float gval, unsigned* nseg) {
+    float tx = r.first[0], ty = r.first[1], tz = r.first[2];
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    int i = r.idx[0], j = r.idx[1], k = r.idx[2];
+    const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
+    float t = 0.0f, acc = 0.0f;
+    int guard = g.nx + g.ny + g.nz + 4;
+    unsigned cnt = 0;
+    while (t < r.tmax && guard-- > 0) {
+        const float tn = fminf(tx, fminf(ty, tz));
+        const float te = fminf(tn, r.tmax);
+        const float chord = te - t;
+        const int64_t o = (int64_t)i + sy * j + sz * k;
+        if (MODE == 0) acc = fmaf(chord, Storage<TV>::load(vol, (uint64_t)o, inv), acc);
+        if (MODE == 1) unsafeAtomicAdd(scatter + o, chord * gval);
+        if (MODE == 2) cnt += chord > 0.0f ? 1u : 0u;
+        t = te;
+        if (tx == tn) { i += r.step[0]; cx += 1.0f; tx = fmaf(cx, r.inc[0], r.first[0]); }
+        if (ty == tn) { j += r.step[1]; cy += 1.0f; ty = fmaf(cy, r.inc[1], r.first[1]); }
+        if (tz == tn) { k += r.step[2]; cz += 1.0f; tz = fmaf(cz, r.inc[2], r.first[2]); }
+        if ((unsigned)i >= (unsigned)g.nx || (unsigned)j >= (unsigned)g.ny || (unsigned)k >= (unsigned)g.nz) break;
+    }
+    if (MODE == 2) *nseg = cnt;
+    return acc;
+}
+
+template <typename TV, typename TS>
+__global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ vol, float vol_inv,
+                                                      TS* __restrict__ sino, float sino_scale) {
+    const int m = blockIdx.x * BX + threadIdx.x;
+    const int n = blockIdx.y * BY + threadIdx.y;
+    const int p = blockIdx.z;
+    if (m >= g.nu || n >= g.nv) return;
+    const SidRay r = sid_setup(g, p, m, n);
+    float v = 0.0f;
+    if (r.ok) v = sid_trace<0>(g, r, vol, vol_inv, nullptr, 0.0f, nullptr);
+    Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), v, sino_scale);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(BX* BY) void siddon_adjoint_kernel(DevGeom g, const TS* __restrict__ sino,
+                                                              float sino_inv, float* __restrict__ vol) {
+    const int m = blockIdx.x * BX + threadIdx.x;
+    const int n = blockIdx.y * BY + threadIdx.y;
+    const int p = blockIdx.z;
+    if (m >= g.nu || n >= g.nv) return;
+    const float gv = Storage<TS>::load(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), sino_inv);
+    if (gv == 0.0f) return;
+    const SidRay r = sid_setup(g, p, m, n);
+    if (r.ok) sid_trace<1, float>(g, r, nullptr, 1.0f, vol, gv, nullptr);
+}
+
+__global__ __launch_bounds__(BX* BY) void siddon_count_kernel(DevGeom g, unsigned long long* total) {
+    const int m = blockIdx.x * BX + threadIdx.x;
+    const int n = blockIdx.y * BY + threadIdx.y;
+    const int p = blockIdx.z;
+    unsigned c = 0;
+    if (m < g.nu && n < g.nv) {
+        const SidRay r = sid_setup(g, p, m, n);
+        if (r.ok) sid_trace<2, float>(g, r, nullptr, 1.0f, nullptr, 0.0f, &c);
+    }
+    unsigned long long cc = c;
+    for (int off = 32; off > 0; off >>= 1) cc += __shfl_down(cc, off, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(total, cc);
+}
+
+// Reference C path, statement for statement in its arithmetic types.
+template <typename TV, typename TS>
+__global__ __launch_bounds__(BX* BY) void siddon_compat_kernel(DevGeom g, const TV* __restrict__ vol, float vol_inv,
+                                                             TS* __restrict__ sino, float sino_scale) {
+    const unsigned long un = blockIdx.x * BX + threadIdx.x;
+    const unsigned long vn = blockIdx.y * BY + threadIdx.y;
+    const int phi = blockIdx.z;
+    const unsigned long nx = g.nx, ny = g.ny, nz = g.nz, nu = g.nu, nv = g.nv;
+    if (un >= nu || vn >= nv) return;
+    const float fdd = (float)g.fdd, fod = (float)g.fod, dl = g.delta;
+    const float xlr = g.cx * dl, ylr = g.cy * dl, zlr = g.cz * dl;
+    const float al = g.view_f[2 * phi], be = g.view_f[2 * phi + 1];
+    const float xs = xlr + fod * al, ys = ylr + fod * be, zs = zlr;
+    const float s = ((float)un - g.cu) * g.du;
+    const float t = ((float)vn - g.cv) * g.dv;
+    const float xd = (xlr - al * (fdd - fod)) - s * be;
+    const float yd = (ylr - be * (fdd - fod)) + s * al;
+    const float zd = zlr + t;
+    const float L = sqrtf((fdd * fdd + s * s) + t * t);
+    const float A = (fdd * fdd + s * s) / (L * L);
+    const float B = (-fod * fdd) / L;
+    const float C = (float)((double)(fod * fod) - 0.25 * (double)(float)nx * (double)(float)nx * (double)dl * (double)dl);
+    const float dlam = B * B - A * C;
+    float ray = 0.0f;
+    if (dlam > 0.0f) {
+        const float lmin = (-B - sqrtf(dlam)) / A;
+        const float lmax = (-B + sqrtf(dlam)) / A;
+        const float P[3] = {(xs + (lmin * (xd - xs) / L)) / dl, (ys + (lmin * (yd - ys) / L)) / dl,
+                            (zs + (lmin * (zd - zs) / L)) / dl};
+        const float S[3] = {xs, ys, zs}, D[3] = {xd, yd, zd};
+        const unsigned long N[3] = {nx, ny, nz};
+        unsigned long idx[3];
+        float lam[3], inc[3];
+        bool up[3];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            up[a] = S[a] <= D[a];
+            const float pm = up[a] ? (float)((double)P[a] - 0.5) : (float)((double)P[a] + 0.5);
+            if (pm < 0.0f) idx[a] = 0;
+            else if (pm > (float)(N[a] - 1)) idx[a] = N[a] - 1;
+            else idx[a] = up[a] ? (unsigned long)floorf(pm) : (unsigned long)ceilf(pm);
+            const float span = D[a] - S[a];
+            if (fabsf(span) > dl) {
+                const double off = up[a] ? 0.5 : -0.5;
+                lam[a] = (float)((double)L * (((double)(float)idx[a] + off) * (double)dl - (double)S[a]) / (double)span);
+            } else {
+                lam[a] = L;
+            }
+            inc[a] = (L / fabsf(span)) * dl;
+        }
+        float lambda = lmin;
+        int it = 0;
+        const int cap = 4 * (g.nx + g.ny + g.nz) + 16;
+        while (lambda <= lmax && it++ < cap) {
+            float lk = lam[0];
+            if (lam[1] < lk) lk = lam[1];
+            if (lam[2] < lk) lk = lam[2];
+            const float chord = lk - lambda;
+            const unsigned long o = (idx[0] % 256) + (idx[1] % 256) * nx + (idx[2] % 256) * nx * ny;
+            const float prod = chord * Storage<TV>::load(vol, o, vol_inv);
+            ray = ray + prod;
+            lambda = lk;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                if (lam[a] <= lk) {
+                    lam[a] = lam[a] + inc[a];
+                    idx[a] = up[a] ? idx[a] + 1 : idx[a] - 1;
+                }
+            }
+        }
+    }
+    Storage<TS>::store(sino, (un % 256) + (vn % 256) * nu + (unsigned long)phi * nu * nv, ray, sino_scale);
+}
+
+}  // namespace
+
+int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStream_t s) {
+    const DevGeom& d = g->dev;
+    if (compat && (d.nx != 256 || d.ny != 256 || d.nz != 256 || d.nu > 256 || d.nv > 256))
+        return fail(TOMO_ERR_UNSUPPORTED,
+                    "compat Siddon is defined for a 256^3 volume and at most 256x256 detector cells "
+                    "(the reference indexes modulo 256)");
+    dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
+    const float inv = 1.0f / vol.scale;
+    TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
+        TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
+            if (compat)
+                hipLaunchKernelGGL((siddon_compat_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p, inv,
+                                   (TS*)sino.p, sino.scale);
+            else
+                hipLaunchKernelGGL((siddon_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p, inv,
+                                   (TS*)sino.p, sino.scale);
+        });
+    });
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+int launch_siddon_adjoint(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
+    if (vol.dtype != TOMO_F32) return fail(TOMO_ERR_UNSUPPORTED, "adjoint output must be F32");
+    const DevGeom& d = g->dev;
+    int rc = launch_fill((float*)vol.p, 0.0f, g->nvox(), s);
+    if (rc) return rc;
+    dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
+    const float inv = 1.0f / sino.scale;
+    TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
+        hipLaunchKernelGGL((siddon_adjoint_kernel<TS>), grid, dim3(BX, BY), 0, s, d, (const TS*)sino.p, inv,
+                           (float*)vol.p);
+    });
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+int count_siddon_segments(tomo_ctx* ctx, const tomo_geom* g, uint64_t* out, hipStream_t s) {
+    const DevGeom& d = g->dev;
+    unsigned long long* dt = nullptr;
+    TOMO_HIP(hipMallocAsync((void**)&dt, sizeof(unsigned long long), s));
+    TOMO_HIP(hipMemsetAsync(dt, 0, sizeof(unsigned long long), s));
+    dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
+    hipLaunchKernelGGL(siddon_count_kernel, grid, dim3(BX, BY), 0, s, d, dt);
+    TOMO_HIP(hipGetLastError());
+    unsigned long long h = 0;
+    TOMO_HIP(hipMemcpyAsync(&h, dt, sizeof(h), hipMemcpyDeviceToHost, s));
+    TOMO_HIP(hipStreamSynchronize(s));
+    TOMO_HIP(hipFreeAsync(dt, s));
+    *out = h;
+    (void)ctx;
+    return TOMO_OK;
+}
+
+}  // namespace tomo

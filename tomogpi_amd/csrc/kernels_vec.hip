@@ -1,0 +1,240 @@
+// kernels_vec.hip - loop glue: element-wise volume / sinogram operations,
+// FP64 reductions, the 7-point Laplacian and the ellipsoid phantoms.
+//
+// Element-wise ops follow Image3D_CPU / Sinogram3D_CPU (C/Image3D_CPU.cu:
+// 108-205, C/Sinogram3D_CPU.cu:311-334): out = a - b, x *= s,
+// y = y + (float)lambda * x with optional clamp at 0 (no fused multiply-add,
+// as the reference host loops).  Squared norms are sum double(x)^2
+// (C/Image3D_CPU.cu:363-390; "L2Norm" without the square root), reduced in a
+// fixed order: per-block partials, then one block.
+#include "device_common.hpp"
+
+namespace tomo {
+namespace {
+
+constexpr int kBlock = 256;
+constexpr int kReduceBlocks = 2048;
+
+__global__ __launch_bounds__(kBlock) void fill_kernel(float* x, float v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) x[i] = v;
+}
+__global__ __launch_bounds__(kBlock) void diff_kernel(const float* a, const float* b, float* o, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        o[i] = a[i] - b[i];
+}
+__global__ __launch_bounds__(kBlock) void scale_kernel(float* x, float v, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        x[i] = x[i] * v;
+}
+__global__ __launch_bounds__(kBlock) void axpy_kernel(float* y, const float* x, float a, int positive, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        float v = y[i] + a * x[i];
+        if (positive && v < 0.0f) v = 0.0f;
+        y[i] = v;
+    }
+}
+
+template <typename TI, typename TO>
+__global__ __launch_bounds__(kBlock) void convert_kernel(const TI* in, float inv, TO* out, float scale, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        Storage<TO>::store(out, i, Storage<TI>::load(in, i, inv), scale);
+}
+
+__device__ __forceinline__ double block_sum(double v) {
+    __shared__ double sh[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) sh[wid] = v;
+    __syncthreads();
+    double r = 0.0;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBlock / 64; ++w) r += sh[w];
+    return r;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void sq_partial_kernel(const T* x, float inv, size_t n, double* partial) {
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const double v = (double)Storage<T>::load(x, i, inv);
+        s += v * v;
+    }
+    const double r = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void dot_partial_kernel(const float* a, const float* b, size_t n, double* partial) {
+    double s = 0.0;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        s += (double)a[i] * (double)b[i];
+    const double r = block_sum(s);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void final_sum_kernel(const double* partial, int count, double* out) {
+    double s = 0.0;
+    for (int i = threadIdx.x; i < count; i += kBlock) s += partial[i];
+    const double r = block_sum(s);
+    if (threadIdx.x == 0) *out = r;
+}
+
+// y = ((row + column) + plane), each pass (2x - previous) - next, zero
+// outside the volume (C/Convolution3D_CPU_GPU.cu:146-148).
+__global__ __launch_bounds__(kBlock) void laplacian_kernel(const float* __restrict__ x, float* __restrict__ y, int nx,
+                                                         int ny, int nz) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    const size_t o = (size_t)i + sy * j + sz * k;
+    const float c = x[o], c2 = c + c;
+    const float xm = i > 0 ? x[o - 1] : 0.0f, xp = i < nx - 1 ? x[o + 1] : 0.0f;
+    const float ym = j > 0 ? x[o - sy] : 0.0f, yp = j < ny - 1 ? x[o + sy] : 0.0f;
+    const float zm = k > 0 ? x[o - sz] : 0.0f, zp = k < nz - 1 ? x[o + sz] : 0.0f;
+    const float h = (c2 - xm) - xp, v = (c2 - ym) - yp, p = (c2 - zm) - zp;
+    y[o] = (h + v) + p;
+}
+
+// Ellipsoid phantoms; 16 doubles per ellipsoid: rotation (9, row-major),
+// centre (3), axis terms (3), density (1).
+//   mode 0 (Shepp-Logan, Data3D_0256/phantom3d.m:76-107): r = normalised
+//     voxel coordinates in [-1,1]; p = M r; q = sum (p_a - c_a)^2 / s_a^2
+//     with axis terms s_a^2;
+//   mode 1 (random ellipsoids): r = voxel offset from the volume centre;
+//     l = R (r - c); q = sum l_a^2 * s_a with axis terms 1/s_a^2.
+// A voxel takes the sum of the densities of the ellipsoids with q <= 1.
+__global__ __launch_bounds__(kBlock) void ellipsoid_kernel(int nx, int ny, int nz, int mode, const double* __restrict__ tab,
+                                                         int count, float* __restrict__ vol) {
+    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int k = blockIdx.z;
+    if (i >= nx || j >= ny) return;
+    const double hx = (nx - 1) / 2.0, hy = (ny - 1) / 2.0, hz = (nz - 1) / 2.0;
+    double acc = 0.0;
+    if (mode == 0) {
+        const double x = ((double)i - hx) / hx, y = ((double)j - hy) / hy, z = ((double)k - hz) / hz;
+        for (int e = 0; e < count; ++e) {
+            const double* t = tab + 16 * e;
+            const double px = (t[0] * x + t[1] * y) + t[2] * z;
+            const double py = (t[3] * x + t[4] * y) + t[5] * z;
+            const double pz = (t[6] * x + t[7] * y) + t[8] * z;
+            const double tx = px - t[9], ty = py - t[10], tz = pz - t[11];
+            const double q = ((tx * tx) / t[12] + (ty * ty) / t[13]) + (tz * tz) / t[14];
+            if (q <= 1.0) acc += t[15];
+        }
+    } else {
+        const double ox = (double)i - hx, oy = (double)j - hy, oz = (double)k - hz;
+        for (int e = 0; e < count; ++e) {
+            const double* t = tab + 16 * e;
+            const double dx = ox - t[9], dy = oy - t[10], dz = oz - t[11];
+            const double l0 = (t[0] * dx + t[1] * dy) + t[2] * dz;
+            const double l1 = (t[3] * dx + t[4] * dy) + t[5] * dz;
+            const double l2 = (t[6] * dx + t[7] * dy) + t[8] * dz;
+            const double q = ((l0 * l0) * t[12] + (l1 * l1) * t[13]) + (l2 * l2) * t[14];
+            if (q <= 1.0) acc += t[15];
+        }
+    }
+    vol[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = (float)acc;
+}
+
+unsigned grid_for(size_t n) {
+    size_t b = (n + kBlock - 1) / kBlock;
+    if (b > 65536) b = 65536;
+    if (b == 0) b = 1;
+    return (unsigned)b;
+}
+
+}  // namespace
+
+int launch_fill(float* x, float v, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_diff(const float* a, const float* b, float* out, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(diff_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, a, b, out, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_scale(float* x, float v, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(scale_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_axpy(float* y, const float* x, float a, int positive, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, y, x, a, positive, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_convert(Buf in, MutBuf out, size_t n, hipStream_t s) {
+    const float inv = 1.0f / in.scale;
+    TOMO_DISPATCH_STORAGE(in.dtype, TI, {
+        TOMO_DISPATCH_STORAGE(out.dtype, TO, {
+            hipLaunchKernelGGL((convert_kernel<TI, TO>), dim3(grid_for(n)), dim3(kBlock), 0, s, (const TI*)in.p, inv,
+                               (TO*)out.p, out.scale, n);
+        });
+    });
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+int launch_laplacian(const float* x, float* y, int nx, int ny, int nz, hipStream_t s) {
+    dim3 grid(ceil_div(nx, 64), ceil_div(ny, 4), nz);
+    hipLaunchKernelGGL(laplacian_kernel, grid, dim3(kBlock), 0, s, x, y, nx, ny, nz);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+static int ensure_partial(tomo_ctx* ctx) {
+    if (!ctx->d_partial) {
+        TOMO_HIP(hipMalloc((void**)&ctx->d_partial, sizeof(double) * (kReduceBlocks + 1)));
+        ctx->partial_cap = kReduceBlocks;
+    }
+    if (!ctx->h_result) TOMO_HIP(hipHostMalloc((void**)&ctx->h_result, sizeof(double)));
+    return TOMO_OK;
+}
+
+static int finish_reduce(tomo_ctx* ctx, int blocks, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(kBlock), 0, s, ctx->d_partial, blocks,
+                       ctx->d_partial + kReduceBlocks);
+    TOMO_HIP(hipGetLastError());
+    TOMO_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_partial + kReduceBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
+    TOMO_HIP(hipStreamSynchronize(s));
+    *out = *ctx->h_result;
+    return TOMO_OK;
+}
+
+int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s) {
+    int rc = ensure_partial(ctx);
+    if (rc) return rc;
+    unsigned blocks = grid_for(n);
+    if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
+    const float inv = 1.0f / x.scale;
+    TOMO_DISPATCH_STORAGE(x.dtype, T, {
+        hipLaunchKernelGGL((sq_partial_kernel<T>), dim3(blocks), dim3(kBlock), 0, s, (const T*)x.p, inv, n,
+                           ctx->d_partial);
+    });
+    TOMO_HIP(hipGetLastError());
+    return finish_reduce(ctx, (int)blocks, out, s);
+}
+
+int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s) {
+    int rc = ensure_partial(ctx);
+    if (rc) return rc;
+    unsigned blocks = grid_for(n);
+    if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, a, b, n, ctx->d_partial);
+    TOMO_HIP(hipGetLastError());
+    return finish_reduce(ctx, (int)blocks, out, s);
+}
+
+int launch_ellipsoids(const tomo_geom* g, int mode, const double* d_table, int count, float* vol, hipStream_t s) {
+    const DevGeom& d = g->dev;
+    dim3 grid(ceil_div(d.nx, 64), ceil_div(d.ny, 4), d.nz);
+    hipLaunchKernelGGL(ellipsoid_kernel, grid, dim3(kBlock), 0, s, d.nx, d.ny, d.nz, mode, d_table, count, vol);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+}  // namespace tomo

@@ -1,0 +1,160 @@
+// kernels_vi.hip - voxel-driven bilinear backprojector (VI) and its exact
+// transpose.
+//
+// Restates K/BackProjector/backprojection3D_kernel_16reg_UM.cuh:23-151 (the
+// reference's device-resident VI kernel): for voxel (i,j,k), x'=i-cx,
+// y'=j-cy, z'=k-cz and view p,
+//   w = delta(cos x' + sin y') + FOD
+//   u = (FDD/du) delta (-sin x' + cos y') / w + cu
+//   v = A z' + cv,   A = (FDD/dv) delta / w
+//   f = delta * sum_p bilinear(g_p, u, v)      (taps outside the detector = 0)
+// The output is overwritten.  As in the reference's "16reg" blocking, a
+// thread owns a column of ZB voxels that share u and A; the per-(column,view)
+// terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
+// and the per-voxel v, bilinear weights and sum in FP32.
+#include "device_common.hpp"
+
+namespace tomo {
+namespace {
+
+constexpr int TX = 16, TY = 16, ZB = 16;
+
+struct ColumnView {
+    float u, A;
+};
+
+__device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double* c, double xp, double yp) {
+    const double w = fma(c[2], xp, fma(c[3], yp, g.fod));
+    const double inv = 1.0 / w;
+    const double un = fma(c[4], xp, c[5] * yp);
+    ColumnView r;
+    r.u = (float)fma(un, inv, (double)g.cu);
+    r.A = (float)(c[6] * inv);
+    return r;
+}
+
+template <typename TS>
+__device__ __forceinline__ float bilinear(const TS* __restrict__ gp, int nu, int nv, float u, float v, float inv) {
+    const float uf = floorf(u), vf = floorf(v);
+    if (!(uf >= -1.0f && uf <= (float)nu && vf >= -1.0f && vf <= (float)nv)) return 0.0f;
+    const int iu = (int)uf, iv = (int)vf;
+    const float fu = u - uf, fv = v - vf;
+    const bool u0 = iu >= 0 && iu < nu, u1 = iu + 1 >= 0 && iu + 1 < nu;
+    const bool v0 = iv >= 0 && iv < nv, v1 = iv + 1 >= 0 && iv + 1 < nv;
+    const int64_t o = (int64_t)iv * nu + iu;
+    const float t00 = tap(gp, o, u0 && v0, inv);
+    const float t01 = tap(gp, o + 1, u1 && v0, inv);
+    const float t10 = tap(gp, o + nu, u0 && v1, inv);
+    const float t11 = tap(gp, o + nu + 1, u1 && v1, inv);
+    const float a = fmaf(fu, t01 - t00, t00);
+    const float b = fmaf(fu, t11 - t10, t10);
+    return fmaf(fv, b - a, a);
+}
+
+template <typename TS, typename TV>
+__global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ sino, float sino_inv,
+                                                  TV* __restrict__ vol, float vol_scale) {
+    const int i = blockIdx.x * TX + (threadIdx.x % TX);
+    const int j = blockIdx.y * TY + (threadIdx.x / TX);
+    const int k0 = blockIdx.z * ZB;
+    if (i >= g.nx || j >= g.ny) return;
+    const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
+    const float zp0 = (float)((double)k0 - (double)g.cz);
+    const float cv = g.cv;
+    float acc[ZB];
+#pragma unroll
+    for (int k = 0; k < ZB; ++k) acc[k] = 0.0f;
+    const size_t plane = (size_t)g.nu * g.nv;
+    for (int p = 0; p < g.nphi; ++p) {
+        const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+        const TS* gp = sino + (size_t)p * plane;
+#pragma unroll
+        for (int k = 0; k < ZB; ++k) {
+            const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+            acc[k] += bilinear(gp, g.nu, g.nv, cvw.u, v, sino_inv);
+        }
+    }
+    const size_t sxy = (size_t)g.nx * g.ny;
+    const size_t base = (size_t)i + (size_t)g.nx * j;
+#pragma unroll
+    for (int k = 0; k < ZB; ++k)
+        if (k0 + k < g.nz) Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, acc[k] * g.delta, vol_scale);
+}
+
+// VI^T: g_p[tap] += delta * w_tap * f(i,j,k) (FP32 atomics, output zeroed by
+// the caller).  Same geometry and weights as vi_kernel.
+template <typename TV>
+__global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV* __restrict__ vol, float vol_inv,
+                                                          float* __restrict__ sino) {
+    const int i = blockIdx.x * TX + (threadIdx.x % TX);
+    const int j = blockIdx.y * TY + (threadIdx.x / TX);
+    const int k0 = blockIdx.z * ZB;
+    if (i >= g.nx || j >= g.ny) return;
+    const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
+    const float zp0 = (float)((double)k0 - (double)g.cz);
+    const size_t sxy = (size_t)g.nx * g.ny;
+    const size_t base = (size_t)i + (size_t)g.nx * j;
+    float fv[ZB];
+#pragma unroll
+    for (int k = 0; k < ZB; ++k)
+        fv[k] = (k0 + k < g.nz) ? Storage<TV>::load(vol, base + (size_t)(k0 + k) * sxy, vol_inv) * g.delta : 0.0f;
+    const size_t plane = (size_t)g.nu * g.nv;
+    const int nu = g.nu, nv = g.nv;
+    for (int p = 0; p < g.nphi; ++p) {
+        const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+        float* gp = sino + (size_t)p * plane;
+        const float uf = floorf(cvw.u);
+        if (!(uf >= -1.0f && uf <= (float)nu)) continue;
+        const int iu = (int)uf;
+        const float fu = cvw.u - uf;
+        const bool u0 = iu >= 0 && iu < nu, u1 = iu + 1 >= 0 && iu + 1 < nu;
+        for (int k = 0; k < ZB; ++k) {
+            if (fv[k] == 0.0f) continue;
+            const float v = fmaf(zp0 + (float)k, cvw.A, g.cv);
+            const float vf = floorf(v);
+            if (!(vf >= -1.0f && vf <= (float)nv)) continue;
+            const int iv = (int)vf;
+            const float fvv = v - vf;
+            const bool v0 = iv >= 0 && iv < nv, v1 = iv + 1 >= 0 && iv + 1 < nv;
+            const int64_t o = (int64_t)iv * nu + iu;
+            const float val = fv[k];
+            if (u0 && v0) unsafeAtomicAdd(gp + o, (1.0f - fu) * (1.0f - fvv) * val);
+            if (u1 && v0) unsafeAtomicAdd(gp + o + 1, fu * (1.0f - fvv) * val);
+            if (u0 && v1) unsafeAtomicAdd(gp + o + nu, (1.0f - fu) * fvv * val);
+            if (u1 && v1) unsafeAtomicAdd(gp + o + nu + 1, fu * fvv * val);
+        }
+    }
+}
+
+}  // namespace
+
+int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
+    const DevGeom& d = g->dev;
+    dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
+    const float inv = 1.0f / sino.scale;
+    TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
+        TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
+            hipLaunchKernelGGL((vi_kernel<TS, TV>), grid, dim3(TX * TY), 0, s, d, (const TS*)sino.p, inv,
+                               (TV*)vol.p, vol.scale);
+        });
+    });
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s) {
+    if (sino.dtype != TOMO_F32) return fail(TOMO_ERR_UNSUPPORTED, "adjoint output must be F32");
+    const DevGeom& d = g->dev;
+    int rc = launch_fill((float*)sino.p, 0.0f, g->nsino(), s);
+    if (rc) return rc;
+    dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
+    const float inv = 1.0f / vol.scale;
+    TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
+        hipLaunchKernelGGL((vi_adjoint_kernel<TV>), grid, dim3(TX * TY), 0, s, d, (const TV*)vol.p, inv,
+                           (float*)sino.p);
+    });
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+}  // namespace tomo

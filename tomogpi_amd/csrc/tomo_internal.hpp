@@ -1,0 +1,117 @@
+// tomo_internal.hpp - internal types of libtomogpi_mi355x.so shared by the
+// host API (api.cpp, loop.cpp, config.cpp) and the kernel files (*.hip).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstddef>
+#include <string>
+#include <vector>
+
+#include "../../include/tomogpi.h"
+
+namespace tomo {
+
+// ---------------------------------------------------------------- errors
+// Thread-local message of the last failure (tomo_last_error()).
+void set_error(const std::string& msg);
+int fail(int status, const std::string& msg);
+int hip_fail(hipError_t e, const char* what);
+
+#define TOMO_HIP(call)                                         \
+    do {                                                       \
+        hipError_t e_ = (call);                                \
+        if (e_ != hipSuccess) return ::tomo::hip_fail(e_, #call); \
+    } while (0)
+
+// -------------------------------------------------------------- geometry
+// Scalars every kernel needs, passed by value.  Types follow the reference
+// (float detector / voxel sizes and centres, double distances).
+struct DevGeom {
+    int32_t nx, ny, nz, nu, nv, nphi;
+    double fod, fdd;
+    float du, dv, cu, cv;
+    float delta, cx, cy, cz;
+    // Per-view table in device memory, kViewStride doubles per view:
+    //   [0] cos phi  [1] sin phi
+    //   [2] delta cos  [3] delta sin  [4] -(delta sin) FDD/du
+    //   [5] (delta cos) FDD/du  [6] delta FDD/dv  [7] unused
+    const double* view;
+    // Reference float tables alpha = (float)cos, beta = (float)sin, 2 per view.
+    const float* view_f;
+};
+constexpr int kViewStride = 8;
+
+}  // namespace tomo
+
+struct tomo_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    // Scratch for reductions: partial sums (double) and the final value.
+    double* d_partial = nullptr;
+    size_t partial_cap = 0;
+    double* h_result = nullptr;  // pinned
+};
+
+struct tomo_geom {
+    tomo_ctx* ctx = nullptr;
+    tomo_geom_desc desc{};
+    tomo_geom_info info{};
+    std::vector<double> phi, view;  // host copies
+    std::vector<float> view_f;
+    double* d_view = nullptr;
+    float* d_view_f = nullptr;
+    tomo::DevGeom dev{};
+    size_t nvox() const { return (size_t)info.nx * info.ny * info.nz; }
+    size_t nsino() const { return (size_t)info.nu * info.nv * info.nphi; }
+};
+
+namespace tomo {
+
+// Derives the geometry exactly as the reference constructors do.
+int derive_geometry(const tomo_geom_desc& d, tomo_geom* g);
+
+// ------------------------------------------------------------- launchers
+// All launchers are stream-ordered and check their launch shape against the
+// buffers they are given.  Element type codes are TOMO_F32/F16/BF16.
+struct Buf {
+    const void* p;
+    int dtype;
+    float scale;
+};
+struct MutBuf {
+    void* p;
+    int dtype;
+    float scale;
+};
+
+int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s);
+int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s);
+int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s);
+int launch_rs_adjoint(const tomo_geom* g, int cube, Buf sino, MutBuf vol, hipStream_t s);
+int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStream_t s);
+int launch_siddon_adjoint(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s);
+// Work counters (roofline accounting): sum of L_r (RS) and number of
+// positive chords (Siddon).  Synchronous.
+int count_rs_samples(tomo_ctx* ctx, const tomo_geom* g, int cube, uint64_t* out, hipStream_t s);
+int count_siddon_segments(tomo_ctx* ctx, const tomo_geom* g, uint64_t* out, hipStream_t s);
+
+int launch_fill(float* x, float v, size_t n, hipStream_t s);
+int launch_diff(const float* a, const float* b, float* out, size_t n, hipStream_t s);
+int launch_scale(float* x, float v, size_t n, hipStream_t s);
+int launch_axpy(float* y, const float* x, float a, int positive, size_t n, hipStream_t s);
+int launch_laplacian(const float* x, float* y, int nx, int ny, int nz, hipStream_t s);
+int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s);
+int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s);
+int launch_convert(Buf in, MutBuf out, size_t n, hipStream_t s);
+
+// Ellipsoid phantoms.  mode 0: Shepp-Logan form (normalised coordinates,
+// rotation then offset, division by squared semi-axes); mode 1: random
+// ellipsoids (voxel offsets, offset then rotation, inverse squared axes).
+// table: 16 doubles per ellipsoid (9 rotation, 3 centre, 3 axis terms, 1
+// density).
+int launch_ellipsoids(const tomo_geom* g, int mode, const double* d_table, int count, float* vol,
+                      hipStream_t s);
+
+}  // namespace tomo
