@@ -73,7 +73,7 @@ uint64_t og_rs_count(const og_geom* g, int cube);       /* N_samp = sum L_r */
 void     og_siddon_f32(const og_geom* g, const float* vol, float* sino);
 void     og_siddon_f64(const og_geom* g, const double* vol, double* sino);
 void     og_siddon_T_f64(const og_geom* g, const double* sino, double* vol);
-uint64_t og_siddon_count(const og_geom* g);           /* N_seg, chord > 0 */
+uint64_t og_siddon_count(const og_geom* g);   /* N_seg: chords > 0, FP32 recipe */
 
 /* Reference-compatible Siddon (C/SiddonProjector_compute_C_mem_CPU.cpp:
  * 20-276): float traversal, quirks kept.  Requires nx=ny=nz=256 and

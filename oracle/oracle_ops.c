@@ -180,6 +180,7 @@ void og_siddon_T_f64(const og_geom* g, const double* sino, double* vol) {
     free(priv);
 }
 
+/* Positive chords of the FP32-recipe traversal (the traversal the kernels run). */
 uint64_t og_siddon_count(const og_geom* g) {
     uint64_t total = 0;
 #pragma omp parallel for collapse(2) reduction(+ : total) schedule(dynamic, 4)
@@ -188,7 +189,7 @@ uint64_t og_siddon_count(const og_geom* g) {
             for (int m = 0; m < g->nu; ++m) {
                 og_sid_ray r;
                 uint64_t c = 0;
-                if (og_sid_setup(g, p, m, n, &r)) sid_trace_f64(g, &r, NULL, NULL, 0.0, &c);
+                if (og_sid_setup(g, p, m, n, &r)) sid_trace_f32(g, &r, NULL, NULL, 0.0f, &c);
                 total += c;
             }
     return total;
