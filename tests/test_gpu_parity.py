@@ -41,6 +41,8 @@ def rel_linf(a, b):
 # partial angular range, non-zero start angle).
 GEOMS = {
     "cube48": scaled_geometry(48, 24),
+    # full waves of 64 detector columns, views on both sides of 45 degrees
+    "cube128": scaled_geometry(128, 60),
     "ragged": Geometry(fod=230.0, fdd=470.0, nphi=37, su=61 * 1.7, sv=29 * 2.3, nu=61, nv=29, sx=41 * 0.9,
                        sy=35 * 0.9, sz=23 * 0.9, nx=41, ny=35, nz=23, num_start=1.0, den_start=3.0,
                        num_stop=5.0, den_stop=4.0),

@@ -114,17 +114,177 @@ __device__ __forceinline__ float sid_trace(const DevGeom& g, const SidRay& r, co
     return acc;
 }
 
+// Wave-wide minimum / maximum of an int (wave64 butterfly).
+__device__ __forceinline__ int wave_min(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+__device__ __forceinline__ int wave_max(int v) {
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Per-lane traversal state of the clean Siddon projector: next crossing,
+// crossings taken and voxel index on each axis, linear index of the voxel in
+// the layout being read, and the running sum.
+struct SidState {
+    float tn[3], cnt[3];
+    int idx[3];
+    unsigned lin;
+    unsigned dlin[3];
+    float t, acc;
+    int guard;
+    bool live;
+};
+
+// One step of the traversal on axis AX (next crossing recomputed from the
+// crossing count, linear index moved by one voxel).
+template <int AX>
+__device__ __forceinline__ void sid_advance(SidState& s, const SidRay& r) {
+    s.idx[AX] += r.step[AX];
+    s.cnt[AX] += 1.0f;
+    s.tn[AX] = fmaf(s.cnt[AX], r.inc[AX], r.first[AX]);
+    s.lin += s.dlin[AX];
+}
+
+// General step (the operations of sid_trace<0>, in the same order).
+template <typename TV>
+__device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const TV* __restrict__ vol, float inv,
+                                         int nx, int ny, int nz) {
+    const float tn = fminf(s.tn[0], fminf(s.tn[1], s.tn[2]));
+    const float te = fminf(tn, r.tmax);
+    s.acc = fmaf(te - s.t, Storage<TV>::load(vol, s.lin, inv), s.acc);
+    s.t = te;
+    if (s.tn[0] == tn) sid_advance<0>(s, r);
+    if (s.tn[1] == tn) sid_advance<1>(s, r);
+    if (s.tn[2] == tn) sid_advance<2>(s, r);
+    --s.guard;
+    s.live = s.t < r.tmax && s.guard > 0 && (unsigned)s.idx[0] < (unsigned)nx && (unsigned)s.idx[1] < (unsigned)ny &&
+             (unsigned)s.idx[2] < (unsigned)nz;
+}
+
+// Segments of slab I along axis A (0 or 1; B is the other in-plane axis).
+// The next A crossing is the slab end; B and z crossings that come first are
+// interior segments.  Every segment is the one sid_step would produce: the
+// interior branch runs only while min(tB, tz) < tA, so its minimum is the
+// minimum of all three, and the slab-end segment only when min(tB, tz) >= tA.
+template <int A, typename TV>
+__device__ __forceinline__ void sid_slab(SidState& s, const SidRay& r, const TV* __restrict__ vol, float inv,
+                                         int nx, int ny, int nz) {
+    constexpr int B = 1 - A;
+    const int nB = B == 0 ? nx : ny;
+    for (int it = 0; it < 8; ++it) {  // at most one B and one z crossing per slab when A dominates
+        const float tn = fminf(s.tn[B], s.tn[2]);
+        if (!(tn < s.tn[A])) break;
+        const float te = fminf(tn, r.tmax);
+        s.acc = fmaf(te - s.t, Storage<TV>::load(vol, s.lin, inv), s.acc);
+        s.t = te;
+        if (s.tn[B] == tn) sid_advance<B>(s, r);
+        if (s.tn[2] == tn) sid_advance<2>(s, r);
+        --s.guard;
+        s.live = s.t < r.tmax && s.guard > 0 && (unsigned)s.idx[B] < (unsigned)nB && (unsigned)s.idx[2] < (unsigned)nz;
+        if (!s.live) return;
+    }
+    const float tn = s.tn[A];
+    const float te = fminf(tn, r.tmax);
+    s.acc = fmaf(te - s.t, Storage<TV>::load(vol, s.lin, inv), s.acc);
+    s.t = te;
+    sid_advance<A>(s, r);
+    if (s.tn[B] == tn) sid_advance<B>(s, r);
+    if (s.tn[2] == tn) sid_advance<2>(s, r);
+    --s.guard;
+    s.live = s.t < r.tmax && s.guard > 0 && (unsigned)s.idx[0] < (unsigned)nx && (unsigned)s.idx[1] < (unsigned)ny &&
+             (unsigned)s.idx[2] < (unsigned)nz;
+}
+
+// Wave walks slabs I, I+sA, ... of axis A together; a lane takes part while
+// its own voxel index on A equals I.
+template <int A, typename TV>
+__device__ __forceinline__ void sid_walk(SidState& s, const SidRay& r, int sA, const TV* __restrict__ vol, float inv,
+                                         int nx, int ny, int nz) {
+    const int nA = A == 0 ? nx : ny;
+    int I = sA > 0 ? wave_min(s.live ? s.idx[A] : 0x7fffffff) : wave_max(s.live ? s.idx[A] : -0x7fffffff);
+    for (int outer = 0; outer <= nA + 1 && __any(s.live); ++outer) {
+        if (s.live && s.idx[A] == I) sid_slab<A>(s, r, vol, inv, nx, ny, nz);
+        I += sA;
+    }
+}
+
+// Forward clean Siddon.  Each lane's sum is the sum of sid_trace<0> (same
+// segments, same operations, same order), so results are bit-identical to
+// the per-ray traversal.  What changes is the schedule and the memory
+// layout: when every ray of the wave has the same dominant in-plane axis A
+// (|d_A| >= the other two components) and the same direction along it, the
+// wave walks the slabs of A together.  The 64 rays of a wave are adjacent
+// detector columns, spread across the other in-plane axis, so at a given slab
+// they read voxels that are contiguous when that other axis varies fastest:
+// x-fastest (the caller's layout) for y-slabs, a y-fastest copy for x-slabs.
+// Waves that mix axes (views near 45 degrees) run the per-lane traversal on
+// the x-fastest volume.
+// Grid: x = detector column group, y = view, z = detector row group.  Blocks
+// are dispatched x fastest and spread over the XCDs round robin, so the
+// chip works through every view of one row band before the next band: the
+// rays of a row band stay in a thin z band of the volume, which the L2 and
+// the Infinity Cache hold, instead of streaming the whole volume from HBM
+// once per view.
 template <typename TV, typename TS>
-__global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ vol, float vol_inv,
+__global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ vol_x,
+                                                      const TV* __restrict__ vol_y, float vol_inv,
                                                       TS* __restrict__ sino, float sino_scale) {
     const int m = blockIdx.x * BX + threadIdx.x;
-    const int n = blockIdx.y * BY + threadIdx.y;
-    const int p = blockIdx.z;
-    if (m >= g.nu || n >= g.nv) return;
-    const SidRay r = sid_setup(g, p, m, n);
-    float v = 0.0f;
-    if (r.ok) v = sid_trace<0>(g, r, vol, vol_inv, nullptr, 0.0f, nullptr);
-    Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), v, sino_scale);
+    const int n = blockIdx.z * BY + threadIdx.y;
+    const int p = blockIdx.y;
+    const bool inside = m < g.nu && n < g.nv;
+    SidRay r;
+    r.ok = false;
+    if (inside) r = sid_setup(g, p, m, n);
+    const bool ok = r.ok && r.tmax > 0.0f;
+    // Slab code of this ray: 0/1 = x axis up/down, 2/3 = y axis up/down when
+    // that axis dominates all three components (inc = delta/|d| smallest).
+    int code = -1;
+    if (ok) {
+        const float ax = r.step[0] ? r.inc[0] : 3.0e38f, ay = r.step[1] ? r.inc[1] : 3.0e38f;
+        const float az = r.step[2] ? r.inc[2] : 3.0e38f;
+        if (r.step[0] != 0 && ax <= ay && ax <= az) code = r.step[0] > 0 ? 0 : 1;
+        else if (r.step[1] != 0 && ay <= ax && ay <= az) code = r.step[1] > 0 ? 2 : 3;
+    }
+    const unsigned long long okmask = __ballot(ok);
+    int mode = -1;
+    if (okmask != 0ull) {
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (__ballot(ok && code == c) == okmask) mode = c;
+    }
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const bool yfast = mode == 0 || mode == 1;
+    const TV* vol = yfast ? vol_y : vol_x;
+    const unsigned s0 = yfast ? (unsigned)ny : 1u, s1 = yfast ? 1u : (unsigned)nx;
+    const unsigned s2 = (unsigned)nx * (unsigned)ny;
+    SidState s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.tn[a] = r.first[a];
+        s.cnt[a] = 0.0f;
+        s.idx[a] = r.idx[a];
+    }
+    // Element counts are < 2^32 (checked when the geometry is built): the
+    // linear index and its signed steps wrap correctly in 32-bit unsigned.
+    s.lin = ok ? (unsigned)r.idx[0] * s0 + (unsigned)r.idx[1] * s1 + (unsigned)r.idx[2] * s2 : 0u;
+    s.dlin[0] = (unsigned)r.step[0] * s0;
+    s.dlin[1] = (unsigned)r.step[1] * s1;
+    s.dlin[2] = (unsigned)r.step[2] * s2;
+    s.t = 0.0f;
+    s.acc = 0.0f;
+    s.guard = nx + ny + nz + 4;
+    s.live = ok;
+    if (mode < 0) {
+        while (s.live) sid_step(s, r, vol, vol_inv, nx, ny, nz);
+    } else if (mode < 2) {
+        sid_walk<0>(s, r, mode == 0 ? 1 : -1, vol, vol_inv, nx, ny, nz);
+    } else {
+        sid_walk<1>(s, r, mode == 2 ? 1 : -1, vol, vol_inv, nx, ny, nz);
+    }
+    if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), s.acc, sino_scale);
 }
 
 template <typename TS>
@@ -238,17 +398,34 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
                     "(the reference indexes modulo 256)");
     dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
     const float inv = 1.0f / vol.scale;
-    TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
-        TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-            if (compat)
+    if (compat) {
+        TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
+            TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 hipLaunchKernelGGL((siddon_compat_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p, inv,
                                    (TS*)sino.p, sino.scale);
-            else
-                hipLaunchKernelGGL((siddon_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p, inv,
-                                   (TS*)sino.p, sino.scale);
+            });
         });
-    });
-    TOMO_HIP(hipGetLastError());
+        TOMO_HIP(hipGetLastError());
+        return TOMO_OK;
+    }
+    // y-fastest copy of the volume for the x-slab waves (stream-ordered pool
+    // allocation, released after the projection on the same stream).
+    const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
+    void* vy = nullptr;
+    TOMO_HIP(hipMallocAsync(&vy, g->nvox() * es, s));
+    hipError_t le = launch_transpose_xy(vol.p, vy, es, d.nx, d.ny, d.nz, s);
+    if (le == hipSuccess) {
+        const dim3 sgrid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
+        TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
+            TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
+                hipLaunchKernelGGL((siddon_kernel<TV, TS>), sgrid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
+                                   (const TV*)vy, inv, (TS*)sino.p, sino.scale);
+            });
+        });
+        le = hipGetLastError();
+    }
+    TOMO_HIP(hipFreeAsync(vy, s));
+    TOMO_HIP(le);
     return TOMO_OK;
 }
 

@@ -138,6 +138,26 @@ __global__ __launch_bounds__(kBlock) void ellipsoid_kernel(int nx, int ny, int n
     vol[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = (float)acc;
 }
 
+// out[j + ny*(i + nx*k)] = in[i + nx*(j + ny*k)] through 32x32 LDS tiles.
+template <typename T>
+__global__ __launch_bounds__(256) void transpose_xy_kernel(const T* __restrict__ in, T* __restrict__ out, int nx,
+                                                         int ny) {
+    __shared__ T tile[32][33];
+    const size_t k = blockIdx.z;
+    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const size_t plane = (size_t)nx * ny;
+    for (int r = ty; r < 32; r += 8) {
+        const int i = i0 + tx, j = j0 + r;
+        if (i < nx && j < ny) tile[r][tx] = in[k * plane + (size_t)j * nx + i];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int j = j0 + tx, i = i0 + r;
+        if (i < nx && j < ny) out[k * plane + (size_t)i * ny + j] = tile[tx][r];
+    }
+}
+
 unsigned grid_for(size_t n) {
     size_t b = (n + kBlock - 1) / kBlock;
     if (b > 65536) b = 65536;
@@ -146,6 +166,17 @@ unsigned grid_for(size_t n) {
 }
 
 }  // namespace
+
+hipError_t launch_transpose_xy(const void* in, void* out, size_t elem_size, int nx, int ny, int nz, hipStream_t s) {
+    const dim3 grid(ceil_div(nx, 32), ceil_div(ny, 32), nz);
+    if (elem_size == 4)
+        hipLaunchKernelGGL((transpose_xy_kernel<uint32_t>), grid, dim3(256), 0, s, (const uint32_t*)in, (uint32_t*)out,
+                           nx, ny);
+    else
+        hipLaunchKernelGGL((transpose_xy_kernel<uint16_t>), grid, dim3(256), 0, s, (const uint16_t*)in, (uint16_t*)out,
+                           nx, ny);
+    return hipGetLastError();
+}
 
 int launch_fill(float* x, float v, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(fill_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, x, v, n);
