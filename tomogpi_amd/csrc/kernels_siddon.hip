@@ -29,6 +29,12 @@ struct SidRay {
     bool ok;
 };
 
+// FP64 ray setup.  tmax is the FOV chord, shortened to the first crossing
+// that takes a voxel index out of the grid (the n-th crossing of axis a is
+// at fma(n, inc_a, first_a), the value the traversal computes).  A traversal
+// that stops at that crossing takes exactly the segments of one that steps
+// out of the grid there and then stops, so no index range test is needed in
+// the loops below.
 __device__ __forceinline__ SidRay sid_setup(const DevGeom& g, int p, int m, int n) {
     SidRay r;
     r.ok = false;
@@ -58,6 +64,7 @@ __device__ __forceinline__ SidRay sid_setup(const DevGeom& g, int p, int m, int 
     if (!(l1 > l0)) return r;
     const double cen[3] = {(double)g.cx, (double)g.cy, (double)g.cz};
     const int nn[3] = {g.nx, g.ny, g.nz};
+    float tmax = (float)(l1 - l0);
 #pragma unroll
     for (int ax = 0; ax < 3; ++ax) {
         const double pe = (S[ax] + l0 * d[ax]) / dl + cen[ax];
@@ -69,31 +76,35 @@ __device__ __forceinline__ SidRay sid_setup(const DevGeom& g, int p, int m, int 
             r.step[ax] = 1;
             r.first[ax] = (float)fmax(((fi + 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0);
             r.inc[ax] = (float)(dl / d[ax]);
+            tmax = fminf(tmax, fmaf((float)(nn[ax] - 1 - r.idx[ax]), r.inc[ax], r.first[ax]));
         } else if (d[ax] < 0.0) {
             r.step[ax] = -1;
             r.first[ax] = (float)fmax(((fi - 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0);
             r.inc[ax] = (float)(-dl / d[ax]);
+            tmax = fminf(tmax, fmaf((float)r.idx[ax], r.inc[ax], r.first[ax]));
         } else {
             r.step[ax] = 0;
             r.first[ax] = 3.0e38f;
             r.inc[ax] = 0.0f;
         }
     }
-    r.tmax = (float)(l1 - l0);
+    r.tmax = tmax;
     r.ok = true;
     return r;
 }
 
-// Traversal shared by the forward projector (mode 0), the transpose
-// (mode 1, scatter chord*gval) and the segment counter (mode 2).
-template <int MODE, typename TV>
-__device__ __forceinline__ float sid_trace(const DevGeom& g, const SidRay& r, const TV* __restrict__ vol, float inv,
-                                           float* __restrict__ scatter, float gval, unsigned* nseg) {
+// Per-ray traversal used by the transpose (MODE 1, scatter chord*gval) and
+// the segment counter (MODE 2): t from 0 to tmax; the next segment ends at
+// the nearest crossing; ties step every tied axis.  The index range test is
+// kept as a second guard for the scatter addresses.
+template <int MODE>
+__device__ __forceinline__ void sid_trace(const DevGeom& g, const SidRay& r, float* __restrict__ scatter, float gval,
+                                          unsigned* nseg) {
     float tx = r.first[0], ty = r.first[1], tz = r.first[2];
     float cx = 0.0f, cy = 0.0f, cz = 0.0f;
     int i = r.idx[0], j = r.idx[1], k = r.idx[2];
     const int64_t sy = g.nx, sz = (int64_t)g.nx * g.ny;
-    float t = 0.0f, acc = 0.0f;
+    float t = 0.0f;
     int guard = g.nx + g.ny + g.nz + 4;
     unsigned cnt = 0;
     while (t < r.tmax && guard-- > 0) {
@@ -101,7 +112,6 @@ __device__ __forceinline__ float sid_trace(const DevGeom& g, const SidRay& r, co
         const float te = fminf(tn, r.tmax);
         const float chord = te - t;
         const int64_t o = (int64_t)i + sy * j + sz * k;
-        if (MODE == 0) acc = fmaf(chord, Storage<TV>::load(vol, (uint64_t)o, inv), acc);
         if (MODE == 1) unsafeAtomicAdd(scatter + o, chord * gval);
         if (MODE == 2) cnt += chord > 0.0f ? 1u : 0u;
         t = te;
@@ -111,7 +121,6 @@ __device__ __forceinline__ float sid_trace(const DevGeom& g, const SidRay& r, co
         if ((unsigned)i >= (unsigned)g.nx || (unsigned)j >= (unsigned)g.ny || (unsigned)k >= (unsigned)g.nz) break;
     }
     if (MODE == 2) *nseg = cnt;
-    return acc;
 }
 
 // Wave-wide minimum / maximum of an int (wave64 butterfly).
@@ -124,112 +133,149 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
-// Per-lane traversal state of the clean Siddon projector: next crossing,
-// crossings taken and voxel index on each axis, linear index of the voxel in
-// the layout being read, and the running sum.
+// Volume reads by linear element index.  BUF: a buffer resource with a
+// 32-bit byte offset (volumes below 4 GiB); an offset past the end reads 0.
+// Otherwise a plain global load.
+template <typename TV, bool BUF> struct VolReader;
+template <typename TV> struct VolReader<TV, false> {
+    const TV* p;
+    float inv;
+    __device__ __forceinline__ float operator()(unsigned lin) const { return Storage<TV>::load(p, lin, inv); }
+};
+template <> struct VolReader<float, true> {
+    __amdgpu_buffer_rsrc_t r;
+    float inv;
+    __device__ __forceinline__ float operator()(unsigned lin) const {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, lin * 4u, 0, 0));
+    }
+};
+template <> struct VolReader<__half, true> {
+    __amdgpu_buffer_rsrc_t r;
+    float inv;
+    __device__ __forceinline__ float operator()(unsigned lin) const {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, lin * 2u, 0, 0);
+        return __half2float(__builtin_bit_cast(__half, b)) * inv;
+    }
+};
+template <> struct VolReader<__hip_bfloat16, true> {
+    __amdgpu_buffer_rsrc_t r;
+    float inv;
+    __device__ __forceinline__ float operator()(unsigned lin) const {
+        const unsigned short b = __builtin_amdgcn_raw_buffer_load_b16(r, lin * 2u, 0, 0);
+        return __uint_as_float((unsigned)b << 16) * inv;
+    }
+};
+
+// Descriptor word 3 for raw (untyped) buffer access on gfx9-family parts.
+constexpr int kBufferWord3 = 0x00020000;
+
+template <typename TV, bool BUF>
+__device__ __forceinline__ VolReader<TV, BUF> make_reader(const TV* p, unsigned bytes, float inv) {
+    VolReader<TV, BUF> v;
+    if constexpr (BUF) {
+        v.r = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, kBufferWord3);
+    } else {
+        v.p = p;
+        (void)bytes;
+    }
+    v.inv = inv;
+    return v;
+}
+
+// Per-lane state of the forward traversal: next crossing and crossings taken
+// per axis, the slab-axis voxel index, the linear index of the current voxel
+// in the layout being read, the running sum.
 struct SidState {
     float tn[3], cnt[3];
-    int idx[3];
+    int iA;
     unsigned lin;
     unsigned dlin[3];
     float t, acc;
-    int guard;
     bool live;
 };
 
-// One step of the traversal on axis AX (next crossing recomputed from the
-// crossing count, linear index moved by one voxel).
 template <int AX>
 __device__ __forceinline__ void sid_advance(SidState& s, const SidRay& r) {
-    s.idx[AX] += r.step[AX];
     s.cnt[AX] += 1.0f;
     s.tn[AX] = fmaf(s.cnt[AX], r.inc[AX], r.first[AX]);
     s.lin += s.dlin[AX];
 }
 
-// General step (the operations of sid_trace<0>, in the same order).
-template <typename TV>
-__device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const TV* __restrict__ vol, float inv,
-                                         int nx, int ny, int nz) {
+// General step (waves without a common slab axis).
+template <typename RD>
+__device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD& rd) {
     const float tn = fminf(s.tn[0], fminf(s.tn[1], s.tn[2]));
     const float te = fminf(tn, r.tmax);
-    s.acc = fmaf(te - s.t, Storage<TV>::load(vol, s.lin, inv), s.acc);
+    s.acc = fmaf(te - s.t, rd(s.lin), s.acc);
     s.t = te;
     if (s.tn[0] == tn) sid_advance<0>(s, r);
     if (s.tn[1] == tn) sid_advance<1>(s, r);
     if (s.tn[2] == tn) sid_advance<2>(s, r);
-    --s.guard;
-    s.live = s.t < r.tmax && s.guard > 0 && (unsigned)s.idx[0] < (unsigned)nx && (unsigned)s.idx[1] < (unsigned)ny &&
-             (unsigned)s.idx[2] < (unsigned)nz;
+    s.live = s.t < r.tmax;
 }
 
 // Segments of slab I along axis A (0 or 1; B is the other in-plane axis).
 // The next A crossing is the slab end; B and z crossings that come first are
-// interior segments.  Every segment is the one sid_step would produce: the
-// interior branch runs only while min(tB, tz) < tA, so its minimum is the
+// interior segments.  Every segment is the one the general step would take:
+// the interior branch runs only while min(tB, tz) < tA, so its minimum is the
 // minimum of all three, and the slab-end segment only when min(tB, tz) >= tA.
-template <int A, typename TV>
-__device__ __forceinline__ void sid_slab(SidState& s, const SidRay& r, const TV* __restrict__ vol, float inv,
-                                         int nx, int ny, int nz) {
+template <int A, typename RD>
+__device__ __forceinline__ void sid_slab(SidState& s, const SidRay& r, const RD& rd, int sA) {
     constexpr int B = 1 - A;
-    const int nB = B == 0 ? nx : ny;
     for (int it = 0; it < 8; ++it) {  // at most one B and one z crossing per slab when A dominates
         const float tn = fminf(s.tn[B], s.tn[2]);
         if (!(tn < s.tn[A])) break;
         const float te = fminf(tn, r.tmax);
-        s.acc = fmaf(te - s.t, Storage<TV>::load(vol, s.lin, inv), s.acc);
+        s.acc = fmaf(te - s.t, rd(s.lin), s.acc);
         s.t = te;
         if (s.tn[B] == tn) sid_advance<B>(s, r);
         if (s.tn[2] == tn) sid_advance<2>(s, r);
-        --s.guard;
-        s.live = s.t < r.tmax && s.guard > 0 && (unsigned)s.idx[B] < (unsigned)nB && (unsigned)s.idx[2] < (unsigned)nz;
-        if (!s.live) return;
+        if (!(s.t < r.tmax)) {
+            s.live = false;
+            return;
+        }
     }
     const float tn = s.tn[A];
     const float te = fminf(tn, r.tmax);
-    s.acc = fmaf(te - s.t, Storage<TV>::load(vol, s.lin, inv), s.acc);
+    s.acc = fmaf(te - s.t, rd(s.lin), s.acc);
     s.t = te;
+    s.iA += sA;
     sid_advance<A>(s, r);
     if (s.tn[B] == tn) sid_advance<B>(s, r);
     if (s.tn[2] == tn) sid_advance<2>(s, r);
-    --s.guard;
-    s.live = s.t < r.tmax && s.guard > 0 && (unsigned)s.idx[0] < (unsigned)nx && (unsigned)s.idx[1] < (unsigned)ny &&
-             (unsigned)s.idx[2] < (unsigned)nz;
+    s.live = s.t < r.tmax;
 }
 
-// Wave walks slabs I, I+sA, ... of axis A together; a lane takes part while
-// its own voxel index on A equals I.
-template <int A, typename TV>
-__device__ __forceinline__ void sid_walk(SidState& s, const SidRay& r, int sA, const TV* __restrict__ vol, float inv,
-                                         int nx, int ny, int nz) {
-    const int nA = A == 0 ? nx : ny;
-    int I = sA > 0 ? wave_min(s.live ? s.idx[A] : 0x7fffffff) : wave_max(s.live ? s.idx[A] : -0x7fffffff);
+// The wave walks slabs I, I+sA, ... of axis A together; a lane takes part
+// while its own slab index equals I.  At most nA+1 outer iterations.
+template <int A, typename RD>
+__device__ __forceinline__ void sid_walk(SidState& s, const SidRay& r, int sA, int nA, const RD& rd) {
+    int I = sA > 0 ? wave_min(s.live ? s.iA : 0x7fffffff) : wave_max(s.live ? s.iA : -0x7fffffff);
     for (int outer = 0; outer <= nA + 1 && __any(s.live); ++outer) {
-        if (s.live && s.idx[A] == I) sid_slab<A>(s, r, vol, inv, nx, ny, nz);
+        if (s.live && s.iA == I) sid_slab<A>(s, r, rd, sA);
         I += sA;
     }
 }
 
-// Forward clean Siddon.  Each lane's sum is the sum of sid_trace<0> (same
-// segments, same operations, same order), so results are bit-identical to
-// the per-ray traversal.  What changes is the schedule and the memory
-// layout: when every ray of the wave has the same dominant in-plane axis A
-// (|d_A| >= the other two components) and the same direction along it, the
-// wave walks the slabs of A together.  The 64 rays of a wave are adjacent
-// detector columns, spread across the other in-plane axis, so at a given slab
-// they read voxels that are contiguous when that other axis varies fastest:
-// x-fastest (the caller's layout) for y-slabs, a y-fastest copy for x-slabs.
-// Waves that mix axes (views near 45 degrees) run the per-lane traversal on
-// the x-fastest volume.
+// Forward clean Siddon.  Each lane takes the segments of the per-ray
+// traversal (same operations, same order), so results are bit-identical to
+// it.  What changes is the schedule and the memory layout: when every ray of
+// the wave has the same dominant in-plane axis A (|d_A| >= the other two
+// components) and the same direction along it, the wave walks the slabs of A
+// together.  The 64 rays of a wave are adjacent detector columns, spread
+// across the other in-plane axis, so at a given slab they read voxels that
+// are contiguous when that other axis varies fastest: x-fastest (the caller's
+// layout) for y-slabs, a y-fastest copy for x-slabs.  Waves that mix axes
+// (views near 45 degrees) step per lane on the x-fastest volume.
 // Grid: x = detector column group, y = view, z = detector row group.  Blocks
-// are dispatched x fastest and spread over the XCDs round robin, so the
-// chip works through every view of one row band before the next band: the
-// rays of a row band stay in a thin z band of the volume, which the L2 and
-// the Infinity Cache hold, instead of streaming the whole volume from HBM
-// once per view.
-template <typename TV, typename TS>
+// are dispatched x fastest and spread over the XCDs round robin, so the chip
+// works through every view of one row band before the next band: the rays of
+// a row band stay in a thin z band of the volume, which the L2 and the
+// Infinity Cache hold, instead of streaming the whole volume from HBM once
+// per view.
+template <typename TV, typename TS, bool BUF>
 __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ vol_x,
-                                                      const TV* __restrict__ vol_y, float vol_inv,
+                                                      const TV* __restrict__ vol_y, unsigned vol_bytes, float vol_inv,
                                                       TS* __restrict__ sino, float sino_scale) {
     const int m = blockIdx.x * BX + threadIdx.x;
     const int n = blockIdx.z * BY + threadIdx.y;
@@ -255,9 +301,9 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
         for (int c = 0; c < 4; ++c)
             if (__ballot(ok && code == c) == okmask) mode = c;
     }
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nx = g.nx, ny = g.ny;
     const bool yfast = mode == 0 || mode == 1;
-    const TV* vol = yfast ? vol_y : vol_x;
+    const auto rd = make_reader<TV, BUF>(yfast ? vol_y : vol_x, vol_bytes, vol_inv);
     const unsigned s0 = yfast ? (unsigned)ny : 1u, s1 = yfast ? 1u : (unsigned)nx;
     const unsigned s2 = (unsigned)nx * (unsigned)ny;
     SidState s;
@@ -265,7 +311,6 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
     for (int a = 0; a < 3; ++a) {
         s.tn[a] = r.first[a];
         s.cnt[a] = 0.0f;
-        s.idx[a] = r.idx[a];
     }
     // Element counts are < 2^32 (checked when the geometry is built): the
     // linear index and its signed steps wrap correctly in 32-bit unsigned.
@@ -275,14 +320,18 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
     s.dlin[2] = (unsigned)r.step[2] * s2;
     s.t = 0.0f;
     s.acc = 0.0f;
-    s.guard = nx + ny + nz + 4;
     s.live = ok;
     if (mode < 0) {
-        while (s.live) sid_step(s, r, vol, vol_inv, nx, ny, nz);
+        // Every step moves one axis toward the grid exit that bounds tmax;
+        // the cap only bounds the loop.
+        int guard = g.nx + g.ny + g.nz + 4;
+        while (s.live && guard-- > 0) sid_step(s, r, rd);
     } else if (mode < 2) {
-        sid_walk<0>(s, r, mode == 0 ? 1 : -1, vol, vol_inv, nx, ny, nz);
+        s.iA = r.idx[0];
+        sid_walk<0>(s, r, mode == 0 ? 1 : -1, nx, rd);
     } else {
-        sid_walk<1>(s, r, mode == 2 ? 1 : -1, vol, vol_inv, nx, ny, nz);
+        s.iA = r.idx[1];
+        sid_walk<1>(s, r, mode == 2 ? 1 : -1, ny, rd);
     }
     if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), s.acc, sino_scale);
 }
@@ -291,23 +340,23 @@ template <typename TS>
 __global__ __launch_bounds__(BX* BY) void siddon_adjoint_kernel(DevGeom g, const TS* __restrict__ sino,
                                                               float sino_inv, float* __restrict__ vol) {
     const int m = blockIdx.x * BX + threadIdx.x;
-    const int n = blockIdx.y * BY + threadIdx.y;
-    const int p = blockIdx.z;
+    const int n = blockIdx.z * BY + threadIdx.y;
+    const int p = blockIdx.y;
     if (m >= g.nu || n >= g.nv) return;
     const float gv = Storage<TS>::load(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), sino_inv);
     if (gv == 0.0f) return;
     const SidRay r = sid_setup(g, p, m, n);
-    if (r.ok) sid_trace<1, float>(g, r, nullptr, 1.0f, vol, gv, nullptr);
+    if (r.ok) sid_trace<1>(g, r, vol, gv, nullptr);
 }
 
 __global__ __launch_bounds__(BX* BY) void siddon_count_kernel(DevGeom g, unsigned long long* total) {
     const int m = blockIdx.x * BX + threadIdx.x;
-    const int n = blockIdx.y * BY + threadIdx.y;
-    const int p = blockIdx.z;
+    const int n = blockIdx.z * BY + threadIdx.y;
+    const int p = blockIdx.y;
     unsigned c = 0;
     if (m < g.nu && n < g.nv) {
         const SidRay r = sid_setup(g, p, m, n);
-        if (r.ok) sid_trace<2, float>(g, r, nullptr, 1.0f, nullptr, 0.0f, &c);
+        if (r.ok) sid_trace<2>(g, r, nullptr, 0.0f, &c);
     }
     unsigned long long cc = c;
     for (int off = 32; off > 0; off >>= 1) cc += __shfl_down(cc, off, 64);
@@ -319,8 +368,8 @@ template <typename TV, typename TS>
 __global__ __launch_bounds__(BX* BY) void siddon_compat_kernel(DevGeom g, const TV* __restrict__ vol, float vol_inv,
                                                              TS* __restrict__ sino, float sino_scale) {
     const unsigned long un = blockIdx.x * BX + threadIdx.x;
-    const unsigned long vn = blockIdx.y * BY + threadIdx.y;
-    const int phi = blockIdx.z;
+    const unsigned long vn = blockIdx.z * BY + threadIdx.y;
+    const int phi = blockIdx.y;
     const unsigned long nx = g.nx, ny = g.ny, nz = g.nz, nu = g.nu, nv = g.nv;
     if (un >= nu || vn >= nv) return;
     const float fdd = (float)g.fdd, fod = (float)g.fod, dl = g.delta;
@@ -396,7 +445,8 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
         return fail(TOMO_ERR_UNSUPPORTED,
                     "compat Siddon is defined for a 256^3 volume and at most 256x256 detector cells "
                     "(the reference indexes modulo 256)");
-    dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
+    // Grid: column group, view, row group (every view of a row band in turn).
+    const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     const float inv = 1.0f / vol.scale;
     if (compat) {
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
@@ -411,15 +461,20 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     // y-fastest copy of the volume for the x-slab waves (stream-ordered pool
     // allocation, released after the projection on the same stream).
     const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
+    const size_t bytes = g->nvox() * es;
     void* vy = nullptr;
-    TOMO_HIP(hipMallocAsync(&vy, g->nvox() * es, s));
+    TOMO_HIP(hipMallocAsync(&vy, bytes, s));
     hipError_t le = launch_transpose_xy(vol.p, vy, es, d.nx, d.ny, d.nz, s);
     if (le == hipSuccess) {
-        const dim3 sgrid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
+        const bool buf = bytes < (1ull << 32);
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-                hipLaunchKernelGGL((siddon_kernel<TV, TS>), sgrid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                   (const TV*)vy, inv, (TS*)sino.p, sino.scale);
+                if (buf)
+                    hipLaunchKernelGGL((siddon_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
+                                       (const TV*)vy, (unsigned)bytes, inv, (TS*)sino.p, sino.scale);
+                else
+                    hipLaunchKernelGGL((siddon_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
+                                       (const TV*)vy, 0u, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();
@@ -434,7 +489,7 @@ int launch_siddon_adjoint(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t 
     const DevGeom& d = g->dev;
     int rc = launch_fill((float*)vol.p, 0.0f, g->nvox(), s);
     if (rc) return rc;
-    dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
+    const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     const float inv = 1.0f / sino.scale;
     TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
         hipLaunchKernelGGL((siddon_adjoint_kernel<TS>), grid, dim3(BX, BY), 0, s, d, (const TS*)sino.p, inv,
@@ -449,7 +504,7 @@ int count_siddon_segments(tomo_ctx* ctx, const tomo_geom* g, uint64_t* out, hipS
     unsigned long long* dt = nullptr;
     TOMO_HIP(hipMallocAsync((void**)&dt, sizeof(unsigned long long), s));
     TOMO_HIP(hipMemsetAsync(dt, 0, sizeof(unsigned long long), s));
-    dim3 grid(ceil_div(d.nu, BX), ceil_div(d.nv, BY), d.nphi);
+    const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     hipLaunchKernelGGL(siddon_count_kernel, grid, dim3(BX, BY), 0, s, d, dt);
     TOMO_HIP(hipGetLastError());
     unsigned long long h = 0;
