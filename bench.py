@@ -35,7 +35,24 @@ def parse():
     ap.add_argument("--nphi", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=4, help="views in the CPU baseline sample")
+    ap.add_argument("--calibration-copy", action="store_true",
+                    help="run one plain device copy of the volume (known bytes) before the timed steps, "
+                         "as the reference kernel for FETCH_SIZE / WRITE_SIZE counter scaling")
     return ap.parse_args()
+
+
+def load_traffic(n, nphi):
+    """HBM bytes per launch measured with rocprofv3 FETCH_SIZE / WRITE_SIZE
+    (profiles/hbm_traffic_512.md), for the workload they were measured on."""
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_512.md")
+    if not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("traffic_json:"):
+            d = json.loads(line.split(":", 1)[1])
+            if d.get("n") == n and d.get("nphi") == nphi:
+                return d.get("kernels", {})
+    return {}
 
 
 def timed(fn, reps, stream):
@@ -111,6 +128,10 @@ def main():
     def rs():
         G.project(vol, sino, lib.PROJ_RS, stream=stream)
 
+    if a.calibration_copy:
+        copy = vol.clone()
+        torch.cuda.synchronize()
+        del copy
     for _ in range(a.warmup):
         siddon(); vi()
     rs()
@@ -147,10 +168,15 @@ def main():
     dom = "vi" if vi_ms >= sid_ms else "siddon"
     dom_bytes = b_vi if dom == "vi" else b_sid
     dom_ms = vi_ms if dom == "vi" else sid_ms
+    dom_kernel = "vi_kernel" if dom == "vi" else "siddon_kernel"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": "vi_kernel" if dom == "vi" else "siddon_kernel",
+    measured = load_traffic(n, nphi)
+    for name, key in (("siddon", "siddon_kernel"), ("vi", "vi_kernel"), ("rs", "rs_kernel")):
+        if key in measured:
+            ops[name]["hbm_traffic_bytes"] = measured[key]
+    roofline = {"bound": "hbm", "kernel": dom_kernel,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": None}
+                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": measured.get(dom_kernel)}
 
     base = None
     if not a.no_cpu_baseline:

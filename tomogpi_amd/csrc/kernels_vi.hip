@@ -51,6 +51,54 @@ __device__ __forceinline__ float bilinear(const TS* __restrict__ gp, int nu, int
     return fmaf(fv, b - a, a);
 }
 
+// Bilinear read with the two u taps of a detector row fetched by one
+// 2-element load.  The u side (column iu, weight fu, which element of the
+// pair is which tap and whether it lies on the detector) depends only on the
+// (column, view) and is prepared once for the ZB voxels of the column.  Tap
+// values and interpolation order are those of bilinear().
+struct USide {
+    float fu;
+    int st;            // first element of the loaded pair, in [0, nu-2]
+    bool ok;           // floor(u) in [-1, nu]
+    bool a_ok, a_hi;   // tap iu: on the detector, taken from the pair's second element
+    bool b_ok, b_hi;   // tap iu+1
+};
+
+__device__ __forceinline__ USide u_side(float u, int nu) {
+    USide s;
+    const float uf = floorf(u);
+    s.ok = uf >= -1.0f && uf <= (float)nu;
+    const int iu = s.ok ? (int)uf : 0;
+    s.fu = u - uf;
+    s.st = min(max(iu, 0), nu - 2);
+    s.a_ok = iu >= 0 && iu < nu;
+    s.a_hi = iu != s.st;
+    s.b_ok = iu + 1 >= 0 && iu + 1 < nu;
+    s.b_hi = iu + 1 != s.st;
+    return s;
+}
+
+template <typename TS>
+__device__ __forceinline__ float bilinear_pairs(const TS* __restrict__ gp, const USide& us, int nu, int nv, float v,
+                                                float inv) {
+    const float vf = floorf(v);
+    if (!(us.ok && vf >= -1.0f && vf <= (float)nv)) return 0.0f;
+    const int iv = (int)vf;
+    const float fv = v - vf;
+    const bool v0 = iv >= 0 && iv < nv, v1 = iv + 1 >= 0 && iv + 1 < nv;
+    const unsigned o0 = v0 ? (unsigned)iv * (unsigned)nu + (unsigned)us.st : 0u;
+    const unsigned o1 = v1 ? (unsigned)(iv + 1) * (unsigned)nu + (unsigned)us.st : 0u;
+    const float2 r0 = Storage<TS>::load2(gp, o0, inv);
+    const float2 r1 = Storage<TS>::load2(gp, o1, inv);
+    const float t00 = (v0 && us.a_ok) ? (us.a_hi ? r0.y : r0.x) : 0.0f;
+    const float t01 = (v0 && us.b_ok) ? (us.b_hi ? r0.y : r0.x) : 0.0f;
+    const float t10 = (v1 && us.a_ok) ? (us.a_hi ? r1.y : r1.x) : 0.0f;
+    const float t11 = (v1 && us.b_ok) ? (us.b_hi ? r1.y : r1.x) : 0.0f;
+    const float a = fmaf(us.fu, t01 - t00, t00);
+    const float b = fmaf(us.fu, t11 - t10, t10);
+    return fmaf(fv, b - a, a);
+}
+
 template <typename TS, typename TV>
 __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ sino, float sino_inv,
                                                   TV* __restrict__ vol, float vol_scale) {
@@ -65,13 +113,26 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
 #pragma unroll
     for (int k = 0; k < ZB; ++k) acc[k] = 0.0f;
     const size_t plane = (size_t)g.nu * g.nv;
-    for (int p = 0; p < g.nphi; ++p) {
-        const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
-        const TS* gp = sino + (size_t)p * plane;
+    if (g.nu >= 2) {
+        for (int p = 0; p < g.nphi; ++p) {
+            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+            const USide us = u_side(cvw.u, g.nu);
+            const TS* gp = sino + (size_t)p * plane;
 #pragma unroll
-        for (int k = 0; k < ZB; ++k) {
-            const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-            acc[k] += bilinear(gp, g.nu, g.nv, cvw.u, v, sino_inv);
+            for (int k = 0; k < ZB; ++k) {
+                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                acc[k] += bilinear_pairs(gp, us, g.nu, g.nv, v, sino_inv);
+            }
+        }
+    } else {
+        for (int p = 0; p < g.nphi; ++p) {
+            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+            const TS* gp = sino + (size_t)p * plane;
+#pragma unroll
+            for (int k = 0; k < ZB; ++k) {
+                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                acc[k] += bilinear(gp, g.nu, g.nv, cvw.u, v, sino_inv);
+            }
         }
     }
     const size_t sxy = (size_t)g.nx * g.ny;
