@@ -1,0 +1,118 @@
+"""GPU: the simple-gradient loop against the oracle's loop, and the
+mixed-precision storage forms against the FP64 oracle.
+
+Loop tolerance: the operators, vector ops and Laplacian are bit-identical to
+the oracle; the FP64 norms are summed in a different order (block partials
+vs OpenMP), so alpha can differ in its last bits and the volumes are compared
+at rel-L2 <= 1e-6, the step sizes at rel 1e-9.
+
+Mixed forms: FP16 (scaled to put max|x| at 2^14) and BF16 storage of inputs
+and outputs, FP32 arithmetic, FP64 geometry.  Error is stated per result as
+relative L2 against the FP64 oracle on the FP32 input; the bound asserted is
+the north-star 1e-3 for FP16 and 4e-3 for BF16 (BF16 keeps 8 significant
+bits: rounding a stored result alone costs about 1e-3 relative).
+"""
+import numpy as np
+import pytest
+import torch
+
+from oracle_ffi import Oracle, ellipsoids, scaled_geometry
+from test_gpu_parity import to_desc, rel_l2
+from tomogpi_amd import lib
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def small(ctx):
+    g = scaled_geometry(40, 20)
+    return g, Oracle(g), lib.Geometry(ctx, to_desc(g))
+
+
+@pytest.mark.parametrize("pair,flags,lam,kopt,positivity", [
+    (lib.PAIR_RSVI, 0, 0.01, 10, 0),
+    (lib.PAIR_RSVI, lib.FLAG_RS_FOV_CUBE, 0.0, 2, 1),
+    (lib.PAIR_SIDDONVI, 0, 0.01, 3, 0),
+])
+def test_gd_loop_matches_oracle(small, pair, flags, lam, kopt, positivity):
+    g, o, G = small
+    print(f"[case] gd pair={pair} flags={flags} lam={lam} kopt={kopt}", flush=True)
+    truth = ellipsoids(g.nx, g.ny, g.nz, count=6, seed=4, amin=2.0, amax=9.0)
+    sino = o.rs(truth) if pair == lib.PAIR_RSVI else o.siddon(truth)
+    ref_vol, ref_stats = o.gd(sino, pair=pair, cube=bool(flags & lib.FLAG_RS_FOV_CUBE), iters=4, kopt=kopt,
+                              lam=lam, positivity=positivity)
+    vol = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+    stats = G.gd_run(torch.from_numpy(sino).cuda(), vol, pair=pair, flags=flags, iters=4, kopt=kopt, lam=lam,
+                     positivity=positivity)
+    torch.cuda.synchronize()
+    for a, b in zip(stats, ref_stats):
+        for k in ("alpha", "residual_sq", "norm_dj", "norm_dj_proj"):
+            assert a[k] == pytest.approx(b[k], rel=1e-9, abs=1e-12), (k, a, b)
+    assert rel_l2(vol.cpu().numpy(), ref_vol) <= 1e-6
+    assert stats[-1]["residual_sq"] < stats[0]["residual_sq"]
+
+
+def test_gd_loop_host_pointers(small):
+    g, o, G = small
+    truth = ellipsoids(g.nx, g.ny, g.nz, count=6, seed=4, amin=2.0, amax=9.0)
+    sino = o.rs(truth)
+    vol = np.zeros(g.vol_shape, dtype=np.float32)
+    G.gd_run(sino, vol, iters=2, kopt=10, lam=0.01)
+    ref_vol, _ = o.gd(sino, iters=2, kopt=10, lam=0.01)
+    assert rel_l2(vol, ref_vol) <= 1e-6
+
+
+def _to16(x, dtype, scale):
+    t = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+    return (t * scale).to(dtype)
+
+
+def _from16(t, scale):
+    return (t.float() / scale).cpu().numpy().astype(np.float64)
+
+
+@pytest.fixture(scope="module")
+def mid(ctx):
+    g = scaled_geometry(96, 48)
+    return g, Oracle(g), lib.Geometry(ctx, to_desc(g))
+
+
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+@pytest.mark.parametrize("op", ["vi", "rs", "siddon"])
+def test_mixed_precision_error_vs_fp64(mid, dt, op):
+    g, o, G = mid
+    rng = np.random.default_rng(12)
+    tdt = torch.float16 if dt == "f16" else torch.bfloat16
+    code = lib.F16 if dt == "f16" else lib.BF16
+    bound = 1e-3 if dt == "f16" else 4e-3
+    if op == "vi":
+        x = o.rs(ellipsoids(g.nx, count=10, seed=8, amin=4.0, amax=20.0))
+        ref = o.vi(x.astype(np.float64), f64=True)
+        sx = 2.0 ** 14 / np.abs(x).max() if dt == "f16" else 1.0
+        sy = 2.0 ** 14 / np.abs(ref).max() if dt == "f16" else 1.0
+        xin = _to16(x, tdt, sx)
+        out = torch.empty(g.vol_shape, dtype=tdt, device="cuda")
+        G.backproject(xin, out, sino_scale=sx, vol_scale=sy)
+    else:
+        x = ellipsoids(g.nx, count=10, seed=8, amin=4.0, amax=20.0) + rng.random(g.vol_shape).astype(np.float32) * 0.1
+        ref = o.rs(x.astype(np.float64), f64=True) if op == "rs" else o.siddon(x.astype(np.float64), f64=True)
+        sx = 2.0 ** 14 / np.abs(x).max() if dt == "f16" else 1.0
+        sy = 2.0 ** 14 / np.abs(ref).max() if dt == "f16" else 1.0
+        xin = _to16(x, tdt, sx)
+        out = torch.empty(g.sino_shape, dtype=tdt, device="cuda")
+        G.project(xin, out, lib.PROJ_RS if op == "rs" else lib.PROJ_SIDDON, vol_scale=sx, sino_scale=sy)
+    torch.cuda.synchronize()
+    err = rel_l2(_from16(out, sy), ref)
+    print(f"[mixed] op={op} storage={dt} rel_l2_vs_fp64={err:.3e}", flush=True)
+    assert err <= bound
+
+
+def test_mixed_inputs_fp32_outputs_close_to_fp32(mid):
+    """BF16-stored inputs with FP32 outputs: the only rounding is the input's."""
+    g, o, G = mid
+    x = ellipsoids(g.nx, count=10, seed=8, amin=4.0, amax=20.0)
+    xin = _to16(x, torch.bfloat16, 1.0)
+    out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+    G.project(xin, out, lib.PROJ_SIDDON)
+    ref = o.siddon(x.astype(np.float64), f64=True)
+    assert rel_l2(out.cpu().numpy(), ref) <= 2e-3
