@@ -257,6 +257,147 @@ __device__ __forceinline__ void sid_walk(SidState& s, const SidRay& r, int sA, i
     }
 }
 
+// z-pair copy of the volume for slabs along A (0: x, 1: y; B is the other
+// in-plane axis): element (a, zz, bb), zz in [0, nz], bb in [0, nB+1], holds
+// f(A=a, B=bb-1, z=zz-1) and f(A=a, B=bb-1, z=zz), 0 outside the grid, with B
+// varying fastest.  Two neighbouring elements are the 2x2 (B, z) block with
+// lower corner (bb-1, zz-1), one 16-byte (FP32) / 8-byte (16-bit) load.
+__device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int nB) {
+    return (((size_t)a * (nz + 1) + zz) * (nB + 2) + bb) * 2;
+}
+
+// A = y: B = x is also the caller's fastest axis; a direct pass.
+template <typename T>
+__global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
+                                                    int nz) {
+    const int bb = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, zz = blockIdx.z;
+    if (bb > nx + 1) return;
+    const int x = bb - 1;
+    T v[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int z = zz - 1 + c;
+        __builtin_memset(&v[c], 0, sizeof(T));
+        if (x >= 0 && x < nx && z >= 0 && z < nz) v[c] = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+    }
+    const size_t o = zpair_index(y, zz, bb, nz, nx);
+    out[o] = v[0];
+    out[o + 1] = v[1];
+}
+
+// A = x: B = y, a 32x32 tile transpose of two z planes through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
+                                                    int nz) {
+    __shared__ T tile[2][32][33];
+    const int x0 = blockIdx.x * 32, b0 = blockIdx.y * 32, zz = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    for (int r = ty; r < 32; r += 8) {
+        const int x = x0 + tx, y = b0 + r - 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int z = zz - 1 + c;
+            T v;
+            __builtin_memset(&v, 0, sizeof(T));
+            if (x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+                v = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+            tile[c][r][tx] = v;
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int x = x0 + r, bb = b0 + tx;
+        if (x < nx && bb <= ny + 1) {
+            const size_t o = zpair_index(x, zz, bb, nz, ny);
+            out[o] = tile[0][tx][r];
+            out[o + 1] = tile[1][tx][r];
+        }
+    }
+}
+
+template <typename TV> struct Quad4;
+template <> struct Quad4<float> {
+    __device__ __forceinline__ static float4 load(const float* p, size_t e, float) {
+        float4 r;
+        __builtin_memcpy(&r, p + e, sizeof(r));
+        return r;
+    }
+};
+template <> struct Quad4<__half> {
+    __device__ __forceinline__ static float4 load(const __half* p, size_t e, float inv) {
+        __half h[4];
+        __builtin_memcpy(h, p + e, sizeof(h));
+        return make_float4(__half2float(h[0]) * inv, __half2float(h[1]) * inv, __half2float(h[2]) * inv,
+                           __half2float(h[3]) * inv);
+    }
+};
+template <> struct Quad4<__hip_bfloat16> {
+    __device__ __forceinline__ static float4 load(const __hip_bfloat16* p, size_t e, float inv) {
+        unsigned short h[4];
+        __builtin_memcpy(h, p + e, sizeof(h));
+        return make_float4(__uint_as_float((unsigned)h[0] << 16) * inv, __uint_as_float((unsigned)h[1] << 16) * inv,
+                           __uint_as_float((unsigned)h[2] << 16) * inv, __uint_as_float((unsigned)h[3] << 16) * inv);
+    }
+};
+
+// Slab walk on the z-pair copy.  Per slab one load gives the current voxel
+// V0, its B neighbour Vb, its z neighbour Vz and Vbz (in the step directions).
+// The wave was admitted only if, for each of its rays, consecutive crossings
+// of B and of z are spaced more than two ulps of tmax beyond the slab
+// length, so a slab holds at most one crossing of each; its segments are
+//   [t, e1] V0,  [e1, e2] Vb or Vz (whichever comes first),  [e2, tA] Vb, Vz or Vbz
+// with e1 <= e2 the interior crossings (tA when absent), every end clipped to
+// tmax.  These are the segments of the general step in the same order; an
+// absent one has zero length and adds exactly 0.  ZNEG: the wave's rays go
+// down in z (one detector row, one sign).
+template <int A, bool ZNEG, typename TV>
+__device__ __forceinline__ void sid_walk_quads(const SidRay& r, const TV* __restrict__ q, float inv, int nA, int nB,
+                                               int nz, bool live, float& acc_out) {
+    constexpr int B = 1 - A;
+    const int sA = r.step[A], sB = r.step[B];
+    const bool bneg = sB < 0;
+    float tA = r.first[A], tB = r.first[B], tz = r.first[2];
+    float cA = 0.0f, cB = 0.0f, cz = 0.0f;
+    int iA = r.idx[A], iB = r.idx[B], iz = r.idx[2];
+    float t = 0.0f, acc = 0.0f;
+    int I = sA > 0 ? wave_min(live ? iA : 0x7fffffff) : wave_max(live ? iA : -0x7fffffff);
+    for (int outer = 0; outer <= nA + 1 && __any(live); ++outer) {
+        if (live && iA == I) {
+            // Block corner; live lanes are inside the grid, the clamps only
+            // keep every address inside the copy.
+            const int bq = min(max(bneg ? iB - 1 : iB, -1), nB - 1);
+            const int zq = min(max(ZNEG ? iz - 1 : iz, -1), nz - 1);
+            const int ia = min(max(iA, 0), nA - 1);
+            const float4 v = Quad4<TV>::load(q, zpair_index(ia, zq + 1, bq + 1, nz, nB), inv);
+            // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
+            const float V0 = bneg ? (ZNEG ? v.w : v.z) : (ZNEG ? v.y : v.x);
+            const float Vb = bneg ? (ZNEG ? v.y : v.x) : (ZNEG ? v.w : v.z);
+            const float Vz = bneg ? (ZNEG ? v.z : v.w) : (ZNEG ? v.x : v.y);
+            const float Vbz = bneg ? (ZNEG ? v.x : v.y) : (ZNEG ? v.z : v.w);
+            const bool hb = tB < tA, hz = tz < tA;
+            const float tb = hb ? tB : tA, tzz = hz ? tz : tA;
+            const float e1 = fminf(tb, tzz), e2 = fmaxf(tb, tzz);
+            const float V1 = tb < tzz ? Vb : (tzz < tb ? Vz : Vbz);
+            const float V2 = hb ? (hz ? Vbz : Vb) : (hz ? Vz : V0);
+            const float c1 = fminf(e1, r.tmax), c2 = fminf(e2, r.tmax), c3 = fminf(tA, r.tmax);
+            acc = fmaf(c1 - t, V0, acc);
+            acc = fmaf(c2 - c1, V1, acc);
+            acc = fmaf(c3 - c2, V2, acc);
+            t = c3;
+            const bool stepB = hb || tB == tA, stepZ = hz || tz == tA;
+            if (stepB) { iB += sB; cB += 1.0f; tB = fmaf(cB, r.inc[B], r.first[B]); }
+            if (stepZ) { iz += r.step[2]; cz += 1.0f; tz = fmaf(cz, r.inc[2], r.first[2]); }
+            iA += sA;
+            cA += 1.0f;
+            tA = fmaf(cA, r.inc[A], r.first[A]);
+            live = t < r.tmax;
+        }
+        I += sA;
+    }
+    acc_out = acc;
+}
+
 // Forward clean Siddon.  Each lane takes the segments of the per-ray
 // traversal (same operations, same order), so results are bit-identical to
 // it.  What changes is the schedule and the memory layout: when every ray of
@@ -275,8 +416,9 @@ __device__ __forceinline__ void sid_walk(SidState& s, const SidRay& r, int sA, i
 // per view.
 template <typename TV, typename TS, bool BUF>
 __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ vol_x,
-                                                      const TV* __restrict__ vol_y, unsigned vol_bytes, float vol_inv,
-                                                      TS* __restrict__ sino, float sino_scale) {
+                                                      const TV* __restrict__ qx, const TV* __restrict__ qy,
+                                                      unsigned vol_bytes, float vol_inv, TS* __restrict__ sino,
+                                                      float sino_scale) {
     const int m = blockIdx.x * BX + threadIdx.x;
     const int n = blockIdx.z * BY + threadIdx.y;
     const int p = blockIdx.y;
@@ -285,14 +427,21 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
     r.ok = false;
     if (inside) r = sid_setup(g, p, m, n);
     const bool ok = r.ok && r.tmax > 0.0f;
-    // Slab code of this ray: 0/1 = x axis up/down, 2/3 = y axis up/down when
-    // that axis dominates all three components (inc = delta/|d| smallest).
+    // Slab code of this ray: 0/1 = x axis up/down, 2/3 = y axis up/down, when
+    // the crossings of the other two axes are spaced farther apart than the
+    // slabs by more than two ulps of tmax (inc = delta/|d|), and the first
+    // slab is no longer than the others.  A FOV entry outside the grid along
+    // the slab axis (the cylinder is wider than the grid when ny < nx) clamps
+    // the first voxel and makes the first slab long; such rays step per lane.
     int code = -1;
     if (ok) {
-        const float ax = r.step[0] ? r.inc[0] : 3.0e38f, ay = r.step[1] ? r.inc[1] : 3.0e38f;
-        const float az = r.step[2] ? r.inc[2] : 3.0e38f;
-        if (r.step[0] != 0 && ax <= ay && ax <= az) code = r.step[0] > 0 ? 0 : 1;
-        else if (r.step[1] != 0 && ay <= ax && ay <= az) code = r.step[1] > 0 ? 2 : 3;
+        const float big = 3.0e38f, margin = 1.0e-6f * r.tmax;
+        const float ix = r.step[0] ? r.inc[0] : big, iy = r.step[1] ? r.inc[1] : big;
+        const float iz = r.step[2] ? r.inc[2] : big;
+        if (r.step[0] != 0 && iy > ix + margin && iz > ix + margin && r.first[0] <= ix)
+            code = r.step[0] > 0 ? 0 : 1;
+        else if (r.step[1] != 0 && ix > iy + margin && iz > iy + margin && r.first[1] <= iy)
+            code = r.step[1] > 0 ? 2 : 3;
     }
     const unsigned long long okmask = __ballot(ok);
     int mode = -1;
@@ -301,39 +450,43 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
         for (int c = 0; c < 4; ++c)
             if (__ballot(ok && code == c) == okmask) mode = c;
     }
-    const int nx = g.nx, ny = g.ny;
-    const bool yfast = mode == 0 || mode == 1;
-    const auto rd = make_reader<TV, BUF>(yfast ? vol_y : vol_x, vol_bytes, vol_inv);
-    const unsigned s0 = yfast ? (unsigned)ny : 1u, s1 = yfast ? 1u : (unsigned)nx;
-    const unsigned s2 = (unsigned)nx * (unsigned)ny;
-    SidState s;
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        s.tn[a] = r.first[a];
-        s.cnt[a] = 0.0f;
-    }
-    // Element counts are < 2^32 (checked when the geometry is built): the
-    // linear index and its signed steps wrap correctly in 32-bit unsigned.
-    s.lin = ok ? (unsigned)r.idx[0] * s0 + (unsigned)r.idx[1] * s1 + (unsigned)r.idx[2] * s2 : 0u;
-    s.dlin[0] = (unsigned)r.step[0] * s0;
-    s.dlin[1] = (unsigned)r.step[1] * s1;
-    s.dlin[2] = (unsigned)r.step[2] * s2;
-    s.t = 0.0f;
-    s.acc = 0.0f;
-    s.live = ok;
+    // one detector row per wave: one z direction (or none)
+    const bool zneg_all = __ballot(ok && r.step[2] > 0) == 0ull;
+    const bool zpos_all = __ballot(ok && r.step[2] < 0) == 0ull;
+    if (!zneg_all && !zpos_all) mode = -1;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    float acc = 0.0f;
     if (mode < 0) {
+        const auto rd = make_reader<TV, BUF>(vol_x, vol_bytes, vol_inv);
+        SidState s;
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            s.tn[a] = r.first[a];
+            s.cnt[a] = 0.0f;
+        }
+        // Element counts are < 2^32 (checked when the geometry is built): the
+        // linear index and its signed steps wrap correctly in 32-bit unsigned.
+        const unsigned s1 = (unsigned)nx, s2 = (unsigned)nx * (unsigned)ny;
+        s.lin = ok ? (unsigned)r.idx[0] + (unsigned)r.idx[1] * s1 + (unsigned)r.idx[2] * s2 : 0u;
+        s.dlin[0] = (unsigned)r.step[0];
+        s.dlin[1] = (unsigned)r.step[1] * s1;
+        s.dlin[2] = (unsigned)r.step[2] * s2;
+        s.t = 0.0f;
+        s.acc = 0.0f;
+        s.live = ok;
         // Every step moves one axis toward the grid exit that bounds tmax;
         // the cap only bounds the loop.
-        int guard = g.nx + g.ny + g.nz + 4;
+        int guard = nx + ny + nz + 4;
         while (s.live && guard-- > 0) sid_step(s, r, rd);
+        acc = s.acc;
     } else if (mode < 2) {
-        s.iA = r.idx[0];
-        sid_walk<0>(s, r, mode == 0 ? 1 : -1, nx, rd);
+        if (zneg_all && !zpos_all) sid_walk_quads<0, true>(r, qx, vol_inv, nx, ny, nz, ok, acc);
+        else sid_walk_quads<0, false>(r, qx, vol_inv, nx, ny, nz, ok, acc);
     } else {
-        s.iA = r.idx[1];
-        sid_walk<1>(s, r, mode == 2 ? 1 : -1, ny, rd);
+        if (zneg_all && !zpos_all) sid_walk_quads<1, true>(r, qy, vol_inv, ny, nx, nz, ok, acc);
+        else sid_walk_quads<1, false>(r, qy, vol_inv, ny, nx, nz, ok, acc);
     }
-    if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), s.acc, sino_scale);
+    if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), acc, sino_scale);
 }
 
 template <typename TS>
@@ -458,28 +611,47 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
         TOMO_HIP(hipGetLastError());
         return TOMO_OK;
     }
-    // y-fastest copy of the volume for the x-slab waves (stream-ordered pool
-    // allocation, released after the projection on the same stream).
+    // z-pair copies of the volume for x slabs and y slabs (stream-ordered
+    // pool allocations, released after the projection on the same stream).
     const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
     const size_t bytes = g->nvox() * es;
-    void* vy = nullptr;
-    TOMO_HIP(hipMallocAsync(&vy, bytes, s));
-    hipError_t le = launch_transpose_xy(vol.p, vy, es, d.nx, d.ny, d.nz, s);
+    const size_t qx_elems = (size_t)d.nx * (d.nz + 1) * (d.ny + 2) * 2;
+    const size_t qy_elems = (size_t)d.ny * (d.nz + 1) * (d.nx + 2) * 2;
+    void *qx = nullptr, *qy = nullptr;
+    TOMO_HIP(hipMallocAsync(&qx, qx_elems * es, s));
+    hipError_t le = hipMallocAsync(&qy, qy_elems * es, s);
+    if (le == hipSuccess) {
+        const dim3 gx(ceil_div(d.nx, 32), ceil_div(d.ny + 2, 32), d.nz + 1);
+        const dim3 gy(ceil_div(d.nx + 2, 256), d.ny, d.nz + 1);
+        if (es == 4) {
+            hipLaunchKernelGGL((zpair_x_kernel<uint32_t>), gx, dim3(256), 0, s, (const uint32_t*)vol.p,
+                               (uint32_t*)qx, d.nx, d.ny, d.nz);
+            hipLaunchKernelGGL((zpair_y_kernel<uint32_t>), gy, dim3(256), 0, s, (const uint32_t*)vol.p,
+                               (uint32_t*)qy, d.nx, d.ny, d.nz);
+        } else {
+            hipLaunchKernelGGL((zpair_x_kernel<uint16_t>), gx, dim3(256), 0, s, (const uint16_t*)vol.p,
+                               (uint16_t*)qx, d.nx, d.ny, d.nz);
+            hipLaunchKernelGGL((zpair_y_kernel<uint16_t>), gy, dim3(256), 0, s, (const uint16_t*)vol.p,
+                               (uint16_t*)qy, d.nx, d.ny, d.nz);
+        }
+        le = hipGetLastError();
+    }
     if (le == hipSuccess) {
         const bool buf = bytes < (1ull << 32);
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)
                     hipLaunchKernelGGL((siddon_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                       (const TV*)vy, (unsigned)bytes, inv, (TS*)sino.p, sino.scale);
+                                       (const TV*)qx, (const TV*)qy, (unsigned)bytes, inv, (TS*)sino.p, sino.scale);
                 else
                     hipLaunchKernelGGL((siddon_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                       (const TV*)vy, 0u, inv, (TS*)sino.p, sino.scale);
+                                       (const TV*)qx, (const TV*)qy, 0u, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();
     }
-    TOMO_HIP(hipFreeAsync(vy, s));
+    if (qy) TOMO_HIP(hipFreeAsync(qy, s));
+    TOMO_HIP(hipFreeAsync(qx, s));
     TOMO_HIP(le);
     return TOMO_OK;
 }

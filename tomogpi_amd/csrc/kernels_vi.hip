@@ -99,8 +99,63 @@ __device__ __forceinline__ float bilinear_pairs(const TS* __restrict__ gp, const
     return fmaf(fv, b - a, a);
 }
 
+// Tap-quad copy of the sinogram: element (p, vv, uu), vv in [0, nv],
+// uu in [0, nu+1], holds the detector values (u, v) and (u, v+1) with
+// u = uu-1, v = vv-1 (0 off the detector).  Two neighbouring elements are the
+// four bilinear taps of the cell with lower corner (u, v), so one 16-byte
+// (FP32) or 8-byte (16-bit) load serves one voxel-view update, and the zero
+// border replaces the per-tap range tests.  Values are copied as stored.
+template <typename TS>
+__global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
+                                                     int nv) {
+    const int uu = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int vv = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int p = blockIdx.z;
+    if (uu > nu + 1 || vv > nv) return;
+    const int u = uu - 1, v = vv - 1;
+    const bool uok = u >= 0 && u < nu;
+    const TS* gp = in + (size_t)p * nu * nv;
+    TS a, b;
+    __builtin_memset(&a, 0, sizeof(a));
+    __builtin_memset(&b, 0, sizeof(b));
+    if (uok && v >= 0 && v < nv) a = gp[(size_t)v * nu + u];
+    if (uok && v + 1 >= 0 && v + 1 < nv) b = gp[(size_t)(v + 1) * nu + u];
+    const size_t o = (((size_t)p * (nv + 1) + vv) * (nu + 2) + uu) * 2;
+    out[o] = a;
+    out[o + 1] = b;
+}
+
+template <typename TS> struct Quad;
+template <> struct Quad<float> {
+    // t00 = (u,v), t10 = (u,v+1), t01 = (u+1,v), t11 = (u+1,v+1)
+    __device__ __forceinline__ static float4 load(const float* p, unsigned e, float) {
+        float4 r;
+        __builtin_memcpy(&r, p + e, sizeof(r));
+        return r;
+    }
+};
+template <> struct Quad<__half> {
+    __device__ __forceinline__ static float4 load(const __half* p, unsigned e, float inv) {
+        __half h[4];
+        __builtin_memcpy(h, p + e, sizeof(h));
+        return make_float4(__half2float(h[0]) * inv, __half2float(h[1]) * inv, __half2float(h[2]) * inv,
+                           __half2float(h[3]) * inv);
+    }
+};
+template <> struct Quad<__hip_bfloat16> {
+    __device__ __forceinline__ static float4 load(const __hip_bfloat16* p, unsigned e, float inv) {
+        unsigned short h[4];
+        __builtin_memcpy(h, p + e, sizeof(h));
+        return make_float4(__uint_as_float((unsigned)h[0] << 16) * inv, __uint_as_float((unsigned)h[1] << 16) * inv,
+                           __uint_as_float((unsigned)h[2] << 16) * inv, __uint_as_float((unsigned)h[3] << 16) * inv);
+    }
+};
+
+// Forward VI on the tap-quad copy.  Same u, A, v, floor, weights and
+// interpolation order as bilinear(); floor(u) = nu (or floor(v) = nv) has no
+// tap on the detector and reads as 0 there, so the range is [-1, n-1] here.
 template <typename TS, typename TV>
-__global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ sino, float sino_inv,
+__global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
                                                   TV* __restrict__ vol, float vol_scale) {
     const int i = blockIdx.x * TX + (threadIdx.x % TX);
     const int j = blockIdx.y * TY + (threadIdx.x / TX);
@@ -112,27 +167,27 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
     float acc[ZB];
 #pragma unroll
     for (int k = 0; k < ZB; ++k) acc[k] = 0.0f;
-    const size_t plane = (size_t)g.nu * g.nv;
-    if (g.nu >= 2) {
-        for (int p = 0; p < g.nphi; ++p) {
-            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
-            const USide us = u_side(cvw.u, g.nu);
-            const TS* gp = sino + (size_t)p * plane;
+    const unsigned row = 2u * (unsigned)(g.nu + 2);             // elements per quad row
+    const size_t plane = (size_t)row * (size_t)(g.nv + 1);      // elements per view
+    const float fnu1 = (float)(g.nu - 1), fnv1 = (float)(g.nv - 1);
+    for (int p = 0; p < g.nphi; ++p) {
+        const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+        const float uf = floorf(cvw.u);
+        const bool uok = uf >= -1.0f && uf <= fnu1;
+        const float fu = cvw.u - uf;
+        const unsigned ucol = uok ? 2u * (unsigned)((int)uf + 1) : 0u;
+        const TS* gp = quads + (size_t)p * plane;
 #pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                acc[k] += bilinear_pairs(gp, us, g.nu, g.nv, v, sino_inv);
-            }
-        }
-    } else {
-        for (int p = 0; p < g.nphi; ++p) {
-            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
-            const TS* gp = sino + (size_t)p * plane;
-#pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                acc[k] += bilinear(gp, g.nu, g.nv, cvw.u, v, sino_inv);
-            }
+        for (int k = 0; k < ZB; ++k) {
+            const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+            const float vf = floorf(v);
+            const bool ok = uok && vf >= -1.0f && vf <= fnv1;
+            const unsigned e = ok ? (unsigned)((int)vf + 1) * row + ucol : 0u;
+            const float4 t = Quad<TS>::load(gp, e, sino_inv);
+            const float fv = v - vf;
+            const float a = fmaf(fu, t.z - t.x, t.x);
+            const float b = fmaf(fu, t.w - t.y, t.y);
+            acc[k] += ok ? fmaf(fv, b - a, a) : 0.0f;
         }
     }
     const size_t sxy = (size_t)g.nx * g.ny;
@@ -191,15 +246,30 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
     const DevGeom& d = g->dev;
-    dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
+    const size_t es = sino.dtype == TOMO_F32 ? 4 : 2;
+    const size_t quad_elems = (size_t)d.nphi * (d.nv + 1) * (d.nu + 2) * 2;
+    if (quad_elems >= (1ull << 32)) return fail(TOMO_ERR_INVALID_ARG, "sinogram too large for the tap-quad copy");
+    // Tap-quad copy of the sinogram from the stream-ordered pool, released
+    // after the backprojection on the same stream.
+    void* quads = nullptr;
+    TOMO_HIP(hipMallocAsync(&quads, quad_elems * es, s));
+    const dim3 qgrid(ceil_div(d.nu + 2, 64), ceil_div(d.nv + 1, 4), d.nphi);
+    const dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
     const float inv = 1.0f / sino.scale;
+    hipError_t le = hipSuccess;
     TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-        TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
-            hipLaunchKernelGGL((vi_kernel<TS, TV>), grid, dim3(TX * TY), 0, s, d, (const TS*)sino.p, inv,
-                               (TV*)vol.p, vol.scale);
-        });
+        hipLaunchKernelGGL((vi_quads_kernel<TS>), qgrid, dim3(256), 0, s, (const TS*)sino.p, (TS*)quads, d.nu, d.nv);
+        le = hipGetLastError();
+        if (le == hipSuccess) {
+            TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
+                hipLaunchKernelGGL((vi_kernel<TS, TV>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
+                                   (TV*)vol.p, vol.scale);
+            });
+            le = hipGetLastError();
+        }
     });
-    TOMO_HIP(hipGetLastError());
+    TOMO_HIP(hipFreeAsync(quads, s));
+    TOMO_HIP(le);
     return TOMO_OK;
 }
 
