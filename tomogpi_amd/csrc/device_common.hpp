@@ -17,13 +17,14 @@
 namespace tomo {
 
 // ---- storage types: value = (float)stored * inv_scale; stored = value*scale
-// load2 reads elements i and i+1 with one load (element-aligned only; a
-// dword-aligned dwordx2 for FP32).
+// (FP32 storage is unscaled).  load4 reads elements i..i+3 with one load:
+// 16 bytes for FP32, 8 for the 16-bit types; i must be even, which with the
+// 256-byte aligned pool buffers keeps the load naturally aligned.
 template <typename T> struct Storage;
 template <> struct Storage<float> {
     __device__ __forceinline__ static float load(const float* p, uint64_t i, float) { return p[i]; }
-    __device__ __forceinline__ static float2 load2(const float* p, uint64_t i, float) {
-        float2 v;
+    __device__ __forceinline__ static float4 load4(const float* p, uint64_t i, float) {
+        float4 v;
         __builtin_memcpy(&v, p + i, sizeof(v));
         return v;
     }
@@ -33,10 +34,11 @@ template <> struct Storage<__half> {
     __device__ __forceinline__ static float load(const __half* p, uint64_t i, float inv) {
         return __half2float(p[i]) * inv;
     }
-    __device__ __forceinline__ static float2 load2(const __half* p, uint64_t i, float inv) {
-        __half2 h;
-        __builtin_memcpy(&h, p + i, sizeof(h));
-        return make_float2(__low2float(h) * inv, __high2float(h) * inv);
+    __device__ __forceinline__ static float4 load4(const __half* p, uint64_t i, float inv) {
+        __half h[4];
+        __builtin_memcpy(h, p + i, sizeof(h));
+        return make_float4(__half2float(h[0]) * inv, __half2float(h[1]) * inv, __half2float(h[2]) * inv,
+                           __half2float(h[3]) * inv);
     }
     __device__ __forceinline__ static void store(__half* p, uint64_t i, float v, float scale) {
         p[i] = __float2half_rn(v * scale);
@@ -46,10 +48,11 @@ template <> struct Storage<__hip_bfloat16> {
     __device__ __forceinline__ static float load(const __hip_bfloat16* p, uint64_t i, float inv) {
         return __bfloat162float(p[i]) * inv;
     }
-    __device__ __forceinline__ static float2 load2(const __hip_bfloat16* p, uint64_t i, float inv) {
-        unsigned short h[2];
+    __device__ __forceinline__ static float4 load4(const __hip_bfloat16* p, uint64_t i, float inv) {
+        unsigned short h[4];
         __builtin_memcpy(h, p + i, sizeof(h));
-        return make_float2(__uint_as_float((unsigned)h[0] << 16) * inv, __uint_as_float((unsigned)h[1] << 16) * inv);
+        return make_float4(__uint_as_float((unsigned)h[0] << 16) * inv, __uint_as_float((unsigned)h[1] << 16) * inv,
+                           __uint_as_float((unsigned)h[2] << 16) * inv, __uint_as_float((unsigned)h[3] << 16) * inv);
     }
     __device__ __forceinline__ static void store(__hip_bfloat16* p, uint64_t i, float v, float scale) {
         p[i] = __float2bfloat16(v * scale);

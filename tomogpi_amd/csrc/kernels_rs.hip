@@ -15,7 +15,7 @@
 // The ray setup is FP64 (the reference parametrises by x, which is singular
 // for rays parallel to y; the arc-length form here is well conditioned), the
 // samples and the sum are FP32.  The output is overwritten.
-#include "device_common.hpp"
+#include "zpair.hpp"
 
 namespace tomo {
 namespace {
@@ -94,73 +94,37 @@ __device__ __forceinline__ RsRay rs_setup(const DevGeom& g, int cube, int p, int
     return r;
 }
 
-template <typename TV>
-__device__ __forceinline__ float trilinear(const TV* __restrict__ f, int nx, int ny, int nz, float x, float y, float z,
-                                           float inv) {
-    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
-    if (!(xf >= -1.0f && xf <= (float)nx && yf >= -1.0f && yf <= (float)ny && zf >= -1.0f && zf <= (float)nz))
-        return 0.0f;
-    const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
-    const float fx = x - xf, fy = y - yf, fz = z - zf;
-    const bool x0 = ix >= 0 && ix < nx, x1 = ix + 1 >= 0 && ix + 1 < nx;
-    const bool y0 = iy >= 0 && iy < ny, y1 = iy + 1 >= 0 && iy + 1 < ny;
-    const bool z0 = iz >= 0 && iz < nz, z1 = iz + 1 >= 0 && iz + 1 < nz;
-    const int64_t sy = nx, sz = (int64_t)nx * ny;
-    const int64_t o = (int64_t)ix + sy * iy + sz * iz;
-    const float t000 = tap(f, o, x0 && y0 && z0, inv), t100 = tap(f, o + 1, x1 && y0 && z0, inv);
-    const float t010 = tap(f, o + sy, x0 && y1 && z0, inv), t110 = tap(f, o + sy + 1, x1 && y1 && z0, inv);
-    const float t001 = tap(f, o + sz, x0 && y0 && z1, inv), t101 = tap(f, o + sz + 1, x1 && y0 && z1, inv);
-    const float t011 = tap(f, o + sy + sz, x0 && y1 && z1, inv), t111 = tap(f, o + sy + sz + 1, x1 && y1 && z1, inv);
-    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
-    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
-    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
-    return fmaf(fz, c1 - c0, c0);
-}
-
-// Two neighbouring elements (c, c+1) of a row of nf >= 2 elements starting
-// at linear index `row`, read with one 2-element load; an element is 0 when
-// it lies outside [0, nf) or the row is invalid.  c is in [-1, nf].
-template <typename TV>
-__device__ __forceinline__ void pair(const TV* __restrict__ base, unsigned row, int c, int nf, bool rowok, float inv,
-                                     float& a, float& b) {
-    const int st = min(max(c, 0), nf - 2);
-    const unsigned o = rowok ? row + (unsigned)st : 0u;
-    const float2 v = Storage<TV>::load2(base, o, inv);
-    a = (rowok && c >= 0 && c < nf) ? (c == st ? v.x : v.y) : 0.0f;
-    b = (rowok && c + 1 >= 0 && c + 1 < nf) ? (c + 1 == st ? v.x : v.y) : 0.0f;
-}
-
-// Trilinear read with the taps fetched as pairs along the fastest axis of
-// the layout: YF = false, x-fastest (x + nx*(y + ny*z)); YF = true,
-// y-fastest (y + ny*(x + nx*z)).  Tap values and the interpolation order
-// (x, then y, then z) are those of trilinear(), so the result is identical.
-template <bool YF, typename TV>
-__device__ __forceinline__ float trilinear_pairs(const TV* __restrict__ f, int nx, int ny, int nz, float x, float y,
+// Trilinear interpolation of f at (x, y, z) (voxel units), taps outside the
+// grid 0, interpolated along x, then y, then z with explicit fmaf.  The eight
+// taps come from two z-pair quads of the slab axis A: the (B, z) block of the
+// planes A = floor(A) and floor(A) + 1.  A = 0 reads qx (B = y), A = 1 reads
+// qy (B = x).  Outside the copy's corner range (B or z corner = n) the block
+// is entirely outside the grid and the quad is 0.
+template <int A, typename TV>
+__device__ __forceinline__ float trilinear_quads(const TV* __restrict__ q, int nx, int ny, int nz, float x, float y,
                                                  float z, float inv) {
     const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
     if (!(xf >= -1.0f && xf <= (float)nx && yf >= -1.0f && yf <= (float)ny && zf >= -1.0f && zf <= (float)nz))
         return 0.0f;
     const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
     const float fx = x - xf, fy = y - yf, fz = z - zf;
-    const bool x0 = ix >= 0 && ix < nx, x1 = ix + 1 >= 0 && ix + 1 < nx;
-    const bool y0 = iy >= 0 && iy < ny, y1 = iy + 1 >= 0 && iy + 1 < ny;
-    const bool z0 = iz >= 0 && iz < nz, z1 = iz + 1 >= 0 && iz + 1 < nz;
-    const unsigned unx = (unsigned)nx, uny = (unsigned)ny;
+    const int ia = A == 0 ? ix : iy, ib = A == 0 ? iy : ix;
+    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    const bool blk = ib < nB && iz < nz;  // ib, iz >= -1 from the range test
+    const bool p0 = blk && ia >= 0 && ia < nA, p1 = blk && ia + 1 < nA;
+    const int bq = min(ib, nB - 1) + 1, zq = min(iz, nz - 1) + 1;
+    const float4 u = Storage<TV>::load4(q, zpair_index(min(max(ia, 0), nA - 1), zq, bq, nz, nB), inv);
+    const float4 w = Storage<TV>::load4(q, zpair_index(min(ia + 1, nA - 1), zq, bq, nz, nB), inv);
+    // u, w = (b, z), (b, z+1), (b+1, z), (b+1, z+1) on planes ia, ia+1
+    const float u0 = p0 ? u.x : 0.0f, u1 = p0 ? u.y : 0.0f, u2 = p0 ? u.z : 0.0f, u3 = p0 ? u.w : 0.0f;
+    const float w0 = p1 ? w.x : 0.0f, w1 = p1 ? w.y : 0.0f, w2 = p1 ? w.z : 0.0f, w3 = p1 ? w.w : 0.0f;
     float t000, t100, t010, t110, t001, t101, t011, t111;
-    if (YF) {
-        const unsigned r00 = uny * ((unsigned)ix + unx * (unsigned)iz);  // row (x0, z0)
-        const unsigned plane = unx * uny;
-        pair(f, r00, iy, ny, x0 && z0, inv, t000, t010);
-        pair(f, r00 + uny, iy, ny, x1 && z0, inv, t100, t110);
-        pair(f, r00 + plane, iy, ny, x0 && z1, inv, t001, t011);
-        pair(f, r00 + plane + uny, iy, ny, x1 && z1, inv, t101, t111);
+    if (A == 0) {
+        t000 = u0; t001 = u1; t010 = u2; t011 = u3;
+        t100 = w0; t101 = w1; t110 = w2; t111 = w3;
     } else {
-        const unsigned r00 = unx * ((unsigned)iy + uny * (unsigned)iz);  // row (y0, z0)
-        const unsigned plane = unx * uny;
-        pair(f, r00, ix, nx, y0 && z0, inv, t000, t100);
-        pair(f, r00 + unx, ix, nx, y1 && z0, inv, t010, t110);
-        pair(f, r00 + plane, ix, nx, y0 && z1, inv, t001, t101);
-        pair(f, r00 + plane + unx, ix, nx, y1 && z1, inv, t011, t111);
+        t000 = u0; t001 = u1; t100 = u2; t101 = u3;
+        t010 = w0; t011 = w1; t110 = w2; t111 = w3;
     }
     const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
     const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
@@ -178,19 +142,18 @@ __device__ __forceinline__ int wave_max(int v) {
 }
 
 // Forward RS projector.  Samples, taps and the order of the sum are those of
-// the per-ray loop (sample s at fma(s, d, entry), trilinear(), acc += in s
-// order), so results are unchanged; the schedule and the layout follow the
-// Siddon projector: when every ray of the wave has the same dominant in-plane
-// axis A (|d_A| >= the other two) and direction, the wave walks the unit
-// slabs floor(A) = I, I+-1, ... together and each lane takes its samples in
-// slab I; its 64 rays (adjacent detector columns) then read contiguous voxels
-// of the layout whose fastest axis is the other in-plane axis (a y-fastest
-// copy for x slabs, the caller's x-fastest volume for y slabs).  Other waves
-// sample per lane from the x-fastest volume.  Grid: x = column group,
+// the per-ray loop (sample s at fma(s, d, entry), trilinear interpolation,
+// acc += in s order), so results are unchanged; the schedule and the layout
+// follow the Siddon projector: when every ray of the wave has the same
+// dominant in-plane axis A (|d_A| >= the other two) and direction, the wave
+// walks the unit slabs floor(A) = I, I+-1, ... together and each lane takes
+// its samples in slab I.  Its 64 rays (adjacent detector columns) then read
+// neighbouring quads of the z-pair copy of A, two 16-byte loads per sample.
+// Other waves sample per lane from the y-slab copy.  Grid: x = column group,
 // y = view, z = row group (every view of a row band before the next band).
 template <typename TV, typename TS>
-__global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, int allow_slab, const TV* __restrict__ vol_x,
-                                                  const TV* __restrict__ vol_y, float vol_inv, TS* __restrict__ sino,
+__global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const TV* __restrict__ qx,
+                                                  const TV* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
                                                   float sino_scale) {
     const int m = blockIdx.x * BX + threadIdx.x;
     const int n = blockIdx.z * BY + threadIdx.y;
@@ -200,7 +163,7 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, int all
     if (inside) r = rs_setup(g, cube, p, m, n);
     const bool ok = r.lr > 0;
     int code = -1;
-    if (ok && allow_slab) {
+    if (ok) {
         const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
         if (ax > 0.0f && ax >= ay && ax >= az) code = r.dx > 0.0f ? 0 : 1;
         else if (ay > 0.0f && ay >= ax && ay >= az) code = r.dy > 0.0f ? 2 : 3;
@@ -215,18 +178,10 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, int all
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     float acc = 0.0f;
     if (mode < 0) {
-        if (allow_slab) {
-            for (int s = 0; s < r.lr; ++s) {
-                const float sf = (float)s;
-                acc += trilinear_pairs<false>(vol_x, nx, ny, nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey),
-                                              fmaf(sf, r.dz, r.ez), vol_inv);
-            }
-        } else {
-            for (int s = 0; s < r.lr; ++s) {
-                const float sf = (float)s;
-                acc += trilinear(vol_x, nx, ny, nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey), fmaf(sf, r.dz, r.ez),
-                                 vol_inv);
-            }
+        for (int s = 0; s < r.lr; ++s) {
+            const float sf = (float)s;
+            acc += trilinear_quads<1>(qy, nx, ny, nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey),
+                                      fmaf(sf, r.dz, r.ez), vol_inv);
         }
     } else {
         const bool xs = mode < 2;
@@ -241,8 +196,8 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, int all
             while (live && slab == I) {
                 const float sf = (float)s;
                 const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
-                acc += xs ? trilinear_pairs<true>(vol_y, nx, ny, nz, x, y, z, vol_inv)
-                          : trilinear_pairs<false>(vol_x, nx, ny, nz, x, y, z, vol_inv);
+                acc += xs ? trilinear_quads<0>(qx, nx, ny, nz, x, y, z, vol_inv)
+                          : trilinear_quads<1>(qy, nx, ny, nz, x, y, z, vol_inv);
                 ++s;
                 live = s < r.lr;
                 slab = (int)floorf(fmaf((float)s, dA, eA));
@@ -307,28 +262,20 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
     const DevGeom& d = g->dev;
     const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     const float inv = 1.0f / vol.scale;
-    // Pair reads need rows of at least two voxels; the y-fastest copy lives in
-    // the stream-ordered pool for the duration of the projection.
-    const int allow_slab = d.nx >= 2 && d.ny >= 2;
-    const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
-    void* vy = nullptr;
-    hipError_t le = hipSuccess;
-    if (allow_slab) {
-        TOMO_HIP(hipMallocAsync(&vy, g->nvox() * es, s));
-        le = launch_transpose_xy(vol.p, vy, es, d.nx, d.ny, d.nz, s);
-    }
+    ZPairs zp;
+    hipError_t le = zp.alloc(vol.p, vol.dtype == TOMO_F32 ? 4 : 2, d.nx, d.ny, d.nz, s);
     if (le == hipSuccess) {
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-                hipLaunchKernelGGL((rs_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, cube, allow_slab,
-                                   (const TV*)vol.p, (const TV*)(allow_slab ? vy : vol.p), inv, (TS*)sino.p,
-                                   sino.scale);
+                hipLaunchKernelGGL((rs_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, cube, (const TV*)zp.qx,
+                                   (const TV*)zp.qy, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();
     }
-    if (vy) TOMO_HIP(hipFreeAsync(vy, s));
+    const hipError_t fe = zp.release(s);
     TOMO_HIP(le);
+    TOMO_HIP(fe);
     return TOMO_OK;
 }
 

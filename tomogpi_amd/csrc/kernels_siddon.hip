@@ -16,7 +16,7 @@
 // evaluation order and quirks: source on the +FOD side (mirror of RS/VI),
 // first voxel half a voxel behind the entry, last chord not clipped, indices
 // % 256 with unsigned wrap, no z clip.  Defined for 256^3 volumes only.
-#include "device_common.hpp"
+#include "zpair.hpp"
 
 namespace tomo {
 namespace {
@@ -182,12 +182,11 @@ __device__ __forceinline__ VolReader<TV, BUF> make_reader(const TV* p, unsigned 
     return v;
 }
 
-// Per-lane state of the forward traversal: next crossing and crossings taken
-// per axis, the slab-axis voxel index, the linear index of the current voxel
-// in the layout being read, the running sum.
+// Per-lane state of the general forward traversal: next crossing and
+// crossings taken per axis, the linear index of the current voxel, the
+// running sum.
 struct SidState {
     float tn[3], cnt[3];
-    int iA;
     unsigned lin;
     unsigned dlin[3];
     float t, acc;
@@ -201,7 +200,8 @@ __device__ __forceinline__ void sid_advance(SidState& s, const SidRay& r) {
     s.lin += s.dlin[AX];
 }
 
-// General step (waves without a common slab axis).
+// General step (waves without a common slab axis): the next segment ends at
+// the nearest crossing; ties step every tied axis.
 template <typename RD>
 __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD& rd) {
     const float tn = fminf(s.tn[0], fminf(s.tn[1], s.tn[2]));
@@ -213,133 +213,6 @@ __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD&
     if (s.tn[2] == tn) sid_advance<2>(s, r);
     s.live = s.t < r.tmax;
 }
-
-// Segments of slab I along axis A (0 or 1; B is the other in-plane axis).
-// The next A crossing is the slab end; B and z crossings that come first are
-// interior segments.  Every segment is the one the general step would take:
-// the interior branch runs only while min(tB, tz) < tA, so its minimum is the
-// minimum of all three, and the slab-end segment only when min(tB, tz) >= tA.
-template <int A, typename RD>
-__device__ __forceinline__ void sid_slab(SidState& s, const SidRay& r, const RD& rd, int sA) {
-    constexpr int B = 1 - A;
-    for (int it = 0; it < 8; ++it) {  // at most one B and one z crossing per slab when A dominates
-        const float tn = fminf(s.tn[B], s.tn[2]);
-        if (!(tn < s.tn[A])) break;
-        const float te = fminf(tn, r.tmax);
-        s.acc = fmaf(te - s.t, rd(s.lin), s.acc);
-        s.t = te;
-        if (s.tn[B] == tn) sid_advance<B>(s, r);
-        if (s.tn[2] == tn) sid_advance<2>(s, r);
-        if (!(s.t < r.tmax)) {
-            s.live = false;
-            return;
-        }
-    }
-    const float tn = s.tn[A];
-    const float te = fminf(tn, r.tmax);
-    s.acc = fmaf(te - s.t, rd(s.lin), s.acc);
-    s.t = te;
-    s.iA += sA;
-    sid_advance<A>(s, r);
-    if (s.tn[B] == tn) sid_advance<B>(s, r);
-    if (s.tn[2] == tn) sid_advance<2>(s, r);
-    s.live = s.t < r.tmax;
-}
-
-// The wave walks slabs I, I+sA, ... of axis A together; a lane takes part
-// while its own slab index equals I.  At most nA+1 outer iterations.
-template <int A, typename RD>
-__device__ __forceinline__ void sid_walk(SidState& s, const SidRay& r, int sA, int nA, const RD& rd) {
-    int I = sA > 0 ? wave_min(s.live ? s.iA : 0x7fffffff) : wave_max(s.live ? s.iA : -0x7fffffff);
-    for (int outer = 0; outer <= nA + 1 && __any(s.live); ++outer) {
-        if (s.live && s.iA == I) sid_slab<A>(s, r, rd, sA);
-        I += sA;
-    }
-}
-
-// z-pair copy of the volume for slabs along A (0: x, 1: y; B is the other
-// in-plane axis): element (a, zz, bb), zz in [0, nz], bb in [0, nB+1], holds
-// f(A=a, B=bb-1, z=zz-1) and f(A=a, B=bb-1, z=zz), 0 outside the grid, with B
-// varying fastest.  Two neighbouring elements are the 2x2 (B, z) block with
-// lower corner (bb-1, zz-1), one 16-byte (FP32) / 8-byte (16-bit) load.
-__device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int nB) {
-    return (((size_t)a * (nz + 1) + zz) * (nB + 2) + bb) * 2;
-}
-
-// A = y: B = x is also the caller's fastest axis; a direct pass.
-template <typename T>
-__global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
-                                                    int nz) {
-    const int bb = blockIdx.x * 256 + threadIdx.x;
-    const int y = blockIdx.y, zz = blockIdx.z;
-    if (bb > nx + 1) return;
-    const int x = bb - 1;
-    T v[2];
-#pragma unroll
-    for (int c = 0; c < 2; ++c) {
-        const int z = zz - 1 + c;
-        __builtin_memset(&v[c], 0, sizeof(T));
-        if (x >= 0 && x < nx && z >= 0 && z < nz) v[c] = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
-    }
-    const size_t o = zpair_index(y, zz, bb, nz, nx);
-    out[o] = v[0];
-    out[o + 1] = v[1];
-}
-
-// A = x: B = y, a 32x32 tile transpose of two z planes through LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
-                                                    int nz) {
-    __shared__ T tile[2][32][33];
-    const int x0 = blockIdx.x * 32, b0 = blockIdx.y * 32, zz = blockIdx.z;
-    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
-    for (int r = ty; r < 32; r += 8) {
-        const int x = x0 + tx, y = b0 + r - 1;
-#pragma unroll
-        for (int c = 0; c < 2; ++c) {
-            const int z = zz - 1 + c;
-            T v;
-            __builtin_memset(&v, 0, sizeof(T));
-            if (x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-                v = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
-            tile[c][r][tx] = v;
-        }
-    }
-    __syncthreads();
-    for (int r = ty; r < 32; r += 8) {
-        const int x = x0 + r, bb = b0 + tx;
-        if (x < nx && bb <= ny + 1) {
-            const size_t o = zpair_index(x, zz, bb, nz, ny);
-            out[o] = tile[0][tx][r];
-            out[o + 1] = tile[1][tx][r];
-        }
-    }
-}
-
-template <typename TV> struct Quad4;
-template <> struct Quad4<float> {
-    __device__ __forceinline__ static float4 load(const float* p, size_t e, float) {
-        float4 r;
-        __builtin_memcpy(&r, p + e, sizeof(r));
-        return r;
-    }
-};
-template <> struct Quad4<__half> {
-    __device__ __forceinline__ static float4 load(const __half* p, size_t e, float inv) {
-        __half h[4];
-        __builtin_memcpy(h, p + e, sizeof(h));
-        return make_float4(__half2float(h[0]) * inv, __half2float(h[1]) * inv, __half2float(h[2]) * inv,
-                           __half2float(h[3]) * inv);
-    }
-};
-template <> struct Quad4<__hip_bfloat16> {
-    __device__ __forceinline__ static float4 load(const __hip_bfloat16* p, size_t e, float inv) {
-        unsigned short h[4];
-        __builtin_memcpy(h, p + e, sizeof(h));
-        return make_float4(__uint_as_float((unsigned)h[0] << 16) * inv, __uint_as_float((unsigned)h[1] << 16) * inv,
-                           __uint_as_float((unsigned)h[2] << 16) * inv, __uint_as_float((unsigned)h[3] << 16) * inv);
-    }
-};
 
 // Slab walk on the z-pair copy.  Per slab one load gives the current voxel
 // V0, its B neighbour Vb, its z neighbour Vz and Vbz (in the step directions).
@@ -369,7 +242,7 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const TV* __rest
             const int bq = min(max(bneg ? iB - 1 : iB, -1), nB - 1);
             const int zq = min(max(ZNEG ? iz - 1 : iz, -1), nz - 1);
             const int ia = min(max(iA, 0), nA - 1);
-            const float4 v = Quad4<TV>::load(q, zpair_index(ia, zq + 1, bq + 1, nz, nB), inv);
+            const float4 v = Storage<TV>::load4(q, zpair_index(ia, zq + 1, bq + 1, nz, nB), inv);
             // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
             const float V0 = bneg ? (ZNEG ? v.w : v.z) : (ZNEG ? v.y : v.x);
             const float Vb = bneg ? (ZNEG ? v.y : v.x) : (ZNEG ? v.w : v.z);
@@ -613,46 +486,27 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     }
     // z-pair copies of the volume for x slabs and y slabs (stream-ordered
     // pool allocations, released after the projection on the same stream).
-    const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
-    const size_t bytes = g->nvox() * es;
-    const size_t qx_elems = (size_t)d.nx * (d.nz + 1) * (d.ny + 2) * 2;
-    const size_t qy_elems = (size_t)d.ny * (d.nz + 1) * (d.nx + 2) * 2;
-    void *qx = nullptr, *qy = nullptr;
-    TOMO_HIP(hipMallocAsync(&qx, qx_elems * es, s));
-    hipError_t le = hipMallocAsync(&qy, qy_elems * es, s);
-    if (le == hipSuccess) {
-        const dim3 gx(ceil_div(d.nx, 32), ceil_div(d.ny + 2, 32), d.nz + 1);
-        const dim3 gy(ceil_div(d.nx + 2, 256), d.ny, d.nz + 1);
-        if (es == 4) {
-            hipLaunchKernelGGL((zpair_x_kernel<uint32_t>), gx, dim3(256), 0, s, (const uint32_t*)vol.p,
-                               (uint32_t*)qx, d.nx, d.ny, d.nz);
-            hipLaunchKernelGGL((zpair_y_kernel<uint32_t>), gy, dim3(256), 0, s, (const uint32_t*)vol.p,
-                               (uint32_t*)qy, d.nx, d.ny, d.nz);
-        } else {
-            hipLaunchKernelGGL((zpair_x_kernel<uint16_t>), gx, dim3(256), 0, s, (const uint16_t*)vol.p,
-                               (uint16_t*)qx, d.nx, d.ny, d.nz);
-            hipLaunchKernelGGL((zpair_y_kernel<uint16_t>), gy, dim3(256), 0, s, (const uint16_t*)vol.p,
-                               (uint16_t*)qy, d.nx, d.ny, d.nz);
-        }
-        le = hipGetLastError();
-    }
+    const size_t bytes = g->nvox() * (vol.dtype == TOMO_F32 ? 4 : 2);
+    ZPairs zp;
+    hipError_t le = zp.alloc(vol.p, vol.dtype == TOMO_F32 ? 4 : 2, d.nx, d.ny, d.nz, s);
     if (le == hipSuccess) {
         const bool buf = bytes < (1ull << 32);
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)
                     hipLaunchKernelGGL((siddon_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                       (const TV*)qx, (const TV*)qy, (unsigned)bytes, inv, (TS*)sino.p, sino.scale);
+                                       (const TV*)zp.qx, (const TV*)zp.qy, (unsigned)bytes, inv, (TS*)sino.p,
+                                       sino.scale);
                 else
                     hipLaunchKernelGGL((siddon_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                       (const TV*)qx, (const TV*)qy, 0u, inv, (TS*)sino.p, sino.scale);
+                                       (const TV*)zp.qx, (const TV*)zp.qy, 0u, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();
     }
-    if (qy) TOMO_HIP(hipFreeAsync(qy, s));
-    TOMO_HIP(hipFreeAsync(qx, s));
+    const hipError_t fe = zp.release(s);
     TOMO_HIP(le);
+    TOMO_HIP(fe);
     return TOMO_OK;
 }
 

@@ -7,7 +7,7 @@
 // as the reference host loops).  Squared norms are sum double(x)^2
 // (C/Image3D_CPU.cu:363-390; "L2Norm" without the square root), reduced in a
 // fixed order: per-block partials, then one block.
-#include "device_common.hpp"
+#include "zpair.hpp"
 
 namespace tomo {
 namespace {
@@ -138,23 +138,55 @@ __global__ __launch_bounds__(kBlock) void ellipsoid_kernel(int nx, int ny, int n
     vol[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = (float)acc;
 }
 
-// out[j + ny*(i + nx*k)] = in[i + nx*(j + ny*k)] through 32x32 LDS tiles.
+// z-pair copy for y slabs (A = y, B = x): B is also the input's fastest axis,
+// a direct pass.  Grid (ceil((nx+2)/256), ny, nz+1).
 template <typename T>
-__global__ __launch_bounds__(256) void transpose_xy_kernel(const T* __restrict__ in, T* __restrict__ out, int nx,
-                                                         int ny) {
-    __shared__ T tile[32][33];
-    const size_t k = blockIdx.z;
-    const int i0 = blockIdx.x * 32, j0 = blockIdx.y * 32;
+__global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
+                                                    int nz) {
+    const int bb = blockIdx.x * 256 + threadIdx.x;
+    const int y = blockIdx.y, zz = blockIdx.z;
+    if (bb > nx + 1) return;
+    const int x = bb - 1;
+    T v[2];
+#pragma unroll
+    for (int c = 0; c < 2; ++c) {
+        const int z = zz - 1 + c;
+        __builtin_memset(&v[c], 0, sizeof(T));
+        if (x >= 0 && x < nx && z >= 0 && z < nz) v[c] = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+    }
+    const size_t o = zpair_index(y, zz, bb, nz, nx);
+    out[o] = v[0];
+    out[o + 1] = v[1];
+}
+
+// z-pair copy for x slabs (A = x, B = y): a 32x32 (x, B) tile transpose of
+// two z planes through LDS.  Grid (ceil(nx/32), ceil((ny+2)/32), nz+1).
+template <typename T>
+__global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
+                                                    int nz) {
+    __shared__ T tile[2][32][33];
+    const int x0 = blockIdx.x * 32, b0 = blockIdx.y * 32, zz = blockIdx.z;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
-    const size_t plane = (size_t)nx * ny;
     for (int r = ty; r < 32; r += 8) {
-        const int i = i0 + tx, j = j0 + r;
-        if (i < nx && j < ny) tile[r][tx] = in[k * plane + (size_t)j * nx + i];
+        const int x = x0 + tx, y = b0 + r - 1;
+#pragma unroll
+        for (int c = 0; c < 2; ++c) {
+            const int z = zz - 1 + c;
+            T v;
+            __builtin_memset(&v, 0, sizeof(T));
+            if (x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+                v = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+            tile[c][r][tx] = v;
+        }
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
-        const int j = j0 + tx, i = i0 + r;
-        if (i < nx && j < ny) out[k * plane + (size_t)i * ny + j] = tile[tx][r];
+        const int x = x0 + r, bb = b0 + tx;
+        if (x < nx && bb <= ny + 1) {
+            const size_t o = zpair_index(x, zz, bb, nz, ny);
+            out[o] = tile[0][tx][r];
+            out[o + 1] = tile[1][tx][r];
+        }
     }
 }
 
@@ -167,14 +199,20 @@ unsigned grid_for(size_t n) {
 
 }  // namespace
 
-hipError_t launch_transpose_xy(const void* in, void* out, size_t elem_size, int nx, int ny, int nz, hipStream_t s) {
-    const dim3 grid(ceil_div(nx, 32), ceil_div(ny, 32), nz);
-    if (elem_size == 4)
-        hipLaunchKernelGGL((transpose_xy_kernel<uint32_t>), grid, dim3(256), 0, s, (const uint32_t*)in, (uint32_t*)out,
-                           nx, ny);
-    else
-        hipLaunchKernelGGL((transpose_xy_kernel<uint16_t>), grid, dim3(256), 0, s, (const uint16_t*)in, (uint16_t*)out,
-                           nx, ny);
+hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s) {
+    const dim3 gx(ceil_div(nx, 32), ceil_div((size_t)ny + 2, 32), nz + 1);
+    const dim3 gy(ceil_div((size_t)nx + 2, 256), ny, nz + 1);
+    if (es == 4) {
+        hipLaunchKernelGGL((zpair_x_kernel<uint32_t>), gx, dim3(256), 0, s, (const uint32_t*)vol, (uint32_t*)qx, nx,
+                           ny, nz);
+        hipLaunchKernelGGL((zpair_y_kernel<uint32_t>), gy, dim3(256), 0, s, (const uint32_t*)vol, (uint32_t*)qy, nx,
+                           ny, nz);
+    } else {
+        hipLaunchKernelGGL((zpair_x_kernel<uint16_t>), gx, dim3(256), 0, s, (const uint16_t*)vol, (uint16_t*)qx, nx,
+                           ny, nz);
+        hipLaunchKernelGGL((zpair_y_kernel<uint16_t>), gy, dim3(256), 0, s, (const uint16_t*)vol, (uint16_t*)qy, nx,
+                           ny, nz);
+    }
     return hipGetLastError();
 }
 

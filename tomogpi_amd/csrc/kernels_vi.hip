@@ -33,72 +33,6 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
     return r;
 }
 
-template <typename TS>
-__device__ __forceinline__ float bilinear(const TS* __restrict__ gp, int nu, int nv, float u, float v, float inv) {
-    const float uf = floorf(u), vf = floorf(v);
-    if (!(uf >= -1.0f && uf <= (float)nu && vf >= -1.0f && vf <= (float)nv)) return 0.0f;
-    const int iu = (int)uf, iv = (int)vf;
-    const float fu = u - uf, fv = v - vf;
-    const bool u0 = iu >= 0 && iu < nu, u1 = iu + 1 >= 0 && iu + 1 < nu;
-    const bool v0 = iv >= 0 && iv < nv, v1 = iv + 1 >= 0 && iv + 1 < nv;
-    const int64_t o = (int64_t)iv * nu + iu;
-    const float t00 = tap(gp, o, u0 && v0, inv);
-    const float t01 = tap(gp, o + 1, u1 && v0, inv);
-    const float t10 = tap(gp, o + nu, u0 && v1, inv);
-    const float t11 = tap(gp, o + nu + 1, u1 && v1, inv);
-    const float a = fmaf(fu, t01 - t00, t00);
-    const float b = fmaf(fu, t11 - t10, t10);
-    return fmaf(fv, b - a, a);
-}
-
-// Bilinear read with the two u taps of a detector row fetched by one
-// 2-element load.  The u side (column iu, weight fu, which element of the
-// pair is which tap and whether it lies on the detector) depends only on the
-// (column, view) and is prepared once for the ZB voxels of the column.  Tap
-// values and interpolation order are those of bilinear().
-struct USide {
-    float fu;
-    int st;            // first element of the loaded pair, in [0, nu-2]
-    bool ok;           // floor(u) in [-1, nu]
-    bool a_ok, a_hi;   // tap iu: on the detector, taken from the pair's second element
-    bool b_ok, b_hi;   // tap iu+1
-};
-
-__device__ __forceinline__ USide u_side(float u, int nu) {
-    USide s;
-    const float uf = floorf(u);
-    s.ok = uf >= -1.0f && uf <= (float)nu;
-    const int iu = s.ok ? (int)uf : 0;
-    s.fu = u - uf;
-    s.st = min(max(iu, 0), nu - 2);
-    s.a_ok = iu >= 0 && iu < nu;
-    s.a_hi = iu != s.st;
-    s.b_ok = iu + 1 >= 0 && iu + 1 < nu;
-    s.b_hi = iu + 1 != s.st;
-    return s;
-}
-
-template <typename TS>
-__device__ __forceinline__ float bilinear_pairs(const TS* __restrict__ gp, const USide& us, int nu, int nv, float v,
-                                                float inv) {
-    const float vf = floorf(v);
-    if (!(us.ok && vf >= -1.0f && vf <= (float)nv)) return 0.0f;
-    const int iv = (int)vf;
-    const float fv = v - vf;
-    const bool v0 = iv >= 0 && iv < nv, v1 = iv + 1 >= 0 && iv + 1 < nv;
-    const unsigned o0 = v0 ? (unsigned)iv * (unsigned)nu + (unsigned)us.st : 0u;
-    const unsigned o1 = v1 ? (unsigned)(iv + 1) * (unsigned)nu + (unsigned)us.st : 0u;
-    const float2 r0 = Storage<TS>::load2(gp, o0, inv);
-    const float2 r1 = Storage<TS>::load2(gp, o1, inv);
-    const float t00 = (v0 && us.a_ok) ? (us.a_hi ? r0.y : r0.x) : 0.0f;
-    const float t01 = (v0 && us.b_ok) ? (us.b_hi ? r0.y : r0.x) : 0.0f;
-    const float t10 = (v1 && us.a_ok) ? (us.a_hi ? r1.y : r1.x) : 0.0f;
-    const float t11 = (v1 && us.b_ok) ? (us.b_hi ? r1.y : r1.x) : 0.0f;
-    const float a = fmaf(us.fu, t01 - t00, t00);
-    const float b = fmaf(us.fu, t11 - t10, t10);
-    return fmaf(fv, b - a, a);
-}
-
 // Tap-quad copy of the sinogram: element (p, vv, uu), vv in [0, nv],
 // uu in [0, nu+1], holds the detector values (u, v) and (u, v+1) with
 // u = uu-1, v = vv-1 (0 off the detector).  Two neighbouring elements are the
@@ -125,35 +59,11 @@ __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in
     out[o + 1] = b;
 }
 
-template <typename TS> struct Quad;
-template <> struct Quad<float> {
-    // t00 = (u,v), t10 = (u,v+1), t01 = (u+1,v), t11 = (u+1,v+1)
-    __device__ __forceinline__ static float4 load(const float* p, unsigned e, float) {
-        float4 r;
-        __builtin_memcpy(&r, p + e, sizeof(r));
-        return r;
-    }
-};
-template <> struct Quad<__half> {
-    __device__ __forceinline__ static float4 load(const __half* p, unsigned e, float inv) {
-        __half h[4];
-        __builtin_memcpy(h, p + e, sizeof(h));
-        return make_float4(__half2float(h[0]) * inv, __half2float(h[1]) * inv, __half2float(h[2]) * inv,
-                           __half2float(h[3]) * inv);
-    }
-};
-template <> struct Quad<__hip_bfloat16> {
-    __device__ __forceinline__ static float4 load(const __hip_bfloat16* p, unsigned e, float inv) {
-        unsigned short h[4];
-        __builtin_memcpy(h, p + e, sizeof(h));
-        return make_float4(__uint_as_float((unsigned)h[0] << 16) * inv, __uint_as_float((unsigned)h[1] << 16) * inv,
-                           __uint_as_float((unsigned)h[2] << 16) * inv, __uint_as_float((unsigned)h[3] << 16) * inv);
-    }
-};
-
-// Forward VI on the tap-quad copy.  Same u, A, v, floor, weights and
-// interpolation order as bilinear(); floor(u) = nu (or floor(v) = nv) has no
-// tap on the detector and reads as 0 there, so the range is [-1, n-1] here.
+// Forward VI on the tap-quad copy.  Bilinear interpolation at (u, v): the
+// taps (u,v), (u,v+1), (u+1,v), (u+1,v+1) of the cell with lower corner
+// (floor u, floor v), 0 off the detector, interpolated along u, then v, with
+// explicit fmaf.  floor(u) = nu (or floor(v) = nv) has no tap on the detector
+// and reads as 0, so the accepted range is [-1, n-1] here.
 template <typename TS, typename TV>
 __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
                                                   TV* __restrict__ vol, float vol_scale) {
@@ -183,7 +93,7 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
             const float vf = floorf(v);
             const bool ok = uok && vf >= -1.0f && vf <= fnv1;
             const unsigned e = ok ? (unsigned)((int)vf + 1) * row + ucol : 0u;
-            const float4 t = Quad<TS>::load(gp, e, sino_inv);
+            const float4 t = Storage<TS>::load4(gp, e, sino_inv);
             const float fv = v - vf;
             const float a = fmaf(fu, t.z - t.x, t.x);
             const float b = fmaf(fu, t.w - t.y, t.y);
@@ -247,8 +157,10 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
     const DevGeom& d = g->dev;
     const size_t es = sino.dtype == TOMO_F32 ? 4 : 2;
-    const size_t quad_elems = (size_t)d.nphi * (d.nv + 1) * (d.nu + 2) * 2;
-    if (quad_elems >= (1ull << 32)) return fail(TOMO_ERR_INVALID_ARG, "sinogram too large for the tap-quad copy");
+    // Offsets inside one view of the copy are 32-bit.
+    const size_t view_elems = (size_t)(d.nv + 1) * (d.nu + 2) * 2;
+    if (view_elems >= (1ull << 32)) return fail(TOMO_ERR_INVALID_ARG, "detector too large for the tap-quad copy");
+    const size_t quad_elems = (size_t)d.nphi * view_elems;
     // Tap-quad copy of the sinogram from the stream-ordered pool, released
     // after the backprojection on the same stream.
     void* quads = nullptr;

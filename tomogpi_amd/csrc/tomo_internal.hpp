@@ -99,7 +99,6 @@ int count_siddon_segments(tomo_ctx* ctx, const tomo_geom* g, uint64_t* out, hipS
 
 int launch_fill(float* x, float v, size_t n, hipStream_t s);
 // out[j + ny*(i + nx*k)] = in[i + nx*(j + ny*k)] (y-fastest copy); 2- or 4-byte elements.
-hipError_t launch_transpose_xy(const void* in, void* out, size_t elem_size, int nx, int ny, int nz, hipStream_t s);
 int launch_diff(const float* a, const float* b, float* out, size_t n, hipStream_t s);
 int launch_scale(float* x, float v, size_t n, hipStream_t s);
 int launch_axpy(float* y, const float* x, float a, int positive, size_t n, hipStream_t s);

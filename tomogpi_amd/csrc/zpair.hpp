@@ -1,0 +1,50 @@
+// zpair.hpp - the z-pair copies of a volume read by the forward projectors.
+//
+// For slabs along A (0: x, 1: y; B is the other in-plane axis) element
+// (a, zz, bb), zz in [0, nz], bb in [0, nB+1], holds f(A=a, B=bb-1, z=zz-1)
+// and f(A=a, B=bb-1, z=zz), 0 outside the grid, with B varying fastest.  Two
+// neighbouring elements are the 2x2 (B, z) block with lower corner
+// (bb-1, zz-1): one 16-byte (FP32) or 8-byte (16-bit) load,
+// Storage<T>::load4 at zpair_index(a, z+1, b+1), returns
+//   (b, z), (b, z+1), (b+1, z), (b+1, z+1)
+// for any corner b in [-1, nB-1], z in [-1, nz-1], with the voxels outside
+// the grid already 0.  qx (A = x) and qy (A = y) each hold
+// nA * (nz+1) * (nB+2) * 2 elements, about twice the volume.
+#pragma once
+
+#include "device_common.hpp"
+
+namespace tomo {
+
+__device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int nB) {
+    return (((size_t)a * (nz + 1) + zz) * (nB + 2) + bb) * 2;
+}
+
+// Element counts of the two copies.
+inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
+inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
+
+// Fills qx and qy from an x-fastest volume of es-byte elements (es 4 or 2).
+hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s);
+
+// Both copies in the stream-ordered pool for the duration of one operator
+// call: alloc() then release() on the same stream.
+struct ZPairs {
+    void* qx = nullptr;
+    void* qy = nullptr;
+    hipError_t alloc(const void* vol, size_t es, int nx, int ny, int nz, hipStream_t s) {
+        hipError_t e = hipMallocAsync(&qx, zpair_elems_x(nx, ny, nz) * es, s);
+        if (e == hipSuccess) e = hipMallocAsync(&qy, zpair_elems_y(nx, ny, nz) * es, s);
+        if (e == hipSuccess) e = launch_zpair(vol, qx, qy, es, nx, ny, nz, s);
+        return e;
+    }
+    hipError_t release(hipStream_t s) {
+        hipError_t e = hipSuccess, e2 = hipSuccess;
+        if (qy) e = hipFreeAsync(qy, s);
+        if (qx) e2 = hipFreeAsync(qx, s);
+        qx = qy = nullptr;
+        return e != hipSuccess ? e : e2;
+    }
+};
+
+}  // namespace tomo
