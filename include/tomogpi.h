@@ -228,6 +228,86 @@ int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* para
                 const float* sinogram, float* volume, tomo_gd_iter_stat* stats,
                 void* stream);
 
+/* ---- serving loop: continuous batching of reconstructions ---------------- */
+/* No counterpart in the reference (it runs one reconstruction per process).
+ * A server owns a worker thread and a stream on the context's device.  Each
+ * request is one simple-gradient reconstruction (the loop of tomo_gd_run,
+ * zero initial volume) of its own geometry and size.  Requests are admitted
+ * in arrival order while the device memory they need fits the budget and a
+ * slot is free; admitted requests keep volumes, sinograms, geometry and loop
+ * scalars resident on the device.  The server advances every admitted request
+ * by one iteration per round, each operator of the round being one shared
+ * persistent launch over all of them (optionally replayed from a captured
+ * hipGraph); requests join and leave between rounds.  A request's arithmetic
+ * does not depend on what shares its launches, so its result is the same
+ * alone and in any mix.  Storage is FP32.                                     */
+typedef struct tomo_server tomo_server;
+
+#define TOMO_SCHEDULE_DIRECT     0 /* the operations of tomo_gd_run          */
+#define TOMO_SCHEDULE_RECURRENT  1 /* P x and L(L x) carried by linearity:
+                                      P x_{k+1} = P x_k + alpha P dJ_k, one
+                                      projection per optimal-step iteration
+                                      fewer (direct form where positivity is
+                                      on or the previous step was fixed)      */
+
+typedef struct tomo_serve_params {
+    uint64_t hbm_budget_bytes;  /* device bytes admitted requests may hold;
+                                   0 = free memory at creation less 1/16    */
+    int32_t  max_active;        /* requests resident at once, 1..64         */
+    int32_t  use_graph;         /* replay each round from a captured graph  */
+    int32_t  schedule;          /* TOMO_SCHEDULE_*                          */
+    int32_t  reserved;
+} tomo_serve_params;
+
+typedef struct tomo_serve_request {
+    tomo_geom_desc geom;
+    tomo_gd_params gd;          /* pair, flags, iterations, Kopt, lambda,
+                                   positivity, alpha_min_init; the flag
+                                   TOMO_FLAG_HOST_PTR makes both pointers host
+                                   pointers                                  */
+    const float* sinogram;      /* measured projections; read at admission  */
+    float*       volume;        /* result, written when the request is done */
+} tomo_serve_request;
+
+#define TOMO_REQ_QUEUED   0
+#define TOMO_REQ_RUNNING  1
+#define TOMO_REQ_DONE     2
+#define TOMO_REQ_FAILED   3
+
+typedef struct tomo_serve_result {
+    int32_t  state;             /* TOMO_REQ_*                               */
+    int32_t  status;            /* TOMO_OK or the failure's status          */
+    double   submit_s;          /* server clock (s since creation)          */
+    double   admit_s;
+    double   done_s;
+    double   latency_s;         /* done - submit                            */
+    double   residual_sq;       /* last iteration's sum (y - P x)^2         */
+    uint64_t device_bytes;      /* device memory held while admitted        */
+} tomo_serve_result;
+
+typedef struct tomo_serve_stats {
+    uint64_t submitted, completed, failed;
+    uint64_t rounds;            /* rounds run                               */
+    int32_t  active, queued, peak_active, reserved;
+    uint64_t bytes_in_use, peak_bytes, budget_bytes;
+    double   latency_p50_s, latency_p99_s, latency_mean_s, latency_max_s;
+    double   first_submit_s, last_done_s;
+    double   throughput_rps;    /* completed / (last_done - first_submit)   */
+} tomo_serve_stats;
+
+int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params, tomo_server** out);
+/* Queues a request; the pointers must stay valid until it is done.          */
+int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req, uint64_t* id);
+int tomo_serve_poll(tomo_server* srv, uint64_t id, tomo_serve_result* res);
+/* Waits until the request is done or failed (TOMO_ERR_BUSY after timeout_s
+ * seconds; a negative timeout waits without limit).                          */
+int tomo_serve_wait(tomo_server* srv, uint64_t id, double timeout_s, tomo_serve_result* res);
+/* Per-iteration records of a finished request (up to n).                    */
+int tomo_serve_iter_stats(tomo_server* srv, uint64_t id, tomo_gd_iter_stat* stats, int n);
+int tomo_serve_get_stats(tomo_server* srv, tomo_serve_stats* st);
+/* Finishes admitted and queued requests, then stops the worker.             */
+int tomo_serve_destroy(tomo_server* srv);
+
 /* ---- phantoms (test and benchmark inputs) -------------------------------- */
 /* 3D Shepp-Logan, modified-SL geometry with densities 2,-.98,-.02,-.02,.01x6
  * (Data3D_0256/phantom3d.m:175-200), voxel centres mapped to [-1,1].         */

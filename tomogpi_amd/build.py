@@ -24,6 +24,7 @@ LIB = os.path.join(PKG, "libtomogpi_mi355x.so")
 CLI_DIR = os.path.join(PKG, "bin")
 CLI = os.path.join(CLI_DIR, "tomogpi")
 ARCH = os.environ.get("TOMOGPI_ARCH", "gfx950")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 
 # -ffp-contract=off + correctly rounded FP32 div/sqrt: the kernels spell out
 # every fused multiply-add, so their FP32 results are reproducible on the CPU.
@@ -32,7 +33,7 @@ COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off", "-Wall", "-Wno-unus
           "-I" + os.path.join(ROOT, "include")]
 DEVICE = ["--offload-arch=" + ARCH, "-fhip-fp32-correctly-rounded-divide-sqrt",
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops"]
-LIB_SOURCES = ["api.cpp", "config.cpp", "loop.cpp", "kernels_vi.hip", "kernels_rs.hip",
+LIB_SOURCES = ["api.cpp", "config.cpp", "loop.cpp", "serve.cpp", "kernels_vi.hip", "kernels_rs.hip",
                "kernels_siddon.hip", "kernels_vec.hip"]
 CLI_SOURCES = ["tomogpi_main.cpp"]
 
@@ -61,6 +62,9 @@ def compile_one(src: str, verbose: bool) -> str:
     cmd = [hipcc()] + COMMON
     if src.endswith(".hip"):
         cmd += DEVICE + ["-x", "hip"]
+    else:
+        # host code: no device pass
+        cmd += ["-x", "c++", "-D__HIP_PLATFORM_AMD__", "-I" + os.path.join(ROCM, "include")]
     cmd += ["-c", path, "-o", obj]
     _run(cmd, verbose)
     return obj

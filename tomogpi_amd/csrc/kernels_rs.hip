@@ -16,6 +16,7 @@
 // for rays parallel to y; the arc-length form here is well conditioned), the
 // samples and the sum are FP32.  The output is overwritten.
 #include "zpair.hpp"
+#include "batch.hpp"
 
 namespace tomo {
 namespace {
@@ -152,12 +153,10 @@ __device__ __forceinline__ int wave_max(int v) {
 // Other waves sample per lane from the y-slab copy.  Grid: x = column group,
 // y = view, z = row group (every view of a row band before the next band).
 template <typename TV, typename TS>
-__global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const TV* __restrict__ qx,
-                                                  const TV* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
-                                                  float sino_scale) {
-    const int m = blockIdx.x * BX + threadIdx.x;
-    const int n = blockIdx.z * BY + threadIdx.y;
-    const int p = blockIdx.y;
+__device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __restrict__ qx, const TV* __restrict__ qy,
+                                        float vol_inv, TS* __restrict__ sino, float sino_scale, int bx, int p, int bz) {
+    const int m = bx * BX + threadIdx.x;
+    const int n = bz * BY + threadIdx.y;
     const bool inside = m < g.nu && n < g.nv;
     RsRay r{0, 0, 0, 0, 0, 0, 0};
     if (inside) r = rs_setup(g, cube, p, m, n);
@@ -208,6 +207,25 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const T
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
         Storage<TS>::store(sino, o, r.lr > 0 ? acc * g.delta : 0.0f, sino_scale);
+    }
+}
+
+template <typename TV, typename TS>
+__global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const TV* __restrict__ qx,
+                                                  const TV* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
+                                                  float sino_scale) {
+    rs_tile<TV, TS>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Shared launch (FP32 storage); item flags bit 0 selects the cube FOV.
+__global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __restrict__ t) {
+    const uint32_t total = t->total;
+    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const uint32_t bx = local % it.gx, rest = local / it.gx;
+        rs_tile<float, float>(it.g, it.flags & 1, (const float*)it.in, (const float*)it.in2, 1.0f, (float*)it.out,
+                              1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
     }
 }
 
@@ -277,6 +295,11 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
     TOMO_HIP(le);
     TOMO_HIP(fe);
     return TOMO_OK;
+}
+
+hipError_t launch_batch_rs(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(rs_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t);
+    return hipGetLastError();
 }
 
 int launch_rs_adjoint(const tomo_geom* g, int cube, Buf sino, MutBuf vol, hipStream_t s) {

@@ -17,6 +17,7 @@
 // first voxel half a voxel behind the entry, last chord not clipped, indices
 // % 256 with unsigned wrap, no z clip.  Defined for 256^3 volumes only.
 #include "zpair.hpp"
+#include "batch.hpp"
 
 namespace tomo {
 namespace {
@@ -288,13 +289,11 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const TV* __rest
 // Infinity Cache hold, instead of streaming the whole volume from HBM once
 // per view.
 template <typename TV, typename TS, bool BUF>
-__global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ vol_x,
-                                                      const TV* __restrict__ qx, const TV* __restrict__ qy,
-                                                      unsigned vol_bytes, float vol_inv, TS* __restrict__ sino,
-                                                      float sino_scale) {
-    const int m = blockIdx.x * BX + threadIdx.x;
-    const int n = blockIdx.z * BY + threadIdx.y;
-    const int p = blockIdx.y;
+__device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restrict__ qx, const TV* __restrict__ qy,
+                                            unsigned qy_bytes, float vol_inv, TS* __restrict__ sino,
+                                            float sino_scale, int bx, int p, int bz) {
+    const int m = bx * BX + threadIdx.x;
+    const int n = bz * BY + threadIdx.y;
     const bool inside = m < g.nu && n < g.nv;
     SidRay r;
     r.ok = false;
@@ -330,20 +329,23 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     float acc = 0.0f;
     if (mode < 0) {
-        const auto rd = make_reader<TV, BUF>(vol_x, vol_bytes, vol_inv);
+        // Per-lane traversal on the y-slab copy, where voxel (x, y, z) is
+        // element 2(x+1) + 2(nx+2)(z+1) + 2(nx+2)(nz+1) y (z-pair index of
+        // corner (x, z) on plane y, first of the pair).
+        const auto rd = make_reader<TV, BUF>(qy, qy_bytes, vol_inv);
         SidState s;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
             s.tn[a] = r.first[a];
             s.cnt[a] = 0.0f;
         }
-        // Element counts are < 2^32 (checked when the geometry is built): the
-        // linear index and its signed steps wrap correctly in 32-bit unsigned.
-        const unsigned s1 = (unsigned)nx, s2 = (unsigned)nx * (unsigned)ny;
-        s.lin = ok ? (unsigned)r.idx[0] + (unsigned)r.idx[1] * s1 + (unsigned)r.idx[2] * s2 : 0u;
-        s.dlin[0] = (unsigned)r.step[0];
-        s.dlin[1] = (unsigned)r.step[1] * s1;
-        s.dlin[2] = (unsigned)r.step[2] * s2;
+        // The copy holds fewer than 2^32 elements (checked by the launcher):
+        // the index and its signed steps wrap correctly in 32-bit unsigned.
+        const unsigned sz = 2u * (unsigned)(nx + 2), sy = sz * (unsigned)(nz + 1);
+        s.lin = ok ? 2u * (unsigned)(r.idx[0] + 1) + sz * (unsigned)(r.idx[2] + 1) + sy * (unsigned)r.idx[1] : 0u;
+        s.dlin[0] = 2u * (unsigned)r.step[0];
+        s.dlin[1] = (unsigned)r.step[1] * sy;
+        s.dlin[2] = (unsigned)r.step[2] * sz;
         s.t = 0.0f;
         s.acc = 0.0f;
         s.live = ok;
@@ -360,6 +362,26 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
         else sid_walk_quads<1, false>(r, qy, vol_inv, ny, nx, nz, ok, acc);
     }
     if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), acc, sino_scale);
+}
+
+template <typename TV, typename TS, bool BUF>
+__global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ qx,
+                                                      const TV* __restrict__ qy, unsigned qy_bytes, float vol_inv,
+                                                      TS* __restrict__ sino, float sino_scale) {
+    siddon_tile<TV, TS, BUF>(g, qx, qy, qy_bytes, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Shared launch: tiles are the single-problem grid's workgroups (column
+// group fastest, then view, then row group); FP32 storage.
+__global__ __launch_bounds__(BX* BY) void siddon_batch_kernel(const BatchTable* __restrict__ t) {
+    const uint32_t total = t->total;
+    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const uint32_t bx = local % it.gx, rest = local / it.gx;
+        siddon_tile<float, float, false>(it.g, (const float*)it.in, (const float*)it.in2, 0u, 1.0f, (float*)it.out,
+                                         1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
+    }
 }
 
 template <typename TS>
@@ -486,20 +508,23 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     }
     // z-pair copies of the volume for x slabs and y slabs (stream-ordered
     // pool allocations, released after the projection on the same stream).
-    const size_t bytes = g->nvox() * (vol.dtype == TOMO_F32 ? 4 : 2);
+    const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
+    const size_t qy_elems = zpair_elems_y(d.nx, d.ny, d.nz);
+    if (qy_elems >= (1ull << 32))
+        return fail(TOMO_ERR_UNSUPPORTED, "volume too large for the 32-bit traversal index of the Siddon projector");
+    const size_t qy_bytes = qy_elems * es;
     ZPairs zp;
-    hipError_t le = zp.alloc(vol.p, vol.dtype == TOMO_F32 ? 4 : 2, d.nx, d.ny, d.nz, s);
+    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s);
     if (le == hipSuccess) {
-        const bool buf = bytes < (1ull << 32);
+        const bool buf = qy_bytes < (1ull << 32);
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)
-                    hipLaunchKernelGGL((siddon_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                       (const TV*)zp.qx, (const TV*)zp.qy, (unsigned)bytes, inv, (TS*)sino.p,
-                                       sino.scale);
+                    hipLaunchKernelGGL((siddon_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, (const TV*)zp.qx,
+                                       (const TV*)zp.qy, (unsigned)qy_bytes, inv, (TS*)sino.p, sino.scale);
                 else
-                    hipLaunchKernelGGL((siddon_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, (const TV*)vol.p,
-                                       (const TV*)zp.qx, (const TV*)zp.qy, 0u, inv, (TS*)sino.p, sino.scale);
+                    hipLaunchKernelGGL((siddon_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, (const TV*)zp.qx,
+                                       (const TV*)zp.qy, 0u, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();
@@ -508,6 +533,11 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     TOMO_HIP(le);
     TOMO_HIP(fe);
     return TOMO_OK;
+}
+
+hipError_t launch_batch_siddon(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(siddon_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t);
+    return hipGetLastError();
 }
 
 int launch_siddon_adjoint(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {

@@ -8,6 +8,7 @@
 // (C/Image3D_CPU.cu:363-390; "L2Norm" without the square root), reduced in a
 // fixed order: per-block partials, then one block.
 #include "zpair.hpp"
+#include "batch.hpp"
 
 namespace tomo {
 namespace {
@@ -80,12 +81,13 @@ __global__ __launch_bounds__(kBlock) void final_sum_kernel(const double* partial
 
 // y = ((row + column) + plane), each pass (2x - previous) - next, zero
 // outside the volume (C/Convolution3D_CPU_GPU.cu:146-148).
-__global__ __launch_bounds__(kBlock) void laplacian_kernel(const float* __restrict__ x, float* __restrict__ y, int nx,
-                                                         int ny, int nz) {
-    const int i = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int k = blockIdx.z;
-    if (i >= nx || j >= ny) return;
+// One tile: 64 x 4 (x, y) voxels of plane k; returns the value written (0
+// outside the volume).
+__device__ __forceinline__ float laplacian_tile(const float* __restrict__ x, float* __restrict__ y, int nx, int ny,
+                                                int nz, int bx, int by, int k) {
+    const int i = bx * 64 + (threadIdx.x & 63);
+    const int j = by * 4 + (threadIdx.x >> 6);
+    if (i >= nx || j >= ny) return 0.0f;
     const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
     const size_t o = (size_t)i + sy * j + sz * k;
     const float c = x[o], c2 = c + c;
@@ -93,7 +95,14 @@ __global__ __launch_bounds__(kBlock) void laplacian_kernel(const float* __restri
     const float ym = j > 0 ? x[o - sy] : 0.0f, yp = j < ny - 1 ? x[o + sy] : 0.0f;
     const float zm = k > 0 ? x[o - sz] : 0.0f, zp = k < nz - 1 ? x[o + sz] : 0.0f;
     const float h = (c2 - xm) - xp, v = (c2 - ym) - yp, p = (c2 - zm) - zp;
-    y[o] = (h + v) + p;
+    const float r = (h + v) + p;
+    y[o] = r;
+    return r;
+}
+
+__global__ __launch_bounds__(kBlock) void laplacian_kernel(const float* __restrict__ x, float* __restrict__ y, int nx,
+                                                         int ny, int nz) {
+    (void)laplacian_tile(x, y, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Ellipsoid phantoms; 16 doubles per ellipsoid: rotation (9, row-major),
@@ -141,10 +150,9 @@ __global__ __launch_bounds__(kBlock) void ellipsoid_kernel(int nx, int ny, int n
 // z-pair copy for y slabs (A = y, B = x): B is also the input's fastest axis,
 // a direct pass.  Grid (ceil((nx+2)/256), ny, nz+1).
 template <typename T>
-__global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
-                                                    int nz) {
-    const int bb = blockIdx.x * 256 + threadIdx.x;
-    const int y = blockIdx.y, zz = blockIdx.z;
+__device__ __forceinline__ void zpair_y_tile(const T* __restrict__ in, T* __restrict__ out, int nx, int ny, int nz,
+                                             int bx, int y, int zz) {
+    const int bb = bx * 256 + threadIdx.x;
     if (bb > nx + 1) return;
     const int x = bb - 1;
     T v[2];
@@ -162,10 +170,9 @@ __global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, 
 // z-pair copy for x slabs (A = x, B = y): a 32x32 (x, B) tile transpose of
 // two z planes through LDS.  Grid (ceil(nx/32), ceil((ny+2)/32), nz+1).
 template <typename T>
-__global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
-                                                    int nz) {
-    __shared__ T tile[2][32][33];
-    const int x0 = blockIdx.x * 32, b0 = blockIdx.y * 32, zz = blockIdx.z;
+__device__ __forceinline__ void zpair_x_tile(const T* __restrict__ in, T* __restrict__ out, int nx, int ny, int nz,
+                                             int bx, int by, int zz, T (*tile)[32][33]) {
+    const int x0 = bx * 32, b0 = by * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (int r = ty; r < 32; r += 8) {
         const int x = x0 + tx, y = b0 + r - 1;
@@ -186,6 +193,204 @@ __global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, 
             const size_t o = zpair_index(x, zz, bb, nz, ny);
             out[o] = tile[0][tx][r];
             out[o + 1] = tile[1][tx][r];
+        }
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
+                                                    int nz) {
+    zpair_y_tile<T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
+                                                    int nz) {
+    __shared__ T tile[2][32][33];
+    zpair_x_tile<T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
+}
+
+// ---- shared launches of the gradient loop (FP32).  Element-wise tiles
+// cover kVecTile consecutive elements, 16 per thread at stride 256; FP64
+// partial sums are per tile, in that fixed order, so a problem's sums do not
+// depend on the batch it runs in.
+
+// Block sum for kernels that loop over tiles: as block_sum, with a trailing
+// barrier so the next tile may reuse the shared slots.
+__device__ __forceinline__ double block_sum_loop(double v) {
+    const double r = block_sum(v);
+    __syncthreads();
+    return r;
+}
+
+#define TOMO_BATCH_LOOP(t)                                                    \
+    for (uint32_t tile = blockIdx.x; tile < (t)->total; tile += gridDim.x)
+
+__device__ __forceinline__ void batch_coords(const BatchItem& it, uint32_t tile, int& bx, int& by, int& bz) {
+    const uint32_t local = tile - it.begin;
+    bx = (int)(local % it.gx);
+    const uint32_t rest = local / it.gx;
+    by = (int)(rest % it.gy);
+    bz = (int)(rest / it.gy);
+}
+
+__global__ __launch_bounds__(256) void zpair_x_batch_kernel(const BatchTable* __restrict__ t) {
+    __shared__ uint32_t tile_lds[2][32][33];
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        int bx, by, bz;
+        batch_coords(it, tile, bx, by, bz);
+        zpair_x_tile<uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by, bz,
+                               tile_lds);
+        __syncthreads();
+    }
+}
+
+__global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        int bx, by, bz;
+        batch_coords(it, tile, bx, by, bz);
+        zpair_y_tile<uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by, bz);
+    }
+}
+
+// out = in - in2; aux[tile] = sum out^2.
+__global__ __launch_bounds__(kBlock) void diff_sq_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const float* a = (const float*)it.in;
+        const float* b = (const float*)it.in2;
+        float* o = (float*)it.out;
+        const uint64_t base = (uint64_t)local * kVecTile;
+        double sq = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < kVecTile / kBlock; ++k) {
+            const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
+            if (e < it.n) {
+                const float v = a[e] - b[e];
+                o[e] = v;
+                sq += (double)v * (double)v;
+            }
+        }
+        const double r = block_sum_loop(sq);
+        if (threadIdx.x == 0) ((double*)it.aux)[local] = r;
+    }
+}
+
+// aux[tile] = sum in^2.
+__global__ __launch_bounds__(kBlock) void sq_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const float* a = (const float*)it.in;
+        const uint64_t base = (uint64_t)local * kVecTile;
+        double sq = 0.0;
+#pragma unroll 4
+        for (int k = 0; k < kVecTile / kBlock; ++k) {
+            const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
+            if (e < it.n) {
+                const double v = (double)a[e];
+                sq += v * v;
+            }
+        }
+        const double r = block_sum_loop(sq);
+        if (threadIdx.x == 0) ((double*)it.aux)[local] = r;
+    }
+}
+
+// out = L in over tiles of 64 x 4 voxels of one plane; aux[tile] = sum out^2
+// when aux is set.
+__global__ __launch_bounds__(kBlock) void laplacian_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        int bx, by, bz;
+        batch_coords(it, tile, bx, by, bz);
+        const float v = laplacian_tile((const float*)it.in, (float*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by, bz);
+        if (it.aux) {
+            const double r = block_sum_loop((double)v * (double)v);
+            if (threadIdx.x == 0) ((double*)it.aux)[tile - it.begin] = r;
+        }
+    }
+}
+
+// out = out + a * in (multiply then add, as the reference host loop), a =
+// (float) slot[alpha] when flags bit 0 is set, else the item's constant;
+// flags bit 1 clamps the result at 0; flags bit 2 writes out = a instead.
+__global__ __launch_bounds__(kBlock) void axpy_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const float a = (it.flags & 1) ? (float)it.slot[kSlotAlpha] : it.a;
+        const bool pos = (it.flags & 2) != 0;
+        const bool fill = (it.flags & 4) != 0;
+        const float* x = (const float*)it.in;
+        float* y = (float*)it.out;
+        const uint64_t base = (uint64_t)local * kVecTile;
+        for (int k = 0; k < kVecTile / kBlock; ++k) {
+            const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
+            if (e < it.n) {
+                float v = fill ? a : y[e] + a * x[e];
+                if (pos && v < 0.0f) v = 0.0f;
+                y[e] = v;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void fill_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        float* y = (float*)it.out;
+        const uint64_t base = (uint64_t)local * kVecTile;
+        for (int k = 0; k < kVecTile / kBlock; ++k) {
+            const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
+            if (e < it.n) y[e] = it.a;
+        }
+    }
+}
+
+// slot[flags] = sum of the n partials in aux (one tile per item).
+__global__ __launch_bounds__(kBlock) void finish_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const double* part = (const double*)it.aux;
+        double s = 0.0;
+        for (uint64_t i = threadIdx.x; i < it.n; i += kBlock) s += part[i];
+        const double r = block_sum_loop(s);
+        if (threadIdx.x == 0) it.slot[it.flags] = r;
+    }
+}
+
+// Step rule of the simple-gradient loop on the problem's scalars (one tile
+// per item; d = lambda, flags bit 0 = optimal step this iteration).  aux,
+// when set, receives (residual, norm dJ, norm dJ proj, alpha) at entry n.
+__global__ __launch_bounds__(64) void step_batch_kernel(const BatchTable* __restrict__ t) {
+    TOMO_BATCH_LOOP(t) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        if (threadIdx.x != 0) continue;
+        double* sl = it.slot;
+        double alpha;
+        if (it.flags & 1) {
+            const double norm_reg = it.d != 0.0 ? 2.0 * sl[kSlotRegSq] : 0.0;
+            const double ndp = 2.0 * sl[kSlotProjSq] + it.d * norm_reg;
+            alpha = -sl[kSlotNormDj] / ndp;
+            sl[kSlotAlphaMin] = fmax(alpha, sl[kSlotAlphaMin]);
+            sl[kSlotNormDjProj] = ndp;
+        } else {
+            alpha = sl[kSlotAlphaMin];
+            sl[kSlotNormDj] = 0.0;
+            sl[kSlotNormDjProj] = 0.0;
+        }
+        sl[kSlotAlpha] = alpha;
+        if (it.aux) {
+            double* h = (double*)it.aux + 4 * it.n;
+            h[0] = sl[kSlotResidual];
+            h[1] = sl[kSlotNormDj];
+            h[2] = sl[kSlotNormDjProj];
+            h[3] = alpha;
         }
     }
 }
@@ -213,6 +418,43 @@ hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, 
         hipLaunchKernelGGL((zpair_y_kernel<uint16_t>), gy, dim3(256), 0, s, (const uint16_t*)vol, (uint16_t*)qy, nx,
                            ny, nz);
     }
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(zpair_x_batch_kernel, dim3(kBatchGrid), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(zpair_y_batch_kernel, dim3(kBatchGrid), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_diff_sq(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(diff_sq_batch_kernel, dim3(kBatchGrid), dim3(kBlock), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_sq(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(sq_batch_kernel, dim3(kBatchGrid), dim3(kBlock), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_laplacian(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(laplacian_batch_kernel, dim3(kBatchGrid), dim3(kBlock), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_axpy(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(axpy_batch_kernel, dim3(kBatchGrid), dim3(kBlock), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_fill(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(fill_batch_kernel, dim3(kBatchGrid), dim3(kBlock), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_finish(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(finish_batch_kernel, dim3(kBatchMax), dim3(kBlock), 0, s, t);
+    return hipGetLastError();
+}
+hipError_t launch_batch_step(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(step_batch_kernel, dim3(kBatchMax), dim3(64), 0, s, t);
     return hipGetLastError();
 }
 

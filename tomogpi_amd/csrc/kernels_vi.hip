@@ -13,6 +13,7 @@
 // terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
 // and the per-voxel v, bilinear weights and sum in FP32.
 #include "device_common.hpp"
+#include "batch.hpp"
 
 namespace tomo {
 namespace {
@@ -40,11 +41,10 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
 // (FP32) or 8-byte (16-bit) load serves one voxel-view update, and the zero
 // border replaces the per-tap range tests.  Values are copied as stored.
 template <typename TS>
-__global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
-                                                     int nv) {
-    const int uu = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int vv = blockIdx.y * 4 + (threadIdx.x >> 6);
-    const int p = blockIdx.z;
+__device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __restrict__ out, int nu, int nv, int bx,
+                                              int by, int p) {
+    const int uu = bx * 64 + (threadIdx.x & 63);
+    const int vv = by * 4 + (threadIdx.x >> 6);
     if (uu > nu + 1 || vv > nv) return;
     const int u = uu - 1, v = vv - 1;
     const bool uok = u >= 0 && u < nu;
@@ -65,11 +65,11 @@ __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in
 // explicit fmaf.  floor(u) = nu (or floor(v) = nv) has no tap on the detector
 // and reads as 0, so the accepted range is [-1, n-1] here.
 template <typename TS, typename TV>
-__global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
-                                                  TV* __restrict__ vol, float vol_scale) {
-    const int i = blockIdx.x * TX + (threadIdx.x % TX);
-    const int j = blockIdx.y * TY + (threadIdx.x / TX);
-    const int k0 = blockIdx.z * ZB;
+__device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__ quads, float sino_inv,
+                                        TV* __restrict__ vol, float vol_scale, float post, int bx, int by, int bz) {
+    const int i = bx * TX + (threadIdx.x % TX);
+    const int j = by * TY + (threadIdx.x / TX);
+    const int k0 = bz * ZB;
     if (i >= g.nx || j >= g.ny) return;
     const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
     const float zp0 = (float)((double)k0 - (double)g.cz);
@@ -104,7 +104,44 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
     const size_t base = (size_t)i + (size_t)g.nx * j;
 #pragma unroll
     for (int k = 0; k < ZB; ++k)
-        if (k0 + k < g.nz) Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, acc[k] * g.delta, vol_scale);
+        if (k0 + k < g.nz) Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, (acc[k] * g.delta) * post, vol_scale);
+}
+
+template <typename TS>
+__global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
+                                                     int nv) {
+    vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+template <typename TS, typename TV>
+__global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
+                                                  TV* __restrict__ vol, float vol_scale) {
+    vi_tile<TS, TV>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// Shared launches (FP32 storage).  Tap quads: tile grid (ceil((nu+2)/64),
+// ceil((nv+1)/4), nphi).  VI: tile grid (ceil(nx/16), ceil(ny/16),
+// ceil(nz/16)); the stored value is (delta * sum) * a.
+__global__ __launch_bounds__(256) void vi_quads_batch_kernel(const BatchTable* __restrict__ t) {
+    const uint32_t total = t->total;
+    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const uint32_t bx = local % it.gx, rest = local / it.gx;
+        vi_quads_tile<float>((const float*)it.in, (float*)it.out, it.g.nu, it.g.nv, (int)bx, (int)(rest % it.gy),
+                             (int)(rest / it.gy));
+    }
+}
+
+__global__ __launch_bounds__(TX* TY) void vi_batch_kernel(const BatchTable* __restrict__ t) {
+    const uint32_t total = t->total;
+    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+        const BatchItem& it = t->item[batch_find(t, tile)];
+        const uint32_t local = tile - it.begin;
+        const uint32_t bx = local % it.gx, rest = local / it.gx;
+        vi_tile<float, float>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx,
+                              (int)(rest % it.gy), (int)(rest / it.gy));
+    }
 }
 
 // VI^T: g_p[tap] += delta * w_tap * f(i,j,k) (FP32 atomics, output zeroed by
@@ -183,6 +220,16 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
     TOMO_HIP(hipFreeAsync(quads, s));
     TOMO_HIP(le);
     return TOMO_OK;
+}
+
+hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(vi_quads_batch_kernel, dim3(kBatchGrid), dim3(256), 0, s, t);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_vi(const BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(TX * TY), 0, s, t);
+    return hipGetLastError();
 }
 
 int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s) {

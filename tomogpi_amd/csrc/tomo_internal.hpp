@@ -69,6 +69,11 @@ struct tomo_geom {
 
 namespace tomo {
 
+// Element counts of the z-pair copies of a volume for x slabs (qx) and y
+// slabs (qy): nA * (nz+1) * (nB+2) pairs.
+inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
+inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
+
 // Derives the geometry exactly as the reference constructors do.
 int derive_geometry(const tomo_geom_desc& d, tomo_geom* g);
 
@@ -98,7 +103,6 @@ int count_rs_samples(tomo_ctx* ctx, const tomo_geom* g, int cube, uint64_t* out,
 int count_siddon_segments(tomo_ctx* ctx, const tomo_geom* g, uint64_t* out, hipStream_t s);
 
 int launch_fill(float* x, float v, size_t n, hipStream_t s);
-// out[j + ny*(i + nx*k)] = in[i + nx*(j + ny*k)] (y-fastest copy); 2- or 4-byte elements.
 int launch_diff(const float* a, const float* b, float* out, size_t n, hipStream_t s);
 int launch_scale(float* x, float v, size_t n, hipStream_t s);
 int launch_axpy(float* y, const float* x, float a, int positive, size_t n, hipStream_t s);

@@ -20,10 +20,6 @@ __device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int
     return (((size_t)a * (nz + 1) + zz) * (nB + 2) + bb) * 2;
 }
 
-// Element counts of the two copies.
-inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
-inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
-
 // Fills qx and qy from an x-fastest volume of es-byte elements (es 4 or 2).
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s);
 

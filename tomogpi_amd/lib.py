@@ -24,6 +24,8 @@ F32, F16, BF16, F64 = 0, 1, 2, 3
 PROJ_RS, PROJ_SIDDON, BP_VI = 0, 1, 0
 FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR = 0x1, 0x2, 0x4
 PAIR_RSVI, PAIR_SIDDONVI = 0, 1
+SCHEDULE_DIRECT, SCHEDULE_RECURRENT = 0, 1
+REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 # Every symbol include/tomogpi.h declares.
 EXPORTS = [
@@ -33,7 +35,8 @@ EXPORTS = [
     "tomo_geom_get_info", "tomo_geom_get_phi", "tomo_project", "tomo_backproject", "tomo_project_adjoint",
     "tomo_backproject_adjoint", "tomo_project_work", "tomo_convert", "tomo_diff", "tomo_scale", "tomo_axpy",
     "tomo_sq_norm", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_phantom_shepp_logan",
-    "tomo_phantom_ellipsoids", "tomo_file_read", "tomo_file_write",
+    "tomo_phantom_ellipsoids", "tomo_file_read", "tomo_file_write", "tomo_serve_create", "tomo_serve_submit",
+    "tomo_serve_poll", "tomo_serve_wait", "tomo_serve_iter_stats", "tomo_serve_get_stats", "tomo_serve_destroy",
 ]
 
 
@@ -78,6 +81,30 @@ class GdIterStat(C.Structure):
                 ("alpha", C.c_double)]
 
 
+class ServeParams(C.Structure):
+    _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("use_graph", C.c_int32),
+                ("schedule", C.c_int32), ("reserved", C.c_int32)]
+
+
+class ServeRequest(C.Structure):
+    _fields_ = [("geom", GeomDesc), ("gd", GdParams), ("sinogram", C.c_void_p), ("volume", C.c_void_p)]
+
+
+class ServeResult(C.Structure):
+    _fields_ = [("state", C.c_int32), ("status", C.c_int32), ("submit_s", C.c_double), ("admit_s", C.c_double),
+                ("done_s", C.c_double), ("latency_s", C.c_double), ("residual_sq", C.c_double),
+                ("device_bytes", C.c_uint64)]
+
+
+class ServeStats(C.Structure):
+    _fields_ = [("submitted", C.c_uint64), ("completed", C.c_uint64), ("failed", C.c_uint64),
+                ("rounds", C.c_uint64), ("active", C.c_int32), ("queued", C.c_int32), ("peak_active", C.c_int32),
+                ("reserved", C.c_int32), ("bytes_in_use", C.c_uint64), ("peak_bytes", C.c_uint64),
+                ("budget_bytes", C.c_uint64), ("latency_p50_s", C.c_double), ("latency_p99_s", C.c_double),
+                ("latency_mean_s", C.c_double), ("latency_max_s", C.c_double), ("first_submit_s", C.c_double),
+                ("last_done_s", C.c_double), ("throughput_rps", C.c_double)]
+
+
 _lib = None
 
 
@@ -116,6 +143,13 @@ def load() -> C.CDLL:
         "tomo_phantom_shepp_logan": ([vp, vp, vp, vp], i),
         "tomo_phantom_ellipsoids": ([vp, vp, i, C.c_uint64, d, d, vp, vp], i),
         "tomo_file_read": ([C.c_char_p, vp, i, sz], i), "tomo_file_write": ([C.c_char_p, vp, i, sz], i),
+        "tomo_serve_create": ([vp, C.POINTER(ServeParams), C.POINTER(vp)], i),
+        "tomo_serve_submit": ([vp, C.POINTER(ServeRequest), C.POINTER(C.c_uint64)], i),
+        "tomo_serve_poll": ([vp, C.c_uint64, C.POINTER(ServeResult)], i),
+        "tomo_serve_wait": ([vp, C.c_uint64, d, C.POINTER(ServeResult)], i),
+        "tomo_serve_iter_stats": ([vp, C.c_uint64, C.POINTER(GdIterStat), i], i),
+        "tomo_serve_get_stats": ([vp, C.POINTER(ServeStats)], i),
+        "tomo_serve_destroy": ([vp], i),
     }
     for name, (args, res) in sig.items():
         fn = getattr(lib, name)
@@ -347,6 +381,73 @@ def dot(ctx: Context, a, b, stream=None) -> float:
     check(ctx.lib.tomo_dot(ctx.h, C.c_void_p(_ptr(a)[0]), C.c_void_p(_ptr(b)[0]), a.numel(), C.byref(out),
                            C.c_void_p(_stream(stream))))
     return out.value
+
+
+def _struct_dict(st) -> dict:
+    return {f[0]: getattr(st, f[0]) for f in st._fields_ if f[0] != "reserved"}
+
+
+class Server:
+    """Serving loop (tomo_server): continuous batching of simple-gradient
+    reconstructions of uneven sizes on one device."""
+
+    def __init__(self, ctx: Context, budget_bytes: int = 0, max_active: int = 16, use_graph: bool = True,
+                 schedule: int = SCHEDULE_DIRECT):
+        self.ctx, self.lib = ctx, ctx.lib
+        p = ServeParams(hbm_budget_bytes=budget_bytes, max_active=max_active, use_graph=int(use_graph),
+                        schedule=schedule)
+        h = C.c_void_p()
+        check(self.lib.tomo_serve_create(ctx.h, C.byref(p), C.byref(h)))
+        self.h = h
+        self._keep = {}  # request id -> arrays that must outlive the request
+
+    def submit(self, desc: AcquisitionDesc, sino, vol, pair=PAIR_RSVI, flags=0, iters=20, kopt=21, lam=0.01,
+               positivity=0, alpha_min_init=-0.0347711) -> int:
+        ps, hs = _ptr(sino)
+        pv, hv = _ptr(vol)
+        if hs != hv:
+            raise ValueError("sinogram and volume must both be host or both be device arrays")
+        gd = GdParams(pair=pair, flags=flags | (FLAG_HOST_PTR if hs else 0), gradient_iteration_nb=iters,
+                      optimal_step_iteration_nb=kopt, lam=lam, positivity=positivity, alpha_min_init=alpha_min_init)
+        r = ServeRequest(geom=desc.to_c(), gd=gd, sinogram=ps, volume=pv)
+        rid = C.c_uint64()
+        check(self.lib.tomo_serve_submit(self.h, C.byref(r), C.byref(rid)))
+        self._keep[rid.value] = (sino, vol)
+        return rid.value
+
+    def poll(self, rid: int) -> dict:
+        r = ServeResult()
+        check(self.lib.tomo_serve_poll(self.h, rid, C.byref(r)))
+        return _struct_dict(r)
+
+    def wait(self, rid: int, timeout: float = -1.0) -> dict:
+        r = ServeResult()
+        check(self.lib.tomo_serve_wait(self.h, rid, timeout, C.byref(r)))
+        self._keep.pop(rid, None)
+        return _struct_dict(r)
+
+    def iter_stats(self, rid: int, n: int) -> list:
+        st = (GdIterStat * max(n, 1))()
+        check(self.lib.tomo_serve_iter_stats(self.h, rid, st, n))
+        return [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj, alpha=x.alpha)
+                for x in st[:n]]
+
+    def stats(self) -> dict:
+        st = ServeStats()
+        check(self.lib.tomo_serve_get_stats(self.h, C.byref(st)))
+        return _struct_dict(st)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.tomo_serve_destroy(self.h)
+            self.h = None
+            self._keep.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---- mirror of the reference operator classes ------------------------------
