@@ -59,6 +59,44 @@ template <> struct Storage<__hip_bfloat16> {
     }
 };
 
+// Raw (untyped) buffer access: descriptor word 3 for gfx9-family parts.  A
+// buffer resource covers `bytes` bytes (below 4 GiB) from `p`; loads take a
+// per-lane 32-bit byte offset plus a wave-uniform one, and read 0 past the
+// end.
+constexpr int kBufferWord3 = 0x00020000;
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void* p, uint32_t bytes) {
+    return __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, kBufferWord3);
+}
+
+// Four consecutive elements at a byte offset, as float (value * inv).
+template <typename T> struct BufLoad4;
+template <> struct BufLoad4<float> {
+    __device__ __forceinline__ static float4 load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, (int)soff, 0);
+        return make_float4(__uint_as_float(v[0]), __uint_as_float(v[1]), __uint_as_float(v[2]),
+                           __uint_as_float(v[3]));
+    }
+};
+template <> struct BufLoad4<__half> {
+    __device__ __forceinline__ static float4 load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float inv) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0);
+        const __half h0 = __builtin_bit_cast(__half, (unsigned short)(v[0] & 0xffffu));
+        const __half h1 = __builtin_bit_cast(__half, (unsigned short)(v[0] >> 16));
+        const __half h2 = __builtin_bit_cast(__half, (unsigned short)(v[1] & 0xffffu));
+        const __half h3 = __builtin_bit_cast(__half, (unsigned short)(v[1] >> 16));
+        return make_float4(__half2float(h0) * inv, __half2float(h1) * inv, __half2float(h2) * inv,
+                           __half2float(h3) * inv);
+    }
+};
+template <> struct BufLoad4<__hip_bfloat16> {
+    __device__ __forceinline__ static float4 load(__amdgpu_buffer_rsrc_t r, uint32_t off, uint32_t soff, float inv) {
+        const auto v = __builtin_amdgcn_raw_buffer_load_b64(r, (int)off, (int)soff, 0);
+        return make_float4(__uint_as_float(v[0] << 16) * inv, __uint_as_float(v[0] & 0xffff0000u) * inv,
+                           __uint_as_float(v[1] << 16) * inv, __uint_as_float(v[1] & 0xffff0000u) * inv);
+    }
+};
+
 // Predicated read: clamped address (always in bounds) then select.
 template <typename T>
 __device__ __forceinline__ float tap(const T* p, int64_t idx, bool ok, float inv) {

@@ -167,14 +167,11 @@ template <> struct VolReader<__hip_bfloat16, true> {
     }
 };
 
-// Descriptor word 3 for raw (untyped) buffer access on gfx9-family parts.
-constexpr int kBufferWord3 = 0x00020000;
-
 template <typename TV, bool BUF>
 __device__ __forceinline__ VolReader<TV, BUF> make_reader(const TV* p, unsigned bytes, float inv) {
     VolReader<TV, BUF> v;
     if constexpr (BUF) {
-        v.r = __builtin_amdgcn_make_buffer_rsrc((void*)p, (short)0, (int)bytes, kBufferWord3);
+        v.r = buffer_rsrc(p, bytes);
     } else {
         v.p = p;
         (void)bytes;
@@ -225,25 +222,56 @@ __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD&
 // tmax.  These are the segments of the general step in the same order; an
 // absent one has zero length and adds exactly 0.  ZNEG: the wave's rays go
 // down in z (one detector row, one sign).
-template <int A, bool ZNEG, typename TV>
-__device__ __forceinline__ void sid_walk_quads(const SidRay& r, const TV* __restrict__ q, float inv, int nA, int nB,
-                                               int nz, bool live, float& acc_out) {
+// Quad reads of a z-pair copy by element index: a buffer resource with a
+// 32-bit byte offset (copies below 4 GiB) or a plain global load.
+template <typename TV, bool BUF> struct QuadReader;
+template <typename TV> struct QuadReader<TV, false> {
+    const TV* p;
+    float inv;
+    __device__ __forceinline__ float4 operator()(uint32_t e) const { return Storage<TV>::load4(p, e, inv); }
+};
+template <typename TV> struct QuadReader<TV, true> {
+    __amdgpu_buffer_rsrc_t r;
+    float inv;
+    __device__ __forceinline__ float4 operator()(uint32_t e) const {
+        return BufLoad4<TV>::load(r, e * (uint32_t)sizeof(TV), 0u, inv);
+    }
+};
+template <typename TV, bool BUF>
+__device__ __forceinline__ QuadReader<TV, BUF> make_quad_reader(const TV* p, uint32_t bytes, float inv) {
+    QuadReader<TV, BUF> q;
+    if constexpr (BUF) q.r = buffer_rsrc(p, bytes);
+    else q.p = p;
+    q.inv = inv;
+    return q;
+}
+
+template <int A, bool ZNEG, typename QR>
+__device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, int nA, int nB, int nz, bool live,
+                                               float& acc_out) {
     constexpr int B = 1 - A;
     const int sA = r.step[A], sB = r.step[B];
     const bool bneg = sB < 0;
     float tA = r.first[A], tB = r.first[B], tz = r.first[2];
     float cA = 0.0f, cB = 0.0f, cz = 0.0f;
-    int iA = r.idx[A], iB = r.idx[B], iz = r.idx[2];
+    int iA = r.idx[A];
     float t = 0.0f, acc = 0.0f;
+    // Element index of the block corner (A = iA, B = iB or iB - 1, z = iz or
+    // iz - 1), carried by the steps.  A live lane's voxel is inside the grid
+    // (tmax stops at the first crossing that leaves it), so the corner is in
+    // the copy; the index of a finished lane is never used.
+    const uint32_t sz_e = 2u * (uint32_t)(nB + 2), sa_e = sz_e * (uint32_t)(nz + 1);
+    uint32_t e = 0u;
+    if (live) {
+        const int bb = (bneg ? r.idx[B] - 1 : r.idx[B]) + 1;
+        const int zz = (ZNEG ? r.idx[2] - 1 : r.idx[2]) + 1;
+        e = (uint32_t)iA * sa_e + (uint32_t)zz * sz_e + 2u * (uint32_t)bb;
+    }
+    const uint32_t dA = (uint32_t)sA * sa_e, dB = 2u * (uint32_t)sB, dZ = (uint32_t)r.step[2] * sz_e;
     int I = sA > 0 ? wave_min(live ? iA : 0x7fffffff) : wave_max(live ? iA : -0x7fffffff);
     for (int outer = 0; outer <= nA + 1 && __any(live); ++outer) {
         if (live && iA == I) {
-            // Block corner; live lanes are inside the grid, the clamps only
-            // keep every address inside the copy.
-            const int bq = min(max(bneg ? iB - 1 : iB, -1), nB - 1);
-            const int zq = min(max(ZNEG ? iz - 1 : iz, -1), nz - 1);
-            const int ia = min(max(iA, 0), nA - 1);
-            const float4 v = Storage<TV>::load4(q, zpair_index(ia, zq + 1, bq + 1, nz, nB), inv);
+            const float4 v = rd(e);
             // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
             const float V0 = bneg ? (ZNEG ? v.w : v.z) : (ZNEG ? v.y : v.x);
             const float Vb = bneg ? (ZNEG ? v.y : v.x) : (ZNEG ? v.w : v.z);
@@ -260,9 +288,10 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const TV* __rest
             acc = fmaf(c3 - c2, V2, acc);
             t = c3;
             const bool stepB = hb || tB == tA, stepZ = hz || tz == tA;
-            if (stepB) { iB += sB; cB += 1.0f; tB = fmaf(cB, r.inc[B], r.first[B]); }
-            if (stepZ) { iz += r.step[2]; cz += 1.0f; tz = fmaf(cz, r.inc[2], r.first[2]); }
+            if (stepB) { e += dB; cB += 1.0f; tB = fmaf(cB, r.inc[B], r.first[B]); }
+            if (stepZ) { e += dZ; cz += 1.0f; tz = fmaf(cz, r.inc[2], r.first[2]); }
             iA += sA;
+            e += dA;
             cA += 1.0f;
             tA = fmaf(cA, r.inc[A], r.first[A]);
             live = t < r.tmax;
@@ -290,8 +319,8 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const TV* __rest
 // per view.
 template <typename TV, typename TS, bool BUF>
 __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restrict__ qx, const TV* __restrict__ qy,
-                                            unsigned qy_bytes, float vol_inv, TS* __restrict__ sino,
-                                            float sino_scale, int bx, int p, int bz) {
+                                            unsigned qx_bytes, unsigned qy_bytes, float vol_inv,
+                                            TS* __restrict__ sino, float sino_scale, int bx, int p, int bz) {
     const int m = bx * BX + threadIdx.x;
     const int n = bz * BY + threadIdx.y;
     const bool inside = m < g.nu && n < g.nv;
@@ -355,20 +384,24 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
         while (s.live && guard-- > 0) sid_step(s, r, rd);
         acc = s.acc;
     } else if (mode < 2) {
-        if (zneg_all && !zpos_all) sid_walk_quads<0, true>(r, qx, vol_inv, nx, ny, nz, ok, acc);
-        else sid_walk_quads<0, false>(r, qx, vol_inv, nx, ny, nz, ok, acc);
+        const auto rq = make_quad_reader<TV, BUF>(qx, qx_bytes, vol_inv);
+        if (zneg_all && !zpos_all) sid_walk_quads<0, true>(r, rq, nx, ny, nz, ok, acc);
+        else sid_walk_quads<0, false>(r, rq, nx, ny, nz, ok, acc);
     } else {
-        if (zneg_all && !zpos_all) sid_walk_quads<1, true>(r, qy, vol_inv, ny, nx, nz, ok, acc);
-        else sid_walk_quads<1, false>(r, qy, vol_inv, ny, nx, nz, ok, acc);
+        const auto rq = make_quad_reader<TV, BUF>(qy, qy_bytes, vol_inv);
+        if (zneg_all && !zpos_all) sid_walk_quads<1, true>(r, rq, ny, nx, nz, ok, acc);
+        else sid_walk_quads<1, false>(r, rq, ny, nx, nz, ok, acc);
     }
     if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), acc, sino_scale);
 }
 
 template <typename TV, typename TS, bool BUF>
 __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ qx,
-                                                      const TV* __restrict__ qy, unsigned qy_bytes, float vol_inv,
-                                                      TS* __restrict__ sino, float sino_scale) {
-    siddon_tile<TV, TS, BUF>(g, qx, qy, qy_bytes, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
+                                                      const TV* __restrict__ qy, unsigned qx_bytes,
+                                                      unsigned qy_bytes, float vol_inv, TS* __restrict__ sino,
+                                                      float sino_scale) {
+    siddon_tile<TV, TS, BUF>(g, qx, qy, qx_bytes, qy_bytes, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y,
+                             blockIdx.z);
 }
 
 // Shared launch: tiles are the single-problem grid's workgroups (column
@@ -379,8 +412,8 @@ __global__ __launch_bounds__(BX* BY) void siddon_batch_kernel(const BatchTable* 
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        siddon_tile<float, float, false>(it.g, (const float*)it.in, (const float*)it.in2, 0u, 1.0f, (float*)it.out,
-                                         1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
+        siddon_tile<float, float, false>(it.g, (const float*)it.in, (const float*)it.in2, 0u, 0u, 1.0f,
+                                         (float*)it.out, 1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
     }
 }
 
@@ -509,22 +542,23 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     // z-pair copies of the volume for x slabs and y slabs (stream-ordered
     // pool allocations, released after the projection on the same stream).
     const size_t es = vol.dtype == TOMO_F32 ? 4 : 2;
-    const size_t qy_elems = zpair_elems_y(d.nx, d.ny, d.nz);
-    if (qy_elems >= (1ull << 32))
+    const size_t qx_elems = zpair_elems_x(d.nx, d.ny, d.nz), qy_elems = zpair_elems_y(d.nx, d.ny, d.nz);
+    if (qx_elems >= (1ull << 32) || qy_elems >= (1ull << 32))
         return fail(TOMO_ERR_UNSUPPORTED, "volume too large for the 32-bit traversal index of the Siddon projector");
-    const size_t qy_bytes = qy_elems * es;
+    const size_t qx_bytes = qx_elems * es, qy_bytes = qy_elems * es;
     ZPairs zp;
     hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s);
     if (le == hipSuccess) {
-        const bool buf = qy_bytes < (1ull << 32);
+        const bool buf = qx_bytes < (1ull << 32) && qy_bytes < (1ull << 32);
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)
                     hipLaunchKernelGGL((siddon_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, (const TV*)zp.qx,
-                                       (const TV*)zp.qy, (unsigned)qy_bytes, inv, (TS*)sino.p, sino.scale);
+                                       (const TV*)zp.qy, (unsigned)qx_bytes, (unsigned)qy_bytes, inv, (TS*)sino.p,
+                                       sino.scale);
                 else
                     hipLaunchKernelGGL((siddon_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, (const TV*)zp.qx,
-                                       (const TV*)zp.qy, 0u, inv, (TS*)sino.p, sino.scale);
+                                       (const TV*)zp.qy, 0u, 0u, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();

@@ -34,19 +34,33 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
     return r;
 }
 
-// Tap-quad copy of the sinogram: element (p, vv, uu), vv in [0, nv],
-// uu in [0, nu+1], holds the detector values (u, v) and (u, v+1) with
-// u = uu-1, v = vv-1 (0 off the detector).  Two neighbouring elements are the
+// Tap-quad copy of the sinogram: element (p, vv, uu), vv in [0, nv+2],
+// uu in [0, nu+3], holds the detector values (u, v) and (u, v+1) with
+// u = uu-2, v = vv-2 (0 off the detector).  Two neighbouring elements are the
 // four bilinear taps of the cell with lower corner (u, v), so one 16-byte
-// (FP32) or 8-byte (16-bit) load serves one voxel-view update, and the zero
-// border replaces the per-tap range tests.  Values are copied as stored.
+// (FP32) or 8-byte (16-bit) load serves one voxel-view update.  Cells with a
+// corner in [-2, nu] x [-2, nv] are in the copy; a cell whose corner is
+// outside [-1, n-1] on either axis has no tap on the detector, and clamping
+// its corner into [-2, n] lands on another such cell, all zeros, which
+// replaces the per-tap range tests.  Values are copied as stored.
+struct QuadShape {
+    uint32_t row;        // elements per row of the copy: 2 (nu + 4)
+    uint32_t view;       // elements per view: row (nv + 3)
+};
+__host__ __device__ __forceinline__ QuadShape quad_shape(int nu, int nv) {
+    QuadShape q;
+    q.row = 2u * (uint32_t)(nu + 4);
+    q.view = q.row * (uint32_t)(nv + 3);
+    return q;
+}
+
 template <typename TS>
 __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __restrict__ out, int nu, int nv, int bx,
                                               int by, int p) {
     const int uu = bx * 64 + (threadIdx.x & 63);
     const int vv = by * 4 + (threadIdx.x >> 6);
-    if (uu > nu + 1 || vv > nv) return;
-    const int u = uu - 1, v = vv - 1;
+    if (uu > nu + 3 || vv > nv + 2) return;
+    const int u = uu - 2, v = vv - 2;
     const bool uok = u >= 0 && u < nu;
     const TS* gp = in + (size_t)p * nu * nv;
     TS a, b;
@@ -54,7 +68,8 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
     __builtin_memset(&b, 0, sizeof(b));
     if (uok && v >= 0 && v < nv) a = gp[(size_t)v * nu + u];
     if (uok && v + 1 >= 0 && v + 1 < nv) b = gp[(size_t)(v + 1) * nu + u];
-    const size_t o = (((size_t)p * (nv + 1) + vv) * (nu + 2) + uu) * 2;
+    const QuadShape qs = quad_shape(nu, nv);
+    const size_t o = (size_t)p * qs.view + (size_t)vv * qs.row + 2u * (uint32_t)uu;
     out[o] = a;
     out[o + 1] = b;
 }
@@ -62,8 +77,10 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
 // Forward VI on the tap-quad copy.  Bilinear interpolation at (u, v): the
 // taps (u,v), (u,v+1), (u+1,v), (u+1,v+1) of the cell with lower corner
 // (floor u, floor v), 0 off the detector, interpolated along u, then v, with
-// explicit fmaf.  floor(u) = nu (or floor(v) = nv) has no tap on the detector
-// and reads as 0, so the accepted range is [-1, n-1] here.
+// explicit fmaf.  The cell corner is clamped into the copy (a cell off the
+// detector reads four zeros and adds +0).  FAST (copy below 4 GiB, one view
+// below 16 MiB): buffer loads at a byte offset computed as one exact FP32
+// fma of small integers; otherwise element indices in integers.
 template <typename TS, typename TV>
 __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__ quads, float sino_inv,
                                         TV* __restrict__ vol, float vol_scale, float post, int bx, int by, int bz) {
@@ -77,27 +94,63 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
     float acc[ZB];
 #pragma unroll
     for (int k = 0; k < ZB; ++k) acc[k] = 0.0f;
-    const unsigned row = 2u * (unsigned)(g.nu + 2);             // elements per quad row
-    const size_t plane = (size_t)row * (size_t)(g.nv + 1);      // elements per view
-    const float fnu1 = (float)(g.nu - 1), fnv1 = (float)(g.nv - 1);
-    for (int p = 0; p < g.nphi; ++p) {
-        const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
-        const float uf = floorf(cvw.u);
-        const bool uok = uf >= -1.0f && uf <= fnu1;
-        const float fu = cvw.u - uf;
-        const unsigned ucol = uok ? 2u * (unsigned)((int)uf + 1) : 0u;
-        const TS* gp = quads + (size_t)p * plane;
+    const QuadShape qs = quad_shape(g.nu, g.nv);
+    const float fnu = (float)g.nu, fnv = (float)g.nv;
+    constexpr uint32_t es = sizeof(TS);
+    const uint64_t view_bytes = (uint64_t)qs.view * es;
+    const bool fast = view_bytes < (1ull << 24) && view_bytes * (uint64_t)g.nphi < (1ull << 32);
+    if (fast) {
+        const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(quads, (uint32_t)(view_bytes * (uint64_t)g.nphi));
+        const float row_b = (float)(qs.row * es);
+        for (int p = 0; p < g.nphi; ++p) {
+            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+            float u = cvw.u, A = cvw.A;
+            if (!(__builtin_isfinite(u) && __builtin_isfinite(A))) {
+                u = -8.0f;
+                A = 0.0f;
+            }
+            const float uf = floorf(u);
+            const float fu = u - uf;
+            const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
+            // byte offset of cell (uc, vc): ((vc + 2) row + 2 (uc + 2)) es
+            const float base = fmaf(uc + 2.0f, (float)(2u * es), 2.0f * row_b);
+            const uint32_t soff = (uint32_t)p * (uint32_t)view_bytes;
 #pragma unroll
-        for (int k = 0; k < ZB; ++k) {
-            const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-            const float vf = floorf(v);
-            const bool ok = uok && vf >= -1.0f && vf <= fnv1;
-            const unsigned e = ok ? (unsigned)((int)vf + 1) * row + ucol : 0u;
-            const float4 t = Storage<TS>::load4(gp, e, sino_inv);
-            const float fv = v - vf;
-            const float a = fmaf(fu, t.z - t.x, t.x);
-            const float b = fmaf(fu, t.w - t.y, t.y);
-            acc[k] += ok ? fmaf(fv, b - a, a) : 0.0f;
+            for (int k = 0; k < ZB; ++k) {
+                const float v = fmaf(zp0 + (float)k, A, cv);
+                const float vf = floorf(v);
+                const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
+                const uint32_t off = (uint32_t)fmaf(vc, row_b, base);
+                const float4 t = BufLoad4<TS>::load(rsrc, off, soff, sino_inv);
+                const float fv = v - vf;
+                const float a = fmaf(fu, t.z - t.x, t.x);
+                const float b = fmaf(fu, t.w - t.y, t.y);
+                acc[k] += fmaf(fv, b - a, a);
+            }
+        }
+    } else {
+        for (int p = 0; p < g.nphi; ++p) {
+            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+            float u = cvw.u, A = cvw.A;
+            if (!(__builtin_isfinite(u) && __builtin_isfinite(A))) {
+                u = -8.0f;
+                A = 0.0f;
+            }
+            const float uf = floorf(u);
+            const float fu = u - uf;
+            const int uc = (int)__builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
+            const TS* gp = quads + (size_t)p * qs.view + 2u * (uint32_t)(uc + 2);
+#pragma unroll
+            for (int k = 0; k < ZB; ++k) {
+                const float v = fmaf(zp0 + (float)k, A, cv);
+                const float vf = floorf(v);
+                const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
+                const float4 t = Storage<TS>::load4(gp, (uint64_t)(uint32_t)(vc + 2) * qs.row, sino_inv);
+                const float fv = v - vf;
+                const float a = fmaf(fu, t.z - t.x, t.x);
+                const float b = fmaf(fu, t.w - t.y, t.y);
+                acc[k] += fmaf(fv, b - a, a);
+            }
         }
     }
     const size_t sxy = (size_t)g.nx * g.ny;
@@ -119,8 +172,8 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
     vi_tile<TS, TV>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Shared launches (FP32 storage).  Tap quads: tile grid (ceil((nu+2)/64),
-// ceil((nv+1)/4), nphi).  VI: tile grid (ceil(nx/16), ceil(ny/16),
+// Shared launches (FP32 storage).  Tap quads: tile grid (ceil((nu+4)/64),
+// ceil((nv+3)/4), nphi).  VI: tile grid (ceil(nx/16), ceil(ny/16),
 // ceil(nz/16)); the stored value is (delta * sum) * a.
 __global__ __launch_bounds__(256) void vi_quads_batch_kernel(const BatchTable* __restrict__ t) {
     const uint32_t total = t->total;
@@ -194,15 +247,12 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
     const DevGeom& d = g->dev;
     const size_t es = sino.dtype == TOMO_F32 ? 4 : 2;
-    // Offsets inside one view of the copy are 32-bit.
-    const size_t view_elems = (size_t)(d.nv + 1) * (d.nu + 2) * 2;
-    if (view_elems >= (1ull << 32)) return fail(TOMO_ERR_INVALID_ARG, "detector too large for the tap-quad copy");
-    const size_t quad_elems = (size_t)d.nphi * view_elems;
+    const size_t quad_elems = vi_quad_elems(d.nu, d.nv, d.nphi);
     // Tap-quad copy of the sinogram from the stream-ordered pool, released
     // after the backprojection on the same stream.
     void* quads = nullptr;
     TOMO_HIP(hipMallocAsync(&quads, quad_elems * es, s));
-    const dim3 qgrid(ceil_div(d.nu + 2, 64), ceil_div(d.nv + 1, 4), d.nphi);
+    const dim3 qgrid(ceil_div(d.nu + 4, 64), ceil_div(d.nv + 3, 4), d.nphi);
     const dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
     const float inv = 1.0f / sino.scale;
     hipError_t le = hipSuccess;

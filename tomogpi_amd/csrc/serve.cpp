@@ -142,7 +142,7 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd) {
     sizes[i++] = ns * 4;                                                 // 10 px
     sizes[i++] = ns * 4;                                                 // 11 r
     sizes[i++] = ns * 4;                                                 // 12 pd
-    sizes[i++] = (size_t)d.nphi * (d.nv + 1) * (d.nu + 2) * 2 * 4;       // 13 tq
+    sizes[i++] = vi_quad_elems(d.nu, d.nv, d.nphi) * 4;                  // 13 tq
     sizes[i++] = (size_t)vec_tiles(ns) * 8;                              // 14 part_res
     sizes[i++] = (size_t)vec_tiles(nv) * 8;                              // 15 part_vol
     sizes[i++] = lam ? (size_t)lap_tiles(d) * 8 : 0;                     // 16 part_reg
@@ -318,7 +318,7 @@ void build_tables(tomo_server* srv, const std::vector<std::shared_ptr<Request>>&
         fr.n = vec_tiles(ns);
         fr.slot = q.slot;
         fr.flags = kSlotResidual;
-        BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 2, 64), cdiv(d.nv + 1, 4), d.nphi);
+        BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 4, 64), cdiv(d.nv + 3, 4), d.nphi);
         vq.in = q.r;
         vq.out = q.tq;
         BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, 16), cdiv(d.ny, 16), cdiv(d.nz, 16));
@@ -633,7 +633,8 @@ extern "C" int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req
     q->host = (gd.flags & TOMO_FLAG_HOST_PTR) != 0;
     int rc = derive_geometry(req->geom, &q->geom);
     if (rc) return rc;
-    if (zpair_elems_y(q->geom.info.nx, q->geom.info.ny, q->geom.info.nz) >= (1ull << 32))
+    const tomo_geom_info& gi = q->geom.info;
+    if (zpair_elems_x(gi.nx, gi.ny, gi.nz) >= (1ull << 32) || zpair_elems_y(gi.nx, gi.ny, gi.nz) >= (1ull << 32))
         return fail(TOMO_ERR_UNSUPPORTED, "volume too large for the Siddon traversal index");
     std::lock_guard<std::mutex> lk(srv->mu);
     if (srv->broken) return fail(TOMO_ERR_HIP, "server stopped after a device failure");
