@@ -35,6 +35,12 @@ def parse():
     ap.add_argument("--nphi", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=4, help="views in the CPU baseline sample")
+    ap.add_argument("--serving-requests", type=int, default=24,
+                    help="requests in the serving sample (0 skips the serving leg)")
+    ap.add_argument("--serving-rates", default="2,10,50", help="Poisson arrival rates (req/s)")
+    ap.add_argument("--serving-schedule", choices=["direct", "recurrent"], default="recurrent")
+    ap.add_argument("--batch-problems", type=int, default=32,
+                    help="uneven SIRT problems in the batched leg (0 skips it)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one plain device copy of the volume (known bytes) before the timed steps, "
                          "as the reference kernel for FETCH_SIZE / WRITE_SIZE counter scaling")
@@ -98,6 +104,147 @@ def cpu_baseline(vol_host, n, nphi, views, threads):
     }
 
 
+def serving_leg(ctx, n_req, rates, schedule, seed=42):
+    """Mixed-size reconstruction stream: the plain sequential FP32 loop
+    (tomo_gd_run, one request after another) against the serving loop at
+    Poisson arrival rates.  Requests: edge n in {128,256,384,512} with
+    weights .4/.3/.2/.1, 180 or 360 views (p = .5), n x n detector, 20
+    gradient iterations with the quadratic regulariser (lambda 0.01, every
+    step optimal), pairs alternating RSVI / SIDDONVI, 16-ellipsoid phantom
+    seeded by the request id.  Sinograms y = P(phantom) are made on the
+    device before the clock starts (excluded from latency)."""
+    import numpy as np
+    import torch
+    from tomogpi_amd import lib
+    rng = np.random.default_rng(seed)
+    edges = rng.choice([128, 256, 384, 512], size=n_req, p=[0.4, 0.3, 0.2, 0.1])
+    views = rng.choice([180, 360], size=n_req)
+    reqs = []
+    for i in range(n_req):
+        n, nphi = int(edges[i]), int(views[i])
+        desc = lib.AcquisitionDesc.scaled(n, nphi)
+        G = lib.Geometry(ctx, desc)
+        pair = lib.PAIR_RSVI if i % 2 == 0 else lib.PAIR_SIDDONVI
+        truth = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+        G.ellipsoids(truth, count=16, seed=1000 + i, amin=0.02 * n, amax=0.3 * n)
+        y = torch.empty(G.sino_shape, dtype=torch.float32, device="cuda")
+        G.project(truth, y, lib.PROJ_RS if pair == lib.PAIR_RSVI else lib.PROJ_SIDDON)
+        del truth
+        reqs.append(dict(n=n, nphi=nphi, desc=desc, G=G, pair=pair, y=y))
+    torch.cuda.synchronize()
+    gd = dict(iters=20, kopt=21, lam=0.01)
+    # Sequential plain form: service time of each request.
+    service = []
+    for r in reqs:
+        vol = torch.zeros(r["G"].vol_shape, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r["G"].gd_run(r["y"], vol, pair=r["pair"], **gd)
+        torch.cuda.synchronize()
+        service.append(time.perf_counter() - t0)
+        r["ref"] = vol
+    seq_total = sum(service)
+    out = {"requests": n_req, "sizes": {str(k): int((edges == k).sum()) for k in (128, 256, 384, 512)},
+           "schedule": "recurrent" if schedule == lib.SCHEDULE_RECURRENT else "direct",
+           "sequential_rps": round(n_req / seq_total, 4), "sequential_busy_s": round(seq_total, 3), "rates": []}
+    for rate in rates:
+        arrivals = np.cumsum(np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, size=n_req))
+        # sequential latency at these arrivals (FIFO, one at a time)
+        t_free, lat_seq = 0.0, []
+        for a_t, s_t in zip(arrivals, service):
+            start = max(a_t, t_free)
+            t_free = start + s_t
+            lat_seq.append(t_free - a_t)
+        srv = lib.Server(ctx, max_active=64, use_graph=True, schedule=schedule)
+        outs = [torch.zeros(r["G"].vol_shape, dtype=torch.float32, device="cuda") for r in reqs]
+        torch.cuda.synchronize()
+        ids = []
+        t0 = time.perf_counter()
+        for i, r in enumerate(reqs):
+            dt = arrivals[i] - (time.perf_counter() - t0)
+            if dt > 0:
+                time.sleep(dt)
+            ids.append(srv.submit(r["desc"], r["y"], outs[i], pair=r["pair"], **gd))
+        for rid in ids:
+            srv.wait(rid)
+        st = srv.stats()
+        srv.close()
+        torch.cuda.synchronize()
+        err = max(float(torch.linalg.vector_norm(o - r["ref"]) / torch.linalg.vector_norm(r["ref"]))
+                  for o, r in zip(outs, reqs))
+        lat = np.array(lat_seq)
+        out["rates"].append({
+            "rate_rps": rate, "serving_rps": round(st["throughput_rps"], 4),
+            "speedup_vs_sequential": round(st["throughput_rps"] * seq_total / n_req, 3),
+            "p50_s": round(st["latency_p50_s"], 3), "p99_s": round(st["latency_p99_s"], 3),
+            "sequential_p50_s": round(float(np.percentile(lat, 50, method="inverted_cdf")), 3),
+            "sequential_p99_s": round(float(np.percentile(lat, 99, method="inverted_cdf")), 3),
+            "peak_active": st["peak_active"], "peak_gib": round(st["peak_bytes"] / 2**30, 2),
+            "max_rel_l2_vs_sequential": float(f"{err:.3e}")})
+        print(f"[serving] rate {rate}: {out['rates'][-1]}", file=sys.stderr, flush=True)
+        del outs
+    for r in reqs:
+        r["G"].close()
+    return out
+
+
+def batch_leg(ctx, n_prob, iters=20, seed=7):
+    """Batched uneven SIRT reconstructions: problems with edge n uniform in
+    {128,192,256,320,384}, views uniform in [90,360], n x n detector,
+    16-ellipsoid phantoms seeded per problem, pairs alternating RSVI /
+    SIDDONVI, `iters` SIRT iterations each, FP32.  One at a time
+    (tomo_sirt_run) against all at once in the server's shared launches;
+    the volumes must agree bit for bit."""
+    import numpy as np
+    import torch
+    from tomogpi_amd import lib
+    rng = np.random.default_rng(seed)
+    edges = rng.choice([128, 192, 256, 320, 384], size=n_prob)
+    views = rng.integers(90, 361, size=n_prob)
+    probs = []
+    for i in range(n_prob):
+        n, nphi = int(edges[i]), int(views[i])
+        desc = lib.AcquisitionDesc.scaled(n, nphi)
+        G = lib.Geometry(ctx, desc)
+        pair = lib.PAIR_RSVI if i % 2 == 0 else lib.PAIR_SIDDONVI
+        truth = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+        G.ellipsoids(truth, count=16, seed=2000 + i, amin=0.02 * n, amax=0.3 * n)
+        y = torch.empty(G.sino_shape, dtype=torch.float32, device="cuda")
+        G.project(truth, y, lib.PROJ_RS if pair == lib.PAIR_RSVI else lib.PROJ_SIDDON)
+        del truth
+        probs.append(dict(desc=desc, G=G, pair=pair, y=y, n=n, nphi=nphi))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for p in probs:
+        p["ref"] = torch.empty(p["G"].vol_shape, dtype=torch.float32, device="cuda")
+        p["G"].sirt_run(p["y"], p["ref"], pair=p["pair"], iters=iters)
+    torch.cuda.synchronize()
+    t_seq = time.perf_counter() - t0
+    print(f"[batch] one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
+    outs = [torch.zeros(p["G"].vol_shape, dtype=torch.float32, device="cuda") for p in probs]
+    srv = lib.Server(ctx, max_active=min(64, n_prob), use_graph=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ids = [srv.submit(p["desc"], p["y"], o, pair=p["pair"], iters=iters, algorithm=lib.ALGO_SIRT)
+           for p, o in zip(probs, outs)]
+    for rid in ids:
+        srv.wait(rid)
+    t_batch = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    torch.cuda.synchronize()
+    same = all(bool(torch.equal(o, p["ref"])) for o, p in zip(outs, probs))
+    upd = sum(2.0 * iters * p["n"] ** 3 * p["nphi"] for p in probs)
+    for p in probs:
+        p["G"].close()
+    print(f"[batch] batched: {t_batch:.2f} s, identical={same}", file=sys.stderr, flush=True)
+    return {"problems": n_prob, "iterations": iters, "algorithm": "SIRT", "storage": "f32",
+            "one_at_a_time_s": round(t_seq, 3), "batched_s": round(t_batch, 3),
+            "speedup": round(t_seq / t_batch, 3), "batched_gups": round(upd / t_batch / 1e9, 2),
+            "peak_active": st["peak_active"], "rounds": st["rounds"], "identical_to_one_at_a_time": same,
+            "note": "BF16 storage is not offered by the shared launches (FP32 only)"}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -116,6 +263,7 @@ def main():
     G.ellipsoids(vol, count=64, seed=0, amin=5.0 * n / 512, amax=120.0 * n / 512)
     sino = torch.empty(G.sino_shape, dtype=torch.float32, device="cuda")
     back = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+    vol_host = vol.cpu().numpy()
     n_seg = G.project_work(lib.PROJ_SIDDON)
     n_samp = G.project_work(lib.PROJ_RS)
 
@@ -151,6 +299,7 @@ def main():
     sid_ms = sum(x.elapsed_time(y) for x, y in sid_ev) / a.steps
     vi_ms = sum(x.elapsed_time(y) for x, y in vi_ev) / a.steps
     rs_ms, _ = timed(rs, max(2, min(a.steps, 3)), stream)
+    print(f"[bench] siddon {sid_ms:.2f} ms, vi {vi_ms:.2f} ms, rs {rs_ms:.2f} ms", file=sys.stderr, flush=True)
 
     upd = float(n) ** 3 * nphi
     nsino = float(G.info["nu"]) * G.info["nv"] * nphi
@@ -178,10 +327,21 @@ def main():
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": measured.get(dom_kernel)}
 
+    serving = None
+    del vol, sino, back
+    torch.cuda.empty_cache()
+    if a.serving_requests > 0:
+        sched = lib.SCHEDULE_RECURRENT if a.serving_schedule == "recurrent" else lib.SCHEDULE_DIRECT
+        serving = serving_leg(ctx, a.serving_requests, [float(x) for x in a.serving_rates.split(",")], sched)
+
+    batched = None
+    if a.batch_problems > 0:
+        batched = batch_leg(ctx, a.batch_problems)
+
     base = None
     if not a.no_cpu_baseline:
         threads = min(16, os.cpu_count() or 1)
-        base = cpu_baseline(vol.cpu().numpy(), n, nphi, a.cpu_views, threads)
+        base = cpu_baseline(vol_host, n, nphi, a.cpu_views, threads)
 
     out = {
         "metric": METRIC,
@@ -203,6 +363,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": base,
         "operators": ops,
+        "serving": serving,
+        "batched": batched,
     }
     print(json.dumps(out), flush=True)
 

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TOMOGPI_ABI_VERSION 1
+#define TOMOGPI_ABI_VERSION 2
 
 /* ---- status codes ------------------------------------------------------ */
 #define TOMO_OK                0
@@ -73,6 +73,12 @@ extern "C" {
                                         z clip. Default is the clean Siddon. */
 #define TOMO_FLAG_HOST_PTR      0x4  /* data pointers are host memory: stage,
                                         compute, copy back, synchronise       */
+#define TOMO_FLAG_FDK           0x8  /* VI: FDK-weighted backprojection
+                                        (back_fdk; K/BackProjector/FDK3D_kernel
+                                        _16reg.cuh:71-88, C/BackProjector.cu:
+                                        34-35): f = sum_p W_p bilinear(g_p),
+                                        W_p = alphaC (FDD/du) / w^2 /
+                                        sqrt(betaC + u'^2 + v'^2), no delta   */
 
 typedef struct tomo_ctx  tomo_ctx;
 typedef struct tomo_geom tomo_geom;
@@ -228,6 +234,26 @@ int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* para
                 const float* sinogram, float* volume, tomo_gd_iter_stat* stats,
                 void* stream);
 
+/* ---- SIRT ----------------------------------------------------------------- */
+/* No counterpart in the reference; defined here for batched benchmarking:
+ *   R = 1 / (P 1) on the sinogram, C = relaxation / (B 1) on the volume (0
+ *   where the sum is not positive), x0 = 0, and per iteration
+ *   r = (y - P x) * R;  x = x + C * (B r)   (clamped at 0 with positivity)
+ * with P, B the projector and backprojector of the pair; FP32 element-wise
+ * steps without fused multiply-adds.                                        */
+typedef struct tomo_sirt_params {
+    int32_t pair;                    /* TOMO_PAIR_*                          */
+    int32_t flags;                   /* operator flags; TOMO_FLAG_HOST_PTR   */
+    int32_t iterations;
+    int32_t positivity;
+    double  relaxation;              /* 1.0 for plain SIRT                    */
+} tomo_sirt_params;
+
+/* volume receives the result (its input contents are ignored); residual_sq
+ * (may be NULL) receives sum (y - P x)^2 before each update.  Synchronous.  */
+int tomo_sirt_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_sirt_params* params,
+                  const float* sinogram, float* volume, double* residual_sq, void* stream);
+
 /* ---- serving loop: continuous batching of reconstructions ---------------- */
 /* No counterpart in the reference (it runs one reconstruction per process).
  * A server owns a worker thread and a stream on the context's device.  Each
@@ -240,7 +266,8 @@ int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* para
  * persistent launch over all of them (optionally replayed from a captured
  * hipGraph); requests join and leave between rounds.  A request's arithmetic
  * does not depend on what shares its launches, so its result is the same
- * alone and in any mix.  Storage is FP32.                                     */
+ * alone and in any mix.  A request is a simple-gradient reconstruction or a
+ * SIRT reconstruction (tomo_sirt_run).  Storage is FP32.                     */
 typedef struct tomo_server tomo_server;
 
 #define TOMO_SCHEDULE_DIRECT     0 /* the operations of tomo_gd_run          */
@@ -259,14 +286,21 @@ typedef struct tomo_serve_params {
     int32_t  reserved;
 } tomo_serve_params;
 
+#define TOMO_ALGO_GD    0
+#define TOMO_ALGO_SIRT  1
+
 typedef struct tomo_serve_request {
     tomo_geom_desc geom;
     tomo_gd_params gd;          /* pair, flags, iterations, Kopt, lambda,
                                    positivity, alpha_min_init; the flag
                                    TOMO_FLAG_HOST_PTR makes both pointers host
-                                   pointers                                  */
+                                   pointers.  SIRT uses pair, flags,
+                                   gradient_iteration_nb and positivity.     */
     const float* sinogram;      /* measured projections; read at admission  */
     float*       volume;        /* result, written when the request is done */
+    int32_t      algorithm;     /* TOMO_ALGO_*                              */
+    int32_t      reserved;
+    double       relaxation;    /* SIRT relaxation (1.0 for plain SIRT)     */
 } tomo_serve_request;
 
 #define TOMO_REQ_QUEUED   0

@@ -59,6 +59,9 @@ void og_geom_free(og_geom* g);
 void og_vi_f32(const og_geom* g, const float* sino, float* vol);
 void og_vi_f64(const og_geom* g, const double* sino, double* vol);
 void og_vi_T_f64(const og_geom* g, const double* vol, double* sino);
+/* FDK-weighted VI (back_fdk, K/BackProjector/FDK3D_kernel_16reg.cuh). */
+void og_vi_fdk_f32(const og_geom* g, const float* sino, float* vol);
+void og_vi_fdk_f64(const og_geom* g, const double* sino, double* vol);
 
 /* RS projector; cube = 0: inscribed cylinder (K/Projector/
  * projection_ER_kernel_UM.cuh:25-153), cube = 1: circumscribed circle +
@@ -109,6 +112,22 @@ typedef struct og_gd_stat {
 } og_gd_stat;
 int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino,
               float* vol, og_gd_stat* stats);
+
+/* SIRT, defined by this build (the reference has none):
+ *   R = 1 / (P 1) on the sinogram, C = relax / (B 1) on the volume (0 where
+ *   the denominator is 0), x0 = 0,
+ *   r = (y - P x) * R;  x = x + C * (B r)   (clamped at 0 with positivity)
+ * with P, B the projector and backprojector of the pair.  stats (may be
+ * NULL) receives sum (y - P x)^2 before each update.                         */
+typedef struct og_sirt_params {
+    int32_t pair, cube;
+    int32_t iters;
+    double  relax;
+    int32_t positivity;
+} og_sirt_params;
+int og_sirt_weights(const og_geom* g, const og_sirt_params* p, float* rw, float* cw);
+int og_sirt_run(const og_geom* g, const og_sirt_params* p, const float* sino, float* vol,
+                double* residual_sq);
 
 /* Thread count used by the OpenMP loops (<= 0: OpenMP default). */
 void og_set_threads(int n);

@@ -306,3 +306,60 @@ int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
     free(est); free(dj); free(t1); free(t2);
     return 0;
 }
+
+/* SIRT weights: rw = 1/(P 1) (sinogram), cw = (float)relax / (B 1) (volume),
+ * 0 where the sum is not positive.  FP32 arithmetic.                        */
+int og_sirt_weights(const og_geom* g, const og_sirt_params* p, float* rw, float* cw) {
+    const size_t nv = (size_t)g->nx * g->ny * g->nz, ns = (size_t)g->nu * g->nv * g->nphi;
+    float* ones_v = (float*)malloc(sizeof(float) * nv);
+    float* ones_s = (float*)malloc(sizeof(float) * ns);
+    if (!ones_v || !ones_s) { free(ones_v); free(ones_s); return 1; }
+    og_gd_params gp;
+    memset(&gp, 0, sizeof(gp));
+    gp.pair = p->pair;
+    gp.cube = p->cube;
+    for (size_t i = 0; i < nv; ++i) ones_v[i] = 1.0f;
+    for (size_t i = 0; i < ns; ++i) ones_s[i] = 1.0f;
+    project(g, &gp, ones_v, rw);
+    backproject(g, &gp, ones_s, cw);
+    const float relax = (float)p->relax;
+    for (size_t i = 0; i < ns; ++i) rw[i] = rw[i] > 0.0f ? 1.0f / rw[i] : 0.0f;
+    for (size_t i = 0; i < nv; ++i) cw[i] = cw[i] > 0.0f ? relax / cw[i] : 0.0f;
+    free(ones_v); free(ones_s);
+    return 0;
+}
+
+int og_sirt_run(const og_geom* g, const og_sirt_params* p, const float* sino, float* vol, double* residual_sq) {
+    const size_t nv = (size_t)g->nx * g->ny * g->nz, ns = (size_t)g->nu * g->nv * g->nphi;
+    float* rw = (float*)malloc(sizeof(float) * ns);
+    float* cw = (float*)malloc(sizeof(float) * nv);
+    float* est = (float*)malloc(sizeof(float) * ns);
+    float* b = (float*)malloc(sizeof(float) * nv);
+    if (!rw || !cw || !est || !b || og_sirt_weights(g, p, rw, cw)) {
+        free(rw); free(cw); free(est); free(b);
+        return 1;
+    }
+    og_gd_params gp;
+    memset(&gp, 0, sizeof(gp));
+    gp.pair = p->pair;
+    gp.cube = p->cube;
+    for (size_t i = 0; i < nv; ++i) vol[i] = 0.0f;
+    for (int k = 0; k < p->iters; ++k) {
+        project(g, &gp, vol, est);
+        double rs = 0.0;
+        for (size_t i = 0; i < ns; ++i) {
+            const float d = sino[i] - est[i];
+            rs += (double)d * (double)d;
+            est[i] = d * rw[i];
+        }
+        if (residual_sq) residual_sq[k] = rs;
+        backproject(g, &gp, est, b);
+        for (size_t i = 0; i < nv; ++i) {
+            float v = vol[i] + cw[i] * b[i];
+            if (p->positivity && v < 0.0f) v = 0.0f;
+            vol[i] = v;
+        }
+    }
+    free(rw); free(cw); free(est); free(b);
+    return 0;
+}

@@ -17,24 +17,28 @@
 #define RFMA fmaf
 #define RFLOOR floorf
 #define RMIN fminf
+#define RSQRT sqrtf
 #include "oracle_ops.inc"
 #undef R
 #undef SFX
 #undef RFMA
 #undef RFLOOR
 #undef RMIN
+#undef RSQRT
 
 #define R double
 #define SFX f64
 #define RFMA fma
 #define RFLOOR floor
 #define RMIN fmin
+#define RSQRT sqrt
 #include "oracle_ops.inc"
 #undef R
 #undef SFX
 #undef RFMA
 #undef RFLOOR
 #undef RMIN
+#undef RSQRT
 
 static int nthreads(void) {
 #ifdef _OPENMP

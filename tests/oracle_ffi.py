@@ -50,6 +50,11 @@ class OgGdParams(C.Structure):
     ]
 
 
+class OgSirtParams(C.Structure):
+    _fields_ = [("pair", C.c_int32), ("cube", C.c_int32), ("iters", C.c_int32), ("relax", C.c_double),
+                ("positivity", C.c_int32)]
+
+
 class OgGdStat(C.Structure):
     _fields_ = [("residual_sq", C.c_double), ("norm_dj", C.c_double),
                 ("norm_dj_proj", C.c_double), ("alpha", C.c_double)]
@@ -123,6 +128,7 @@ class Oracle:
             lib.og_geom_free.argtypes = [G]
             fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
             for name, a in [("og_vi_f32", [G, fp, fp]), ("og_vi_f64", [G, dp, dp]),
+                            ("og_vi_fdk_f32", [G, fp, fp]), ("og_vi_fdk_f64", [G, dp, dp]),
                             ("og_vi_T_f64", [G, dp, dp]),
                             ("og_rs_f32", [G, C.c_int, fp, fp]), ("og_rs_f64", [G, C.c_int, dp, dp]),
                             ("og_rs_T_f64", [G, C.c_int, dp, dp]),
@@ -142,6 +148,10 @@ class Oracle:
             lib.og_ellipsoids.argtypes = [C.c_int] * 4 + [C.c_uint64, C.c_double, C.c_double, fp]
             lib.og_ellipsoid_table.argtypes = [C.c_int] * 4 + [C.c_uint64, C.c_double, C.c_double, dp, dp]
             lib.og_gd_run.argtypes = [G, C.POINTER(OgGdParams), fp, fp, C.POINTER(OgGdStat)]
+            lib.og_sirt_weights.argtypes = [G, C.POINTER(OgSirtParams), fp, fp]
+            lib.og_sirt_weights.restype = C.c_int
+            lib.og_sirt_run.argtypes = [G, C.POINTER(OgSirtParams), fp, fp, dp]
+            lib.og_sirt_run.restype = C.c_int
             lib.og_gd_run.restype = C.c_int
             lib.og_set_threads.argtypes = [C.c_int]
             cls._lib = lib
@@ -189,6 +199,17 @@ class Oracle:
             s = np.ascontiguousarray(sino, dtype=np.float32)
             out = self._vol(np.float32)
             self.lib().og_vi_f32(C.byref(self._g), _f32p(s), _f32p(out))
+        return out
+
+    def vi_fdk(self, sino, f64=False):
+        if f64:
+            s = np.ascontiguousarray(sino, dtype=np.float64)
+            out = self._vol(np.float64)
+            self.lib().og_vi_fdk_f64(C.byref(self._g), _f64p(s), _f64p(out))
+        else:
+            s = np.ascontiguousarray(sino, dtype=np.float32)
+            out = self._vol(np.float32)
+            self.lib().og_vi_fdk_f32(C.byref(self._g), _f32p(s), _f32p(out))
         return out
 
     def vi_T(self, vol):
@@ -256,6 +277,20 @@ class Oracle:
         assert rc == 0
         return v, [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj,
                         alpha=x.alpha) for x in stats[:iters]]
+
+    def sirt_weights(self, pair=0, cube=False, relax=1.0):
+        p = OgSirtParams(pair=pair, cube=int(cube), iters=0, relax=relax, positivity=0)
+        rw, cw = self._sino(np.float32), self._vol(np.float32)
+        assert self.lib().og_sirt_weights(C.byref(self._g), C.byref(p), _f32p(rw), _f32p(cw)) == 0
+        return rw, cw
+
+    def sirt(self, sino, pair=0, cube=False, iters=3, relax=1.0, positivity=0):
+        s = np.ascontiguousarray(sino, dtype=np.float32)
+        v = self._vol(np.float32)
+        res = np.zeros(max(iters, 1), dtype=np.float64)
+        p = OgSirtParams(pair=pair, cube=int(cube), iters=iters, relax=relax, positivity=positivity)
+        assert self.lib().og_sirt_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(v), _f64p(res)) == 0
+        return v, list(res[:iters])
 
 
 def laplacian(x):

@@ -69,6 +69,16 @@ def test_cli_proj_back(tmp_path, storage, cube):
     assert np.array_equal(back, o.vi(ref))
 
 
+def test_cli_back_fdk(tmp_path):
+    o = Oracle(geom())
+    sino = o.rs(shepp_logan(N))
+    write_workdir(tmp_path)
+    sino.tofile(tmp_path / f"Projections_{N}" / "P_ER_GPU.s")
+    run(tmp_path, "--op=back_fdk", "--storage=mem_GPU")
+    got = np.fromfile(tmp_path / "back_fdk.v", dtype=np.float32).reshape(o.geom.vol_shape)
+    assert np.array_equal(got, o.vi_fdk(sino))
+
+
 def test_cli_siddon_pair_readme_spelling(tmp_path):
     vol = ellipsoids(N, count=5, seed=2, amin=2.0, amax=8.0)
     vol.tofile(tmp_path / "phantom.v")

@@ -79,6 +79,18 @@ def test_vi_bit_exact(setup):
     assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
 
 
+def test_vi_fdk_bit_exact(setup):
+    """FDK-weighted VI (TOMO_FLAG_FDK, the reference's back_fdk)."""
+    g, o, G = setup
+    rng = np.random.default_rng(12)
+    sino = rng.random(g.sino_shape, dtype=np.float32)
+    ref = o.vi_fdk(sino)
+    out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+    G.backproject(dev(sino), out, flags=lib.FLAG_FDK)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
 @pytest.mark.parametrize("cube", [False, True])
 def test_rs_bit_exact(setup, cube):
     g, o, G = setup

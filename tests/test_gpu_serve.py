@@ -174,3 +174,30 @@ def test_oversized_request_fails_alone(ctx, problems, alone):
     srv.close()
     torch.cuda.synchronize()
     assert np.array_equal(outs[2].cpu().numpy(), alone[2][0])
+
+
+def test_sirt_served_in_a_mix_equals_plain_sirt(ctx, problems):
+    """SIRT requests served beside gradient requests give the plain SIRT loop's
+    volume bit for bit (no norm enters SIRT's update)."""
+    srv = lib.Server(ctx, max_active=8)
+    outs, ids, plain = {}, {}, {}
+    gd_ids = _submit(srv, problems, [0, 1], outs)
+    for i in (2, 4):
+        g, sino = problems[i]
+        n, nphi, pair, lam, kopt, pos = CASES[i]
+        y = torch.from_numpy(sino).cuda()
+        ref = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        lib.Geometry(ctx, to_desc(g)).sirt_run(y, ref, pair=pair, iters=ITERS, relaxation=0.8)
+        plain[i] = ref
+        outs[i] = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+        torch.cuda.synchronize()
+        ids[i] = srv.submit(to_desc(g), y, outs[i], pair=pair, iters=ITERS, algorithm=lib.ALGO_SIRT,
+                            relaxation=0.8)
+    for rid in list(ids.values()) + gd_ids:
+        assert srv.wait(rid, timeout=120.0)["state"] == lib.REQ_DONE
+    hist = srv.iter_stats(ids[2], ITERS)
+    srv.close()
+    torch.cuda.synchronize()
+    for i in (2, 4):
+        assert np.array_equal(outs[i].cpu().numpy(), plain[i].cpu().numpy()), i
+    assert hist[-1]["residual_sq"] < hist[0]["residual_sq"]

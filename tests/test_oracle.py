@@ -26,6 +26,11 @@ them retypes the oracle's own formula:
             normdJ^2 / (2 normdJProj) each iteration; alpha_min rule.
 * phantoms - Shepp-Logan centre value and shell volume; ellipsoid table
             invariants and voxel membership.
+* FDK VI  - constant-sinogram closed form with the FDK weight evaluated on the
+            vector construction (source distance, detector offsets).
+* SIRT    - first iteration composed from the pinned operators (weights,
+            relaxation, sign); exact homogeneity; convergence on consistent
+            data.
 """
 import numpy as np
 import pytest
@@ -88,6 +93,30 @@ def test_vi_constant_sinogram_closed_form(f64):
     exp *= gr.derived(g)["delta"]
     assert np.abs(f - exp).max() <= (1e-9 if f64 else 1e-5) * np.abs(exp).max()
     assert exp.max() > 0 and (exp == exp.max()).sum() > 10  # interior voxels see every view
+
+
+@pytest.mark.parametrize("f64", [True, False])
+def test_vi_fdk_constant_sinogram_closed_form(f64):
+    """FDK weighting (FDK3D_kernel_16reg.cuh:71-88) on g = 1: f = sum_p W_p x
+    (valid-tap weights), W_p = alphaC (FDD/du) / w^2 / sqrt(betaC + u'^2 +
+    v'^2) with w the source distance along the central ray and (u', v') the
+    detector offsets of the voxel's projection, from the vector construction."""
+    g = small_geom(20, 12, nu=26, nv=14)
+    o = Oracle(g)
+    f = o.vi_fdk(np.ones(g.sino_shape), f64=f64)
+    d = gr.derived(g)
+    X = gr.voxel_centres(g)
+    ku = g.fdd / d["du"]
+    alpha_c = float(np.float32(np.pi * g.fod * g.fdd / g.nphi))
+    exp = np.zeros(g.vol_shape)
+    for p, phi in enumerate(gr.angles(g)):
+        er, _, _ = gr.frame(phi)
+        w = X @ er + g.fod
+        u, v = gr.project_point(g, p, X)
+        W = alpha_c * ku / w ** 2 / np.sqrt(ku * ku + (u - d["cu"]) ** 2 + (v - d["cv"]) ** 2)
+        exp += W * _support_weight(u, g.nu) * _support_weight(v, g.nv)
+    assert np.abs(f - exp).max() <= (1e-9 if f64 else 2e-5) * np.abs(exp).max()
+    assert exp.max() > 0
 
 
 def test_vi_single_cell_cone():
@@ -388,6 +417,41 @@ def test_loop_positivity_clamps():
     y = -o.rs(ellipsoids(16, count=4, seed=5, amin=2.0, amax=6.0))
     x, _ = o.gd(y, pair=0, iters=2, kopt=10, lam=0.0, positivity=1)
     assert x.min() >= 0.0
+
+
+# -------------------------------------------------------------------- SIRT
+# SIRT is defined by this build: x += C B (R (y - P x)), R = 1/(P 1),
+# C = relax/(B 1), with the pair's (independently pinned) operators.
+@pytest.mark.parametrize("pair", [0, 1])
+def test_sirt_first_iteration_from_pinned_operators(pair):
+    g = scaled_geometry(20, 12)
+    o = Oracle(g)
+    y = o.rs(ellipsoids(20, count=4, seed=6, amin=2.0, amax=7.0))
+    proj = (lambda v: o.rs(v)) if pair == 0 else (lambda v: o.siddon(v))
+    p1 = proj(np.ones(g.vol_shape, dtype=np.float32))
+    b1 = o.vi(np.ones(g.sino_shape, dtype=np.float32))
+    rw = np.where(p1 > 0, np.float32(1.0) / np.where(p1 > 0, p1, 1), 0).astype(np.float32)
+    cw = np.where(b1 > 0, np.float32(0.75) / np.where(b1 > 0, b1, 1), 0).astype(np.float32)
+    ow, oc = o.sirt_weights(pair=pair, relax=0.75)
+    assert np.array_equal(ow, rw) and np.array_equal(oc, cw)
+    x1, res = o.sirt(y, pair=pair, iters=1, relax=0.75)
+    expect = (np.float32(0.0) + cw * o.vi((y - np.float32(0.0)) * rw)).astype(np.float32)
+    assert np.array_equal(x1, expect)
+    assert res[0] == pytest.approx(float((y.astype(np.float64) ** 2).sum()), rel=1e-12)
+
+
+def test_sirt_is_homogeneous_and_converges():
+    g = scaled_geometry(20, 16)
+    o = Oracle(g)
+    truth = ellipsoids(20, count=5, seed=7, amin=2.0, amax=7.0)
+    y = o.rs(truth)
+    x, res = o.sirt(y, pair=0, iters=8)
+    x2, _ = o.sirt(2.0 * y, pair=0, iters=8)
+    assert np.array_equal(x2, 2.0 * x)          # scaling y by 2 is exact throughout
+    assert all(b < a for a, b in zip(res, res[1:]))
+    assert res[-1] < 0.2 * res[0]
+    x1, _ = o.sirt(y, pair=0, iters=1)
+    assert np.linalg.norm(x - truth) < np.linalg.norm(x1 - truth)
 
 
 # ----------------------------------------------------------------- phantoms

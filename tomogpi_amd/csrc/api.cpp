@@ -87,7 +87,7 @@ int do_rs(const tomo_geom* g, int cube, Buf in, MutBuf out, hipStream_t s) { ret
 int do_siddon(const tomo_geom* g, int compat, Buf in, MutBuf out, hipStream_t s) {
     return launch_siddon(g, compat, in, out, s);
 }
-int do_vi(const tomo_geom* g, int, Buf in, MutBuf out, hipStream_t s) { return launch_vi(g, in, out, s); }
+int do_vi(const tomo_geom* g, int fdk, Buf in, MutBuf out, hipStream_t s) { return launch_vi(g, in, out, s, fdk); }
 int do_rs_adj(const tomo_geom* g, int cube, Buf in, MutBuf out, hipStream_t s) {
     return launch_rs_adjoint(g, cube, in, out, s);
 }
@@ -250,7 +250,7 @@ int tomo_backproject(tomo_ctx* ctx, const tomo_geom* g, int backprojector, int f
     if (!g) return fail(TOMO_ERR_INVALID_ARG, "null geometry");
     if (backprojector != TOMO_BP_VI) return fail(TOMO_ERR_INVALID_ARG, "unknown backprojector");
     return run_operator(ctx, g, flags, sinogram, sinogram_dtype, sinogram_scale, g->nsino(), volume, volume_dtype,
-                        volume_scale, g->nvox(), stream, do_vi, 0);
+                        volume_scale, g->nvox(), stream, do_vi, (flags & TOMO_FLAG_FDK) ? 1 : 0);
 }
 
 int tomo_project_adjoint(tomo_ctx* ctx, const tomo_geom* g, int projector, int flags, const void* sinogram,

@@ -23,6 +23,7 @@ struct BatchItem {
     DevGeom g;
     const void* in;   // operator input / first element-wise operand
     const void* in2;  // second input (z-pair qy, element-wise operand b)
+    const void* in3;  // per-element weight of the element-wise ops (SIRT), or null
     void* out;        // output
     void* aux;        // second output (z-pair qy) or FP64 partial sums (one per tile)
     double* slot;     // per-problem FP64 scalars

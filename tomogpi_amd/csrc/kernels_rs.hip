@@ -142,16 +142,41 @@ __device__ __forceinline__ int wave_max(int v) {
     return v;
 }
 
+__device__ __forceinline__ float wave_minf(float v) {
+    for (int off = 32; off > 0; off >>= 1) v = fminf(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+// Samples of one lane, taken one per iteration k of the wave's loop: sample
+// s = k - k0 for k0 <= k < k0 + lr.  Same positions (fma(s, d, entry)) and
+// the same order of the sum as the per-ray loop.
+template <int A, typename TV>
+__device__ __forceinline__ float rs_samples(const RsRay& r, const TV* __restrict__ q, int nx, int ny, int nz,
+                                            float inv, int k0, int kend) {
+    float acc = 0.0f;
+    for (int k = 0; k < kend; ++k) {
+        const int s = k - k0;
+        if (s >= 0 && s < r.lr) {
+            const float sf = (float)s;
+            acc += trilinear_quads<A>(q, nx, ny, nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey), fmaf(sf, r.dz, r.ez),
+                                      inv);
+        }
+    }
+    return acc;
+}
+
 // Forward RS projector.  Samples, taps and the order of the sum are those of
 // the per-ray loop (sample s at fma(s, d, entry), trilinear interpolation,
-// acc += in s order), so results are unchanged; the schedule and the layout
-// follow the Siddon projector: when every ray of the wave has the same
-// dominant in-plane axis A (|d_A| >= the other two) and direction, the wave
-// walks the unit slabs floor(A) = I, I+-1, ... together and each lane takes
-// its samples in slab I.  Its 64 rays (adjacent detector columns) then read
-// neighbouring quads of the z-pair copy of A, two 16-byte loads per sample.
-// Other waves sample per lane from the y-slab copy.  Grid: x = column group,
-// y = view, z = row group (every view of a row band before the next band).
+// acc += in s order), so results are unchanged; what changes is the
+// schedule.  When every ray of the wave has the same dominant in-plane axis
+// A (|d_A| >= the other two) and direction, each lane starts its samples k0
+// iterations after the wave's first entry along A, k0 = its entry's distance
+// behind that one in samples (|d_A| per sample): at iteration k every lane
+// samples near the same A plane, one sample per lane per iteration.  Its 64
+// rays (adjacent detector columns) then read neighbouring quads of the
+// z-pair copy of A, two 16-byte loads per sample.  Other waves sample from
+// k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
+// (every view of a row band before the next band).
 template <typename TV, typename TS>
 __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __restrict__ qx, const TV* __restrict__ qy,
                                         float vol_inv, TS* __restrict__ sino, float sino_scale, int bx, int p, int bz) {
@@ -174,36 +199,20 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __
         for (int c = 0; c < 4; ++c)
             if (__ballot(ok && code == c) == okmask) mode = c;
     }
-    const int nx = g.nx, ny = g.ny, nz = g.nz;
-    float acc = 0.0f;
-    if (mode < 0) {
-        for (int s = 0; s < r.lr; ++s) {
-            const float sf = (float)s;
-            acc += trilinear_quads<1>(qy, nx, ny, nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey),
-                                      fmaf(sf, r.dz, r.ez), vol_inv);
-        }
-    } else {
+    int k0 = 0;
+    if (mode >= 0) {
         const bool xs = mode < 2;
-        const int sA = (mode & 1) ? -1 : 1;
-        const float eA = xs ? r.ex : r.ey, dA = xs ? r.dx : r.dy;
-        const int nA = xs ? nx : ny;
-        int s = 0;
-        bool live = ok;
-        int slab = live ? (int)floorf(fmaf(0.0f, dA, eA)) : 0;
-        int I = sA > 0 ? wave_min(live ? slab : 0x7fffffff) : wave_max(live ? slab : -0x7fffffff);
-        for (int outer = 0; outer <= nA + 4 && __any(live); ++outer) {
-            while (live && slab == I) {
-                const float sf = (float)s;
-                const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
-                acc += xs ? trilinear_quads<0>(qx, nx, ny, nz, x, y, z, vol_inv)
-                          : trilinear_quads<1>(qy, nx, ny, nz, x, y, z, vol_inv);
-                ++s;
-                live = s < r.lr;
-                slab = (int)floorf(fmaf((float)s, dA, eA));
-            }
-            I += sA;
-        }
+        const float pa = (mode & 1) ? -(xs ? r.ex : r.ey) : (xs ? r.ex : r.ey);  // entry along the travel direction
+        const float da = fabsf(xs ? r.dx : r.dy);                                  // >= 1/sqrt(3) for admitted rays
+        const float pmin = wave_minf(ok ? pa : 3.0e38f);
+        // Entries are inside the volume's bounding box, so k0 <= sqrt(3) N.
+        if (ok) k0 = (int)((pa - pmin) / da + 0.5f);
     }
+    const int kend = wave_max(ok ? k0 + r.lr : 0);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    float acc;
+    if (mode >= 0 && mode < 2) acc = rs_samples<0>(r, qx, nx, ny, nz, vol_inv, k0, kend);
+    else acc = rs_samples<1>(r, qy, nx, ny, nz, vol_inv, k0, kend);
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
         Storage<TS>::store(sino, o, r.lr > 0 ? acc * g.delta : 0.0f, sino_scale);

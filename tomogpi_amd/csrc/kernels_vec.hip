@@ -35,6 +35,29 @@ __global__ __launch_bounds__(kBlock) void axpy_kernel(float* y, const float* x, 
     }
 }
 
+// SIRT element-wise steps (FP32, no fused multiply-add):
+//   weights  w = s > 0 ? a / s : 0
+//   residual d = y - e; out = d * w
+//   update   x = x + c * b, clamped at 0 with positivity
+__global__ __launch_bounds__(kBlock) void recip_weight_kernel(const float* s_in, float a, float* w, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        const float v = s_in[i];
+        w[i] = v > 0.0f ? a / v : 0.0f;
+    }
+}
+__global__ __launch_bounds__(kBlock) void mul_kernel(float* x, const float* w, size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        x[i] = x[i] * w[i];
+}
+__global__ __launch_bounds__(kBlock) void waxpy_kernel(float* x, const float* c, const float* b, int positive,
+                                                     size_t n) {
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock) {
+        float v = x[i] + c[i] * b[i];
+        if (positive && v < 0.0f) v = 0.0f;
+        x[i] = v;
+    }
+}
+
 template <typename TI, typename TO>
 __global__ __launch_bounds__(kBlock) void convert_kernel(const TI* in, float inv, TO* out, float scale, size_t n) {
     for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
@@ -255,13 +278,14 @@ __global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __
     }
 }
 
-// out = in - in2; aux[tile] = sum out^2.
+// d = in - in2; out = d (or d * in3 when in3 is set); aux[tile] = sum d^2.
 __global__ __launch_bounds__(kBlock) void diff_sq_batch_kernel(const BatchTable* __restrict__ t) {
     TOMO_BATCH_LOOP(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const float* a = (const float*)it.in;
         const float* b = (const float*)it.in2;
+        const float* w = (const float*)it.in3;
         float* o = (float*)it.out;
         const uint64_t base = (uint64_t)local * kVecTile;
         double sq = 0.0;
@@ -270,7 +294,7 @@ __global__ __launch_bounds__(kBlock) void diff_sq_batch_kernel(const BatchTable*
             const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
             if (e < it.n) {
                 const float v = a[e] - b[e];
-                o[e] = v;
+                o[e] = w ? v * w[e] : v;
                 sq += (double)v * (double)v;
             }
         }
@@ -317,7 +341,8 @@ __global__ __launch_bounds__(kBlock) void laplacian_batch_kernel(const BatchTabl
 
 // out = out + a * in (multiply then add, as the reference host loop), a =
 // (float) slot[alpha] when flags bit 0 is set, else the item's constant;
-// flags bit 1 clamps the result at 0; flags bit 2 writes out = a instead.
+// flags bit 1 clamps the result at 0; flags bit 2 writes out = a instead;
+// with in3 set, a is the element in3[e] (the SIRT volume weight).
 __global__ __launch_bounds__(kBlock) void axpy_batch_kernel(const BatchTable* __restrict__ t) {
     TOMO_BATCH_LOOP(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
@@ -326,12 +351,13 @@ __global__ __launch_bounds__(kBlock) void axpy_batch_kernel(const BatchTable* __
         const bool pos = (it.flags & 2) != 0;
         const bool fill = (it.flags & 4) != 0;
         const float* x = (const float*)it.in;
+        const float* w = (const float*)it.in3;
         float* y = (float*)it.out;
         const uint64_t base = (uint64_t)local * kVecTile;
         for (int k = 0; k < kVecTile / kBlock; ++k) {
             const uint64_t e = base + (uint64_t)k * kBlock + threadIdx.x;
             if (e < it.n) {
-                float v = fill ? a : y[e] + a * x[e];
+                float v = fill ? a : y[e] + (w ? w[e] : a) * x[e];
                 if (pos && v < 0.0f) v = 0.0f;
                 y[e] = v;
             }
@@ -372,6 +398,14 @@ __global__ __launch_bounds__(64) void step_batch_kernel(const BatchTable* __rest
         const BatchItem& it = t->item[batch_find(t, tile)];
         if (threadIdx.x != 0) continue;
         double* sl = it.slot;
+        if (it.flags & 2) {  // SIRT: record the residual only
+            if (it.aux) {
+                double* h = (double*)it.aux + 4 * it.n;
+                h[0] = sl[kSlotResidual];
+                h[1] = h[2] = h[3] = 0.0;
+            }
+            continue;
+        }
         double alpha;
         if (it.flags & 1) {
             const double norm_reg = it.d != 0.0 ? 2.0 * sl[kSlotRegSq] : 0.0;
@@ -475,6 +509,21 @@ int launch_scale(float* x, float v, size_t n, hipStream_t s) {
 }
 int launch_axpy(float* y, const float* x, float a, int positive, size_t n, hipStream_t s) {
     hipLaunchKernelGGL(axpy_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, y, x, a, positive, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_recip_weight(const float* s_in, float a, float* w, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(recip_weight_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, s_in, a, w, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_mul(float* x, const float* w, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(mul_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, x, w, n);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+int launch_waxpy(float* x, const float* c, const float* b, int positive, size_t n, hipStream_t s) {
+    hipLaunchKernelGGL(waxpy_kernel, dim3(grid_for(n)), dim3(kBlock), 0, s, x, c, b, positive, n);
     TOMO_HIP(hipGetLastError());
     return TOMO_OK;
 }

@@ -22,8 +22,16 @@ constexpr int TX = 16, TY = 16, ZB = 16;
 
 struct ColumnView {
     float u, A;
+    float num, base;  // FDK weight terms (FDK only)
 };
 
+// Per (column, view) terms in FP64.  FDK (back_fdk,
+// K/BackProjector/FDK3D_kernel_16reg.cuh:71-88; constants C/BackProjector.cu:
+// 34-35) adds num = alphaC (FDD/du) / w^2 and base = betaC + u'^2, u' = u - cu
+// in pixels, alphaC = (float)(pi FOD FDD / Nphi), betaC = (FDD/du)^2.  A
+// column-view with a non-finite u or A contributes 0: it reads off the
+// detector (u = -8, A = 0) with a zero weight.
+template <bool FDK>
 __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double* c, double xp, double yp) {
     const double w = fma(c[2], xp, fma(c[3], yp, g.fod));
     const double inv = 1.0 / w;
@@ -31,6 +39,20 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
     ColumnView r;
     r.u = (float)fma(un, inv, (double)g.cu);
     r.A = (float)(c[6] * inv);
+    r.num = 0.0f;
+    r.base = 1.0f;
+    const bool fin = __builtin_isfinite(r.u) && __builtin_isfinite(r.A);
+    if (FDK && fin) {
+        const double ku = g.fdd / (double)g.du;
+        const double alpha_c = (double)(float)(M_PI * g.fod * g.fdd / (double)g.nphi);
+        const double up = un * inv;
+        r.num = (float)(alpha_c * ku / (w * w));
+        r.base = (float)(ku * ku + up * up);
+    }
+    if (!fin) {
+        r.u = -8.0f;
+        r.A = 0.0f;
+    }
     return r;
 }
 
@@ -81,7 +103,9 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
 // detector reads four zeros and adds +0).  FAST (copy below 4 GiB, one view
 // below 16 MiB): buffer loads at a byte offset computed as one exact FP32
 // fma of small integers; otherwise element indices in integers.
-template <typename TS, typename TV>
+// VI stores delta * sum; FDK weights each term by W = num / sqrt(fma(v', v',
+// base)), v' = z' A, and stores the sum itself.
+template <typename TS, typename TV, bool FDK>
 __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__ quads, float sino_inv,
                                         TV* __restrict__ vol, float vol_scale, float post, int bx, int by, int bz) {
     const int i = bx * TX + (threadIdx.x % TX);
@@ -99,57 +123,53 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
     constexpr uint32_t es = sizeof(TS);
     const uint64_t view_bytes = (uint64_t)qs.view * es;
     const bool fast = view_bytes < (1ull << 24) && view_bytes * (uint64_t)g.nphi < (1ull << 32);
+    // one voxel-view term from its taps
+    auto term = [&](const ColumnView& cvw, float fu, int k, float v, float vf, const float4& t) {
+        const float fv = v - vf;
+        const float a = fmaf(fu, t.z - t.x, t.x);
+        const float b = fmaf(fu, t.w - t.y, t.y);
+        const float r = fmaf(fv, b - a, a);
+        if (FDK) {
+            const float vp = (zp0 + (float)k) * cvw.A;
+            const float wt = cvw.num / sqrtf(fmaf(vp, vp, cvw.base));
+            acc[k] += wt * r;
+        } else {
+            acc[k] += r;
+        }
+    };
     if (fast) {
         const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(quads, (uint32_t)(view_bytes * (uint64_t)g.nphi));
         const float row_b = (float)(qs.row * es);
         for (int p = 0; p < g.nphi; ++p) {
-            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
-            float u = cvw.u, A = cvw.A;
-            if (!(__builtin_isfinite(u) && __builtin_isfinite(A))) {
-                u = -8.0f;
-                A = 0.0f;
-            }
-            const float uf = floorf(u);
-            const float fu = u - uf;
+            const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
+            const float uf = floorf(cvw.u);
+            const float fu = cvw.u - uf;
             const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
             // byte offset of cell (uc, vc): ((vc + 2) row + 2 (uc + 2)) es
             const float base = fmaf(uc + 2.0f, (float)(2u * es), 2.0f * row_b);
             const uint32_t soff = (uint32_t)p * (uint32_t)view_bytes;
 #pragma unroll
             for (int k = 0; k < ZB; ++k) {
-                const float v = fmaf(zp0 + (float)k, A, cv);
+                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
                 const float vf = floorf(v);
                 const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
                 const uint32_t off = (uint32_t)fmaf(vc, row_b, base);
-                const float4 t = BufLoad4<TS>::load(rsrc, off, soff, sino_inv);
-                const float fv = v - vf;
-                const float a = fmaf(fu, t.z - t.x, t.x);
-                const float b = fmaf(fu, t.w - t.y, t.y);
-                acc[k] += fmaf(fv, b - a, a);
+                term(cvw, fu, k, v, vf, BufLoad4<TS>::load(rsrc, off, soff, sino_inv));
             }
         }
     } else {
         for (int p = 0; p < g.nphi; ++p) {
-            const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
-            float u = cvw.u, A = cvw.A;
-            if (!(__builtin_isfinite(u) && __builtin_isfinite(A))) {
-                u = -8.0f;
-                A = 0.0f;
-            }
-            const float uf = floorf(u);
-            const float fu = u - uf;
+            const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
+            const float uf = floorf(cvw.u);
+            const float fu = cvw.u - uf;
             const int uc = (int)__builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
             const TS* gp = quads + (size_t)p * qs.view + 2u * (uint32_t)(uc + 2);
 #pragma unroll
             for (int k = 0; k < ZB; ++k) {
-                const float v = fmaf(zp0 + (float)k, A, cv);
+                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
                 const float vf = floorf(v);
                 const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                const float4 t = Storage<TS>::load4(gp, (uint64_t)(uint32_t)(vc + 2) * qs.row, sino_inv);
-                const float fv = v - vf;
-                const float a = fmaf(fu, t.z - t.x, t.x);
-                const float b = fmaf(fu, t.w - t.y, t.y);
-                acc[k] += fmaf(fv, b - a, a);
+                term(cvw, fu, k, v, vf, Storage<TS>::load4(gp, (uint64_t)(uint32_t)(vc + 2) * qs.row, sino_inv));
             }
         }
     }
@@ -157,7 +177,9 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
     const size_t base = (size_t)i + (size_t)g.nx * j;
 #pragma unroll
     for (int k = 0; k < ZB; ++k)
-        if (k0 + k < g.nz) Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, (acc[k] * g.delta) * post, vol_scale);
+        if (k0 + k < g.nz)
+            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? acc[k] : (acc[k] * g.delta) * post,
+                               vol_scale);
 }
 
 template <typename TS>
@@ -166,10 +188,10 @@ __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in
     vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <typename TS, typename TV>
+template <typename TS, typename TV, bool FDK>
 __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
                                                   TV* __restrict__ vol, float vol_scale) {
-    vi_tile<TS, TV>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
+    vi_tile<TS, TV, FDK>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Shared launches (FP32 storage).  Tap quads: tile grid (ceil((nu+4)/64),
@@ -192,7 +214,7 @@ __global__ __launch_bounds__(TX* TY) void vi_batch_kernel(const BatchTable* __re
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        vi_tile<float, float>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx,
+        vi_tile<float, float, false>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx,
                               (int)(rest % it.gy), (int)(rest / it.gy));
     }
 }
@@ -217,7 +239,7 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
     const size_t plane = (size_t)g.nu * g.nv;
     const int nu = g.nu, nv = g.nv;
     for (int p = 0; p < g.nphi; ++p) {
-        const ColumnView cvw = column_view(g, g.view + (size_t)p * kViewStride, xp, yp);
+        const ColumnView cvw = column_view<false>(g, g.view + (size_t)p * kViewStride, xp, yp);
         float* gp = sino + (size_t)p * plane;
         const float uf = floorf(cvw.u);
         if (!(uf >= -1.0f && uf <= (float)nu)) continue;
@@ -244,7 +266,7 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 
 }  // namespace
 
-int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
+int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) {
     const DevGeom& d = g->dev;
     const size_t es = sino.dtype == TOMO_F32 ? 4 : 2;
     const size_t quad_elems = vi_quad_elems(d.nu, d.nv, d.nphi);
@@ -261,8 +283,12 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s) {
         le = hipGetLastError();
         if (le == hipSuccess) {
             TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
-                hipLaunchKernelGGL((vi_kernel<TS, TV>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
-                                   (TV*)vol.p, vol.scale);
+                if (fdk)
+                    hipLaunchKernelGGL((vi_kernel<TS, TV, true>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
+                                       (TV*)vol.p, vol.scale);
+                else
+                    hipLaunchKernelGGL((vi_kernel<TS, TV, false>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads,
+                                       inv, (TV*)vol.p, vol.scale);
             });
             le = hipGetLastError();
         }

@@ -11,7 +11,7 @@
 //   else          alpha = alpha_min
 //   x += (float)alpha dJ, clamped at 0 if positivity
 // Volumes and sinograms stay on the device; only the FP64 norms return to the
-// host for the step rule.
+// host for the step rule.  The SIRT loop (tomo_sirt_run) follows.
 #include <cmath>
 
 #include "tomo_internal.hpp"
@@ -91,7 +91,70 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
     return TOMO_OK;
 }
 
+// SIRT (defined by this library; the reference has none):
+//   R = 1/(P 1), C = relaxation/(B 1), x0 = 0,
+//   r = (y - P x) * R;  x = x + C * (B r)  (+ clamp at 0).
+int sirt_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_sirt_params& pr, const float* sino, float* vol,
+             double* residual_sq, hipStream_t s) {
+    const size_t nv = g->nvox(), ns = g->nsino();
+    DevBuf rw, cw, est, b;
+    TOMO_HIP(hipMalloc(&rw.p, ns * sizeof(float)));
+    TOMO_HIP(hipMalloc(&est.p, ns * sizeof(float)));
+    TOMO_HIP(hipMalloc(&cw.p, nv * sizeof(float)));
+    TOMO_HIP(hipMalloc(&b.p, nv * sizeof(float)));
+    tomo_gd_params gp{};
+    gp.pair = pr.pair;
+    gp.flags = pr.flags;
+    float* R = (float*)rw.p;
+    float* Cv = (float*)cw.p;
+    float* e = (float*)est.p;
+    float* bv = (float*)b.p;
+    int rc;
+    // weights from the operators applied to ones
+    if ((rc = launch_fill(bv, 1.0f, nv, s))) return rc;
+    if ((rc = project(g, gp, bv, R, s))) return rc;
+    if ((rc = launch_recip_weight(R, 1.0f, R, ns, s))) return rc;
+    if ((rc = launch_fill(e, 1.0f, ns, s))) return rc;
+    if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{Cv, TOMO_F32, 1.0f}, s))) return rc;
+    if ((rc = launch_recip_weight(Cv, (float)pr.relaxation, Cv, nv, s))) return rc;
+    if ((rc = launch_fill(vol, 0.0f, nv, s))) return rc;
+    for (int k = 0; k < pr.iterations; ++k) {
+        if ((rc = project(g, gp, vol, e, s))) return rc;
+        if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
+        if (residual_sq) {
+            if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &residual_sq[k], s))) return rc;
+        }
+        if ((rc = launch_mul(e, R, ns, s))) return rc;
+        if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{bv, TOMO_F32, 1.0f}, s))) return rc;
+        if ((rc = launch_waxpy(vol, Cv, bv, pr.positivity, nv, s))) return rc;
+    }
+    TOMO_HIP(hipStreamSynchronize(s));
+    return TOMO_OK;
+}
+
 }  // namespace tomo
+
+extern "C" int tomo_sirt_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_sirt_params* params,
+                             const float* sinogram, float* volume, double* residual_sq, void* stream) {
+    using namespace tomo;
+    if (!ctx || !g || !params || !sinogram || !volume) return fail(TOMO_ERR_INVALID_ARG, "null argument");
+    TOMO_HIP(hipSetDevice(ctx->device));
+    if (params->pair != TOMO_PAIR_RSVI && params->pair != TOMO_PAIR_SIDDONVI)
+        return fail(TOMO_ERR_INVALID_ARG, "unknown pair");
+    if (params->iterations < 0) return fail(TOMO_ERR_INVALID_ARG, "negative iteration count");
+    hipStream_t s = reinterpret_cast<hipStream_t>(stream);
+    if (!(params->flags & TOMO_FLAG_HOST_PTR)) return sirt_run(ctx, g, *params, sinogram, volume, residual_sq, s);
+    const size_t nv = g->nvox(), ns = g->nsino();
+    DevBuf ds, dv;
+    TOMO_HIP(hipMalloc(&ds.p, ns * sizeof(float)));
+    TOMO_HIP(hipMalloc(&dv.p, nv * sizeof(float)));
+    TOMO_HIP(hipMemcpyAsync(ds.p, sinogram, ns * sizeof(float), hipMemcpyHostToDevice, s));
+    int rc = sirt_run(ctx, g, *params, (const float*)ds.p, (float*)dv.p, residual_sq, s);
+    if (rc) return rc;
+    TOMO_HIP(hipMemcpyAsync(volume, dv.p, nv * sizeof(float), hipMemcpyDeviceToHost, s));
+    TOMO_HIP(hipStreamSynchronize(s));
+    return TOMO_OK;
+}
 
 extern "C" int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram,
                            float* volume, tomo_gd_iter_stat* stats, void* stream) {

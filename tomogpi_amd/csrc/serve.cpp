@@ -123,17 +123,18 @@ namespace {
 uint32_t vec_tiles(uint64_t n) { return cdiv(n, kVecTile); }
 uint32_t lap_tiles(const DevGeom& d) { return cdiv(d.nx, 64) * cdiv(d.ny, 4) * (uint32_t)d.nz; }
 
-Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd) {
+Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo) {
     const DevGeom& d = g.dev;
     const size_t nv = g.nvox(), ns = g.nsino();
-    const bool lam = gd.lambda != 0.0;
+    const bool lam = gd.lambda != 0.0 && algo == TOMO_ALGO_GD;
+    const bool sirt = algo == TOMO_ALGO_SIRT;
     size_t sizes[24] = {0};
     int i = 0;
     sizes[i++] = (size_t)d.nphi * kViewStride * sizeof(double);          // 0 view
     sizes[i++] = (size_t)d.nphi * 2 * sizeof(float);                     // 1 view_f
     sizes[i++] = nv * 4;                                                 // 2 x
     sizes[i++] = nv * 4;                                                 // 3 dj
-    sizes[i++] = lam ? nv * 4 : 0;                                       // 4 t1
+    sizes[i++] = (lam || sirt) ? nv * 4 : 0;                             // 4 t1 (SIRT: volume weight C)
     sizes[i++] = lam ? nv * 4 : 0;                                       // 5 t2
     sizes[i++] = lam ? nv * 4 : 0;                                       // 6 t3
     sizes[i++] = zpair_elems_x(d.nx, d.ny, d.nz) * 4;                    // 7 qx
@@ -141,7 +142,7 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd) {
     sizes[i++] = ns * 4;                                                 // 9 y
     sizes[i++] = ns * 4;                                                 // 10 px
     sizes[i++] = ns * 4;                                                 // 11 r
-    sizes[i++] = ns * 4;                                                 // 12 pd
+    sizes[i++] = ns * 4;                                                 // 12 pd (SIRT: sinogram weight R)
     sizes[i++] = vi_quad_elems(d.nu, d.nv, d.nphi) * 4;                  // 13 tq
     sizes[i++] = (size_t)vec_tiles(ns) * 8;                              // 14 part_res
     sizes[i++] = (size_t)vec_tiles(nv) * 8;                              // 15 part_vol
@@ -162,7 +163,7 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd) {
 // Runs on the worker without the lock.  Returns TOMO_OK or a failure (the
 // request is then marked failed by the caller).
 int admit(tomo_server* srv, Request& q) {
-    const Layout L = layout_of(q.geom, q.req.gd);
+    const Layout L = layout_of(q.geom, q.req.gd, q.req.algorithm);
     TOMO_HIP(hipMallocFromPoolAsync(&q.block, L.total, srv->pool, srv->stream));
     char* b = (char*)q.block;
     auto at = [&](int j) { return (void*)(b + L.off[j]); };
@@ -170,10 +171,11 @@ int admit(tomo_server* srv, Request& q) {
     q.geom.d_view_f = (float*)at(1);
     q.geom.dev.view = q.geom.d_view;
     q.geom.dev.view_f = q.geom.d_view_f;
-    const bool lam = q.req.gd.lambda != 0.0;
+    const bool sirt = q.req.algorithm == TOMO_ALGO_SIRT;
+    const bool lam = q.req.gd.lambda != 0.0 && !sirt;
     q.x = (float*)at(2);
     q.dj = (float*)at(3);
-    q.t1 = lam ? (float*)at(4) : nullptr;
+    q.t1 = (lam || sirt) ? (float*)at(4) : nullptr;
     q.t2 = lam ? (float*)at(5) : nullptr;
     q.t3 = lam ? (float*)at(6) : nullptr;
     q.qx = (float*)at(7);
@@ -196,8 +198,22 @@ int admit(tomo_server* srv, Request& q) {
     TOMO_HIP(hipMemcpyAsync(q.geom.d_view_f, q.geom.view_f.data(), q.geom.view_f.size() * sizeof(float),
                             hipMemcpyHostToDevice, s));
     TOMO_HIP(hipMemcpyAsync(q.y, q.req.sinogram, ns * 4, q.host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
-    int rc = launch_fill(q.x, 0.0f, nv, s);
-    if (rc) return rc;
+    int rc;
+    if (sirt) {
+        // R = 1/(P 1) into pd, C = relaxation/(B 1) into t1, with the
+        // single-problem operators (the same arithmetic as tomo_sirt_run).
+        const Buf ones_v{q.x, TOMO_F32, 1.0f};
+        const MutBuf pd{q.pd, TOMO_F32, 1.0f};
+        if ((rc = launch_fill(q.x, 1.0f, nv, s))) return rc;
+        if (q.req.gd.pair == TOMO_PAIR_SIDDONVI) rc = launch_siddon(&q.geom, 0, ones_v, pd, s);
+        else rc = launch_rs(&q.geom, (q.req.gd.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, ones_v, pd, s);
+        if (rc) return rc;
+        if ((rc = launch_recip_weight(q.pd, 1.0f, q.pd, ns, s))) return rc;
+        if ((rc = launch_fill(q.r, 1.0f, ns, s))) return rc;
+        if ((rc = launch_vi(&q.geom, Buf{q.r, TOMO_F32, 1.0f}, MutBuf{q.t1, TOMO_F32, 1.0f}, s))) return rc;
+        if ((rc = launch_recip_weight(q.t1, (float)q.req.relaxation, q.t1, nv, s))) return rc;
+    }
+    if ((rc = launch_fill(q.x, 0.0f, nv, s))) return rc;
     double slot0[kSlotCount] = {0};
     slot0[kSlotAlphaMin] = q.req.gd.alpha_min_init;
     TOMO_HIP(hipMemcpyAsync(q.slot, slot0, sizeof(slot0), hipMemcpyHostToDevice, s));
@@ -248,15 +264,9 @@ void build_tables(tomo_server* srv, const std::vector<std::shared_ptr<Request>>&
         const tomo_gd_params& gd = q.req.gd;
         const size_t nv = q.geom.nvox(), ns = q.geom.nsino();
         const int k = q.k + 1;  // 1-based iteration about to run
-        const bool opt = k < gd.optimal_step_iteration_nb;
-        const bool lam = gd.lambda != 0.0;
+        const uint32_t pgx = cdiv(d.nu, 64), pgz = cdiv(d.nv, 4);
         const bool sid = gd.pair == TOMO_PAIR_SIDDONVI;
         const int cube = (gd.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0;
-        // Recurrent form: P x (and L(L x)) from the previous iteration's
-        // P dJ (and L(L dJ)); at k = 1 x = 0, so P x = 0 and L(L x) = 0.
-        const bool rec = recurrent_server && !gd.positivity && (k == 1 || q.prev_opt);
-        const bool rec_zero = rec && k == 1;
-        const uint32_t pgx = cdiv(d.nu, 64), pgz = cdiv(d.nv, 4);
         auto add_projection = [&](const float* vol, float* out, int opx, int opy, int ops, int opr) {
             BatchItem& zx = tb[opx].add(d, cdiv(d.nx, 32), cdiv(d.ny + 2, 32), d.nz + 1);
             zx.in = vol;
@@ -270,6 +280,44 @@ void build_tables(tomo_server* srv, const std::vector<std::shared_ptr<Request>>&
             p.out = out;
             p.flags = cube;
         };
+        if (q.req.algorithm == TOMO_ALGO_SIRT) {
+            add_projection(q.x, q.px, OP_ZPX_X, OP_ZPY_X, OP_SID_X, OP_RS_X);
+            BatchItem& df = tb[OP_DIFF].add_vec(d, ns);
+            df.in = q.y;
+            df.in2 = q.px;
+            df.in3 = q.pd;  // R
+            df.out = q.r;
+            df.aux = q.part_res;
+            BatchItem& fr = tb[OP_FIN_RES].add_one(d);
+            fr.aux = q.part_res;
+            fr.n = vec_tiles(ns);
+            fr.slot = q.slot;
+            fr.flags = kSlotResidual;
+            BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 4, 64), cdiv(d.nv + 3, 4), d.nphi);
+            vq.in = q.r;
+            vq.out = q.tq;
+            BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, 16), cdiv(d.ny, 16), cdiv(d.nz, 16));
+            vi.in = q.tq;
+            vi.out = q.dj;
+            vi.a = 1.0f;
+            BatchItem& st = tb[OP_STEP].add_one(d);
+            st.slot = q.slot;
+            st.flags = 2;
+            st.aux = q.hist;
+            st.n = (uint64_t)(k - 1);
+            BatchItem& up = tb[OP_UPDATE].add_vec(d, nv);
+            up.in = q.dj;
+            up.in3 = q.t1;  // C
+            up.out = q.x;
+            up.flags = gd.positivity ? 2 : 0;
+            continue;
+        }
+        const bool opt = k < gd.optimal_step_iteration_nb;
+        const bool lam = gd.lambda != 0.0;
+        // Recurrent form: P x (and L(L x)) from the previous iteration's
+        // P dJ (and L(L dJ)); at k = 1 x = 0, so P x = 0 and L(L x) = 0.
+        const bool rec = recurrent_server && !gd.positivity && (k == 1 || q.prev_opt);
+        const bool rec_zero = rec && k == 1;
         if (!rec) {
             add_projection(q.x, q.px, OP_ZPX_X, OP_ZPY_X, OP_SID_X, OP_RS_X);
             if (lam) {
@@ -493,7 +541,7 @@ void worker_main(tomo_server* srv) {
                 // Admission: the head of the queue while it fits.
                 while (!srv->queue.empty() && !srv->broken) {
                     auto q = srv->queue.front();
-                    const uint64_t need = layout_of(q->geom, q->req.gd).total;
+                    const uint64_t need = layout_of(q->geom, q->req.gd, q->req.algorithm).total;
                     if ((int)(running.size() + admitted.size()) >= srv->params.max_active) break;
                     if (srv->bytes_in_use + need > srv->budget) {
                         if (!running.empty() || !admitted.empty()) break;
@@ -628,6 +676,10 @@ extern "C" int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req
     if (gd.flags & TOMO_FLAG_SIDDON_COMPAT)
         return fail(TOMO_ERR_UNSUPPORTED, "the serving loop runs the clean Siddon form only");
     if (gd.gradient_iteration_nb < 0) return fail(TOMO_ERR_INVALID_ARG, "negative iteration count");
+    if (req->algorithm != TOMO_ALGO_GD && req->algorithm != TOMO_ALGO_SIRT)
+        return fail(TOMO_ERR_INVALID_ARG, "unknown algorithm");
+    if (req->algorithm == TOMO_ALGO_SIRT && !(req->relaxation > 0.0))
+        return fail(TOMO_ERR_INVALID_ARG, "SIRT relaxation must be positive");
     auto q = std::make_shared<Request>();
     q->req = *req;
     q->host = (gd.flags & TOMO_FLAG_HOST_PTR) != 0;
@@ -659,7 +711,7 @@ void fill_result(const Request& q, tomo_serve_result* res) {
     res->done_s = q.done_s;
     res->latency_s = q.state >= TOMO_REQ_DONE ? q.done_s - q.submit_s : 0.0;
     res->residual_sq = q.residual_sq;
-    res->device_bytes = layout_of(q.geom, q.req.gd).total;
+    res->device_bytes = layout_of(q.geom, q.req.gd, q.req.algorithm).total;
 }
 }  // namespace
 

@@ -96,7 +96,8 @@ struct MutBuf {
     float scale;
 };
 
-int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s);
+// fdk: FDK-weighted form (back_fdk) instead of delta * sum.
+int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk = 0);
 int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s);
 int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s);
 int launch_rs_adjoint(const tomo_geom* g, int cube, Buf sino, MutBuf vol, hipStream_t s);
@@ -112,6 +113,10 @@ int launch_diff(const float* a, const float* b, float* out, size_t n, hipStream_
 int launch_scale(float* x, float v, size_t n, hipStream_t s);
 int launch_axpy(float* y, const float* x, float a, int positive, size_t n, hipStream_t s);
 int launch_laplacian(const float* x, float* y, int nx, int ny, int nz, hipStream_t s);
+// SIRT element-wise steps: w = s > 0 ? a / s : 0; x *= w; x = x + c * b (+clamp).
+int launch_recip_weight(const float* s_in, float a, float* w, size_t n, hipStream_t s);
+int launch_mul(float* x, const float* w, size_t n, hipStream_t s);
+int launch_waxpy(float* x, const float* c, const float* b, int positive, size_t n, hipStream_t s);
 int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s);
 int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s);
 int launch_convert(Buf in, MutBuf out, size_t n, hipStream_t s);

@@ -1,7 +1,7 @@
 // tomogpi_main.cpp - command-line driver with the surface of TomoGPI_exe
 // (src/TomoGPI_exe/TomoGPI_exe.cu:152-377 and C/Iter3D_main.cu:14-421):
 //
-//   tomogpi N [--op=proj|back] [--pair=RSVI|SiddonVI] [--compute=CUDA|CPU|C|OCL]
+//   tomogpi N [--op=proj|back|back_fdk] [--pair=RSVI|SiddonVI] [--compute=CUDA|CPU|C|OCL]
 //             [--storage=mem_CPU|mem_GPU] [--precision=float|half|double]
 //             [--no_load] [--no_save] [--ngpu=K] [--nstreams=K]
 //
@@ -16,6 +16,7 @@
 // with FP32 accumulation; double runs the FP32 operators.
 //   --op=proj   loads configCT:realVolumeName, writes proj.s
 //   --op=back   loads <workdir>/<configTiff:tiffName>.s, writes back.v
+//   --op=back_fdk  same input, FDK-weighted backprojection, writes back_fdk.v
 //   (no --op)   simple-gradient reconstruction from the same sinogram, starting
 //               from configCT:initVolumeName (or zero), writes
 //               <workdir>/TomoIter/Results/Result_<date>/final_Volume.v
@@ -108,7 +109,7 @@ float fp16_scale(const std::vector<float>& x) {
 
 int main(int argc, char** argv) {
     if (argc < 2) {
-        std::printf("usage: tomogpi N [--op=proj|back] [--pair=RSVI|SiddonVI] [--compute=...] [--storage=...] "
+        std::printf("usage: tomogpi N [--op=proj|back|back_fdk] [--pair=RSVI|SiddonVI] [--compute=...] [--storage=...] "
                     "[--precision=...] [--no_load] [--no_save]\n");
         return 1;
     }
@@ -164,14 +165,15 @@ int main(int argc, char** argv) {
     clk::time_point t0, t1;
 
     // One operator call on host arrays: device staging of in/out, FP16 storage for --precision=half.
+    int bflags = 0;  // backprojector flags (TOMO_FLAG_FDK for --op=back_fdk)
     auto run_op = [&](bool forward, const std::vector<float>& in, std::vector<float>& out) -> int {
         const size_t nin = in.size(), nout = out.size();
         if (!half) {
             if (forward)
                 return tomo_project(ctx, geom, proj, pflags | TOMO_FLAG_HOST_PTR, in.data(), TOMO_F32, 1.0f,
                                     out.data(), TOMO_F32, 1.0f, nullptr);
-            return tomo_backproject(ctx, geom, TOMO_BP_VI, TOMO_FLAG_HOST_PTR, in.data(), TOMO_F32, 1.0f, out.data(),
-                                    TOMO_F32, 1.0f, nullptr);
+            return tomo_backproject(ctx, geom, TOMO_BP_VI, bflags | TOMO_FLAG_HOST_PTR, in.data(), TOMO_F32, 1.0f,
+                                    out.data(), TOMO_F32, 1.0f, nullptr);
         }
         void *d32 = nullptr, *d16 = nullptr, *o16 = nullptr, *o32 = nullptr;
         int rc = TOMO_OK;
@@ -184,7 +186,7 @@ int main(int argc, char** argv) {
         if ((rc = tomo_convert(ctx, d32, TOMO_F32, 1.0f, d16, TOMO_F16, sin_, nin, nullptr))) goto done;
         // FP16 in, FP32 accumulate, FP32 out (a per-array output scale needs the output's range)
         rc = forward ? tomo_project(ctx, geom, proj, pflags, d16, TOMO_F16, sin_, o32, TOMO_F32, sout, nullptr)
-                     : tomo_backproject(ctx, geom, TOMO_BP_VI, 0, d16, TOMO_F16, sin_, o32, TOMO_F32, sout, nullptr);
+                     : tomo_backproject(ctx, geom, TOMO_BP_VI, bflags, d16, TOMO_F16, sin_, o32, TOMO_F32, sout, nullptr);
         if (rc) goto done;
         if ((rc = tomo_synchronize(ctx, nullptr))) goto done;
         rc = tomo_copy_to_host(ctx, out.data(), o32, nout * 4);
@@ -203,14 +205,18 @@ int main(int argc, char** argv) {
         CHECK(run_op(true, vol, sino), "projection");
         t1 = clk::now();
         if (o.save) CHECK(tomo_file_write("proj.s", sino.data(), TOMO_F32, nsino), "proj.s");
-    } else if (o.have_op && o.op == "back") {
+    } else if (o.have_op && (o.op == "back" || o.op == "back_fdk")) {
+        // back_fdk: FDK-weighted VI (C/Iter3D_main.cu:296-316), written to back_fdk.v
+        const bool fdk = o.op == "back_fdk";
+        bflags = fdk ? TOMO_FLAG_FDK : 0;
         if (o.load) CHECK(tomo_file_read(sino_path.c_str(), sino.data(), TOMO_F32, nsino), sino_path);
         t0 = clk::now();
         CHECK(run_op(false, sino, vol), "backprojection");
         t1 = clk::now();
-        if (o.save) CHECK(tomo_file_write("back.v", vol.data(), TOMO_F32, nvox), "back.v");
+        const char* name = fdk ? "back_fdk.v" : "back.v";
+        if (o.save) CHECK(tomo_file_write(name, vol.data(), TOMO_F32, nvox), name);
     } else if (o.have_op) {
-        std::fprintf(stderr, "tomogpi: --op=%s is not supported (proj, back or none)\n", o.op.c_str());
+        std::fprintf(stderr, "tomogpi: --op=%s is not supported (proj, back, back_fdk or none)\n", o.op.c_str());
         return TOMO_ERR_UNSUPPORTED;
     } else {
         int iters = 0, kopt = 0, pos = 0;

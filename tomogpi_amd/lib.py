@@ -22,9 +22,10 @@ LIB_PATH = os.path.join(PKG, "libtomogpi_mi355x.so")
 TOMO_OK = 0
 F32, F16, BF16, F64 = 0, 1, 2, 3
 PROJ_RS, PROJ_SIDDON, BP_VI = 0, 1, 0
-FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR = 0x1, 0x2, 0x4
+FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR, FLAG_FDK = 0x1, 0x2, 0x4, 0x8
 PAIR_RSVI, PAIR_SIDDONVI = 0, 1
 SCHEDULE_DIRECT, SCHEDULE_RECURRENT = 0, 1
+ALGO_GD, ALGO_SIRT = 0, 1
 REQ_QUEUED, REQ_RUNNING, REQ_DONE, REQ_FAILED = 0, 1, 2, 3
 
 # Every symbol include/tomogpi.h declares.
@@ -34,7 +35,7 @@ EXPORTS = [
     "tomo_synchronize", "tomo_geom_create", "tomo_geom_desc_from_config", "tomo_geom_destroy",
     "tomo_geom_get_info", "tomo_geom_get_phi", "tomo_project", "tomo_backproject", "tomo_project_adjoint",
     "tomo_backproject_adjoint", "tomo_project_work", "tomo_convert", "tomo_diff", "tomo_scale", "tomo_axpy",
-    "tomo_sq_norm", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_phantom_shepp_logan",
+    "tomo_sq_norm", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_sirt_run", "tomo_phantom_shepp_logan",
     "tomo_phantom_ellipsoids", "tomo_file_read", "tomo_file_write", "tomo_serve_create", "tomo_serve_submit",
     "tomo_serve_poll", "tomo_serve_wait", "tomo_serve_iter_stats", "tomo_serve_get_stats", "tomo_serve_destroy",
 ]
@@ -86,8 +87,14 @@ class ServeParams(C.Structure):
                 ("schedule", C.c_int32), ("reserved", C.c_int32)]
 
 
+class SirtParams(C.Structure):
+    _fields_ = [("pair", C.c_int32), ("flags", C.c_int32), ("iterations", C.c_int32), ("positivity", C.c_int32),
+                ("relaxation", C.c_double)]
+
+
 class ServeRequest(C.Structure):
-    _fields_ = [("geom", GeomDesc), ("gd", GdParams), ("sinogram", C.c_void_p), ("volume", C.c_void_p)]
+    _fields_ = [("geom", GeomDesc), ("gd", GdParams), ("sinogram", C.c_void_p), ("volume", C.c_void_p),
+                ("algorithm", C.c_int32), ("reserved", C.c_int32), ("relaxation", C.c_double)]
 
 
 class ServeResult(C.Structure):
@@ -140,6 +147,7 @@ def load() -> C.CDLL:
         "tomo_dot": ([vp, vp, vp, sz, C.POINTER(d), vp], i),
         "tomo_laplacian": ([vp, vp, vp, vp, vp], i),
         "tomo_gd_run": ([vp, vp, C.POINTER(GdParams), vp, vp, C.POINTER(GdIterStat), vp], i),
+        "tomo_sirt_run": ([vp, vp, C.POINTER(SirtParams), vp, vp, C.POINTER(d), vp], i),
         "tomo_phantom_shepp_logan": ([vp, vp, vp, vp], i),
         "tomo_phantom_ellipsoids": ([vp, vp, i, C.c_uint64, d, d, vp, vp], i),
         "tomo_file_read": ([C.c_char_p, vp, i, sz], i), "tomo_file_write": ([C.c_char_p, vp, i, sz], i),
@@ -368,6 +376,18 @@ class Geometry:
         return [dict(residual_sq=s.residual_sq, norm_dj=s.norm_dj, norm_dj_proj=s.norm_dj_proj, alpha=s.alpha)
                 for s in stats[:iters]]
 
+    def sirt_run(self, sino, vol, pair=PAIR_RSVI, flags=0, iters=20, relaxation=1.0, positivity=0, stream=None):
+        """SIRT from x0 = 0 (tomo_sirt_run); returns the residuals sum (y - P x)^2."""
+        p = SirtParams(pair=pair, flags=flags, iterations=iters, positivity=positivity, relaxation=relaxation)
+        res = (C.c_double * max(iters, 1))()
+        ps, hs = _ptr(sino)
+        pv, hv = _ptr(vol)
+        if hs:
+            p.flags |= FLAG_HOST_PTR
+        check(self.lib.tomo_sirt_run(self.ctx.h, self.h, C.byref(p), C.c_void_p(ps), C.c_void_p(pv), res,
+                                     None if hs else C.c_void_p(_stream(stream))))
+        return list(res[:iters])
+
 
 def sq_norm(ctx: Context, x, scale=1.0, stream=None) -> float:
     out = C.c_double()
@@ -402,14 +422,14 @@ class Server:
         self._keep = {}  # request id -> arrays that must outlive the request
 
     def submit(self, desc: AcquisitionDesc, sino, vol, pair=PAIR_RSVI, flags=0, iters=20, kopt=21, lam=0.01,
-               positivity=0, alpha_min_init=-0.0347711) -> int:
+               positivity=0, alpha_min_init=-0.0347711, algorithm=ALGO_GD, relaxation=1.0) -> int:
         ps, hs = _ptr(sino)
         pv, hv = _ptr(vol)
         if hs != hv:
             raise ValueError("sinogram and volume must both be host or both be device arrays")
         gd = GdParams(pair=pair, flags=flags | (FLAG_HOST_PTR if hs else 0), gradient_iteration_nb=iters,
                       optimal_step_iteration_nb=kopt, lam=lam, positivity=positivity, alpha_min_init=alpha_min_init)
-        r = ServeRequest(geom=desc.to_c(), gd=gd, sinogram=ps, volume=pv)
+        r = ServeRequest(geom=desc.to_c(), gd=gd, sinogram=ps, volume=pv, algorithm=algorithm, relaxation=relaxation)
         rid = C.c_uint64()
         check(self.lib.tomo_serve_submit(self.h, C.byref(r), C.byref(rid)))
         self._keep[rid.value] = (sino, vol)
