@@ -234,6 +234,18 @@ int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* para
                 const float* sinogram, float* volume, tomo_gd_iter_stat* stats,
                 void* stream);
 
+/* Conjugate gradient on the same criterion and step rule (the configIteration
+ * key "gradient Conjugate"; reference: Iter3D::doConjugateGradient,
+ * C/Iter3D_Algo_Gradient.cu:375-559, whose direction starts at 0 and whose
+ * beta divides by that zero norm at iteration 2, so the method is defined
+ * here):  g = the loop's dJ;  d = g, then d = beta d + g with
+ * beta = |g|^2/|g_prev|^2 (Fletcher-Reeves);  alpha = -(g.d) /
+ * (2|P d|^2 + lambda 2|L d|^2);  x += alpha d.  optimal_step_iteration_nb and
+ * alpha_min_init are not used; stats hold |g|^2 in norm_dj.  Synchronous.    */
+int tomo_cg_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* params,
+                const float* sinogram, float* volume, tomo_gd_iter_stat* stats,
+                void* stream);
+
 /* ---- SIRT ----------------------------------------------------------------- */
 /* No counterpart in the reference; defined here for batched benchmarking:
  *   R = 1 / (P 1) on the sinogram, C = relaxation / (B 1) on the volume (0

@@ -113,6 +113,16 @@ typedef struct og_gd_stat {
 int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino,
               float* vol, og_gd_stat* stats);
 
+/* Conjugate gradient on the criterion of og_gd_run (defined by this build:
+ * the reference's doConjugateGradient, C/Iter3D_Algo_Gradient.cu:375-559,
+ * starts its direction at 0 and divides by its zero norm at iteration 2):
+ *   g = -2 B (y - P x) + 2 lambda L(L x)       (the loop's dJ)
+ *   d = g (k = 1),  d = g + beta d,  beta = |g|^2 / |g_prev|^2 (Fletcher-Reeves)
+ *   alpha = -(g . d) / (2 |P d|^2 + lambda 2 |L d|^2);  x += alpha d
+ * stats: residual_sq, norm_dj = |g|^2, norm_dj_proj = the denominator, alpha.
+ * kopt and alpha_min_init are not used.                                      */
+int og_cg_run(const og_geom* g, const og_gd_params* p, const float* sino, float* vol, og_gd_stat* stats);
+
 /* SIRT, defined by this build (the reference has none):
  *   R = 1 / (P 1) on the sinogram, C = relax / (B 1) on the volume (0 where
  *   the denominator is 0), x0 = 0,

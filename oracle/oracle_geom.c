@@ -307,6 +307,65 @@ int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
     return 0;
 }
 
+static double dot_f32(const float* a, const float* b, size_t n) {
+    double s = 0.0;
+#pragma omp parallel for reduction(+ : s) schedule(static)
+    for (size_t i = 0; i < n; ++i) s += (double)a[i] * (double)b[i];
+    return s;
+}
+
+int og_cg_run(const og_geom* g, const og_gd_params* p, const float* sino, float* vol, og_gd_stat* stats) {
+    const size_t nv = (size_t)g->nx * g->ny * g->nz, ns = (size_t)g->nu * g->nv * g->nphi;
+    float* est = (float*)malloc(sizeof(float) * ns);
+    float* gr = (float*)malloc(sizeof(float) * nv);
+    float* d = (float*)malloc(sizeof(float) * nv);
+    float* t1 = (float*)malloc(sizeof(float) * nv);
+    float* t2 = (float*)malloc(sizeof(float) * nv);
+    if (!est || !gr || !d || !t1 || !t2) { free(est); free(gr); free(d); free(t1); free(t2); return 1; }
+    double gg_prev = 0.0;
+    for (int k = 1; k <= p->iters; ++k) {
+        og_gd_stat st = {0, 0, 0, 0};
+        project(g, p, vol, est);
+        for (size_t i = 0; i < ns; ++i) est[i] = sino[i] - est[i];
+        st.residual_sq = sq_norm(est, ns);
+        backproject(g, p, est, gr);
+        for (size_t i = 0; i < nv; ++i) gr[i] = gr[i] * -2.0f;
+        if (p->lambda != 0.0) {
+            og_laplacian_f32(g->nx, g->ny, g->nz, vol, t1);
+            og_laplacian_f32(g->nx, g->ny, g->nz, t1, t2);
+            const float w = (float)(2.0 * (double)(float)p->lambda);
+            for (size_t i = 0; i < nv; ++i) gr[i] = gr[i] + w * t2[i];
+        }
+        const double gg = sq_norm(gr, nv);
+        if (k == 1) {
+            for (size_t i = 0; i < nv; ++i) d[i] = gr[i];
+        } else {
+            const float bf = (float)(gg / gg_prev);
+            for (size_t i = 0; i < nv; ++i) d[i] = bf * d[i] + gr[i];
+        }
+        gg_prev = gg;
+        const double gd = dot_f32(gr, d, nv);
+        project(g, p, d, est);
+        double reg = 0.0;
+        if (p->lambda != 0.0) {
+            og_laplacian_f32(g->nx, g->ny, g->nz, d, t1);
+            reg = 2.0 * sq_norm(t1, nv);
+        }
+        st.norm_dj = gg;
+        st.norm_dj_proj = 2.0 * sq_norm(est, ns) + p->lambda * reg;
+        st.alpha = -gd / st.norm_dj_proj;
+        const float af = (float)st.alpha;
+        for (size_t i = 0; i < nv; ++i) {
+            float v = vol[i] + af * d[i];
+            if (p->positivity && v < 0.0f) v = 0.0f;
+            vol[i] = v;
+        }
+        if (stats) stats[k - 1] = st;
+    }
+    free(est); free(gr); free(d); free(t1); free(t2);
+    return 0;
+}
+
 /* SIRT weights: rw = 1/(P 1) (sinogram), cw = (float)relax / (B 1) (volume),
  * 0 where the sum is not positive.  FP32 arithmetic.                        */
 int og_sirt_weights(const og_geom* g, const og_sirt_params* p, float* rw, float* cw) {

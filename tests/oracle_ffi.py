@@ -148,6 +148,8 @@ class Oracle:
             lib.og_ellipsoids.argtypes = [C.c_int] * 4 + [C.c_uint64, C.c_double, C.c_double, fp]
             lib.og_ellipsoid_table.argtypes = [C.c_int] * 4 + [C.c_uint64, C.c_double, C.c_double, dp, dp]
             lib.og_gd_run.argtypes = [G, C.POINTER(OgGdParams), fp, fp, C.POINTER(OgGdStat)]
+            lib.og_cg_run.argtypes = [G, C.POINTER(OgGdParams), fp, fp, C.POINTER(OgGdStat)]
+            lib.og_cg_run.restype = C.c_int
             lib.og_sirt_weights.argtypes = [G, C.POINTER(OgSirtParams), fp, fp]
             lib.og_sirt_weights.restype = C.c_int
             lib.og_sirt_run.argtypes = [G, C.POINTER(OgSirtParams), fp, fp, dp]
@@ -275,6 +277,16 @@ class Oracle:
         stats = (OgGdStat * max(iters, 1))()
         rc = self.lib().og_gd_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(v), stats)
         assert rc == 0
+        return v, [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj,
+                        alpha=x.alpha) for x in stats[:iters]]
+
+    def cg(self, sino, vol0=None, pair=0, cube=False, iters=3, lam=0.0, positivity=0, matched=False):
+        s = np.ascontiguousarray(sino, dtype=np.float32)
+        v = self._vol(np.float32) if vol0 is None else np.array(vol0, dtype=np.float32, order="C")
+        p = OgGdParams(pair=pair, cube=int(cube), compat=0, iters=iters, kopt=0, lam=lam, positivity=positivity,
+                       alpha_min_init=0.0, matched=int(matched))
+        stats = (OgGdStat * max(iters, 1))()
+        assert self.lib().og_cg_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(v), stats) == 0
         return v, [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj,
                         alpha=x.alpha) for x in stats[:iters]]
 

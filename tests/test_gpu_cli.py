@@ -18,7 +18,7 @@ N, NPHI = 32, 16
 
 
 def write_workdir(tmp, compute_line="compute COMPUTE_CUDA", iters=3, kopt=10, lam=0.01, init="none",
-                  real="none"):
+                  real="none", gradient="Simple"):
     wd = tmp / f"Projections_{N}"
     wd.mkdir()
     (wd / "configCT").write_text(
@@ -31,7 +31,7 @@ def write_workdir(tmp, compute_line="compute COMPUTE_CUDA", iters=3, kopt=10, la
     (wd / "configIteration").write_text(
         f"[Iteration]\nglobalIterationNb 1\ngradientIterationNb {iters}\noptimalStepIterationNb {kopt}\n"
         f"[PenaltyFunction]\nregularizationParameter {lam}\npenaltyFunction GGaussian (Huber, GGaussian)\n"
-        "gradient Simple (Simple,Conjugate)\nhuberThreshold 0.0001\npositivity 0\nbeta 2.0\n")
+        f"gradient {gradient} (Simple,Conjugate)\nhuberThreshold 0.0001\npositivity 0\nbeta 2.0\n")
     (wd / "configComputeArchitecture").write_text(
         "[precision]\nprecision float (half,float,double)\nstorage mem_CPU (mem_CPU,mem_GPU)\n"
         f"{compute_line}\narchitecture ARCHITECTURE_GPU\npair PAIR_RSVI (PAIR_SFTR,PAIR_RSVI,PAIR_SIDDONVI)\n")
@@ -103,6 +103,18 @@ def test_cli_iterative_reconstruction(tmp_path):
     assert np.linalg.norm(final - ref) / np.linalg.norm(ref) <= 1e-6
     assert (res[0] / "configCT").exists() and (res[0] / "Iteration.dat").exists()
     assert "Gradient iteration 3" in out
+
+
+def test_cli_conjugate_gradient(tmp_path):
+    o = Oracle(geom())
+    sino = o.rs(ellipsoids(N, count=5, seed=2, amin=2.0, amax=8.0), cube=True)
+    wd = write_workdir(tmp_path, iters=3, lam=0.01, gradient="Conjugate")
+    sino.tofile(wd / "P_ER_GPU.s")
+    run(tmp_path)
+    res = list((wd / "TomoIter" / "Results").iterdir())
+    final = np.fromfile(res[0] / "final_Volume.v", dtype=np.float32).reshape(o.geom.vol_shape)
+    ref, _ = o.cg(sino, pair=0, cube=True, iters=3, lam=0.01)
+    assert np.linalg.norm(final - ref) / np.linalg.norm(ref) <= 1e-6
 
 
 def test_cli_missing_sinogram_is_an_error(tmp_path):

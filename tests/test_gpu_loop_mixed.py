@@ -134,3 +134,21 @@ def test_sirt_matches_oracle(small, pair, flags):
     assert np.array_equal(vol.cpu().numpy(), ref)
     assert res == pytest.approx(ref_res, rel=1e-9)
     assert res[-1] < res[0]
+
+
+@pytest.mark.parametrize("pair,lam", [(lib.PAIR_RSVI, 0.01), (lib.PAIR_SIDDONVI, 0.0)])
+def test_cg_loop_matches_oracle(small, pair, lam):
+    """Conjugate gradient (tomo_cg_run) against the oracle's loop: same
+    operators; FP64 norms and the g.d product summed in a different order."""
+    g, o, G = small
+    truth = ellipsoids(g.nx, g.ny, g.nz, count=6, seed=4, amin=2.0, amax=9.0)
+    sino = o.rs(truth) if pair == lib.PAIR_RSVI else o.siddon(truth)
+    ref, ref_stats = o.cg(sino, pair=pair, iters=5, lam=lam)
+    vol = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+    stats = G.gd_run(torch.from_numpy(sino).cuda(), vol, pair=pair, iters=5, lam=lam, conjugate=True)
+    torch.cuda.synchronize()
+    for a, b in zip(stats, ref_stats):
+        for k in ("alpha", "residual_sq", "norm_dj", "norm_dj_proj"):
+            assert a[k] == pytest.approx(b[k], rel=1e-8, abs=1e-12), (k, a, b)
+    assert rel_l2(vol.cpu().numpy(), ref) <= 1e-6
+    assert stats[-1]["residual_sq"] < stats[0]["residual_sq"]

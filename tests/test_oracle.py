@@ -28,6 +28,9 @@ them retypes the oracle's own formula:
             invariants and voxel membership.
 * FDK VI  - constant-sinogram closed form with the FDK weight evaluated on the
             vector construction (source distance, detector offsets).
+* CG      - with the exact transpose, every step is an exact line search on
+            the quadratic criterion; the first step is the gradient step; CG
+            ends below the gradient loop.
 * SIRT    - first iteration composed from the pinned operators (weights,
             relaxation, sign); exact homogeneity; convergence on consistent
             data.
@@ -397,6 +400,30 @@ def test_loop_is_exact_line_search_with_matched_pair():
         predicted = st["norm_dj"] ** 2 / (2.0 * st["norm_dj_proj"])
         assert Js[k] - Js[k + 1] == pytest.approx(predicted, rel=2e-3)
     assert all(a > b for a, b in zip(Js, Js[1:]))
+
+
+def test_cg_exact_line_search_and_beats_gradient_with_matched_pair():
+    """Conjugate gradient (Fletcher-Reeves) on J = |y-Px|^2 + lambda |Lx|^2 with
+    the exact transpose: each step lowers J by |g|^4 / (2 d'Hd) (exact line
+    search with g'd = |g|^2), the first step is the gradient step, and after
+    a few iterations J is below the gradient loop's."""
+    g = scaled_geometry(20, 10)
+    o = Oracle(g)
+    y = o.rs(ellipsoids(20, count=5, seed=1, amin=3.0, amax=7.0))
+    lam = 0.05
+    Js = [_J(o, g, np.zeros(g.vol_shape, dtype=np.float32), y, lam)]
+    for k in range(1, 6):
+        x, st = o.cg(y, pair=0, iters=k, lam=lam, matched=True)
+        Js.append(_J(o, g, x, y, lam))
+    for k in range(5):
+        predicted = st[k]["norm_dj"] ** 2 / (2.0 * st[k]["norm_dj_proj"])
+        assert Js[k] - Js[k + 1] == pytest.approx(predicted, rel=2e-3)
+    x1_cg, s1 = o.cg(y, pair=0, iters=1, lam=lam, matched=True)
+    x1_gd, g1 = o.gd(y, pair=0, iters=1, kopt=10, lam=lam, matched=True)
+    assert s1[0]["alpha"] == pytest.approx(g1[0]["alpha"], rel=1e-12)
+    assert np.abs(x1_cg - x1_gd).max() <= 1e-6 * np.abs(x1_gd).max()
+    xg, _ = o.gd(y, pair=0, iters=5, kopt=10, lam=lam, matched=True)
+    assert Js[5] < _J(o, g, xg, y, lam)
 
 
 def test_loop_alpha_min_rule():

@@ -91,6 +91,72 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
     return TOMO_OK;
 }
 
+// Conjugate gradient on the criterion of gd_run (Fletcher-Reeves):
+//   g = -2 B (y - P x) + 2 lambda L(L x)
+//   d = g (k = 1),  d = beta d + g,  beta = |g|^2 / |g_prev|^2
+//   alpha = -(g . d) / (2 |P d|^2 + lambda 2 |L d|^2);  x += alpha d
+// The reference's doConjugateGradient (C/Iter3D_Algo_Gradient.cu:375-559)
+// starts its direction at 0 and divides by that zero norm at iteration 2,
+// so this loop defines the method instead of restating it.
+int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const float* sino, float* vol,
+           tomo_gd_iter_stat* stats, hipStream_t s) {
+    const size_t nv = g->nvox(), ns = g->nsino();
+    DevBuf est, gb, db, t1, t2;
+    TOMO_HIP(hipMalloc(&est.p, ns * sizeof(float)));
+    TOMO_HIP(hipMalloc(&gb.p, nv * sizeof(float)));
+    TOMO_HIP(hipMalloc(&db.p, nv * sizeof(float)));
+    TOMO_HIP(hipMalloc(&t1.p, nv * sizeof(float)));
+    if (pr.lambda != 0.0) TOMO_HIP(hipMalloc(&t2.p, nv * sizeof(float)));
+    float* e = (float*)est.p;
+    float* gr = (float*)gb.p;
+    float* d = (float*)db.p;
+    float* a = (float*)t1.p;
+    float* b = (float*)t2.p;
+    const int nx = g->info.nx, ny = g->info.ny, nz = g->info.nz;
+    double gg_prev = 0.0;
+    int rc;
+    for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
+        tomo_gd_iter_stat st{0, 0, 0, 0};
+        if ((rc = project(g, pr, vol, e, s))) return rc;
+        if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
+        if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
+        if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{gr, TOMO_F32, 1.0f}, s))) return rc;
+        if ((rc = launch_scale(gr, -2.0f, nv, s))) return rc;
+        if (pr.lambda != 0.0) {
+            if ((rc = launch_laplacian(vol, a, nx, ny, nz, s))) return rc;
+            if ((rc = launch_laplacian(a, b, nx, ny, nz, s))) return rc;
+            if ((rc = launch_axpy(gr, b, (float)(2.0 * (double)(float)pr.lambda), 0, nv, s))) return rc;
+        }
+        double gg = 0.0;
+        if ((rc = reduce_sq_norm(ctx, Buf{gr, TOMO_F32, 1.0f}, nv, &gg, s))) return rc;
+        if (k == 1) {
+            TOMO_HIP(hipMemcpyAsync(d, gr, nv * sizeof(float), hipMemcpyDeviceToDevice, s));
+        } else {
+            if ((rc = launch_scale(d, (float)(gg / gg_prev), nv, s))) return rc;
+            if ((rc = launch_axpy(d, gr, 1.0f, 0, nv, s))) return rc;
+        }
+        gg_prev = gg;
+        double gd = 0.0;
+        if ((rc = reduce_dot(ctx, gr, d, nv, &gd, s))) return rc;
+        if ((rc = project(g, pr, d, e, s))) return rc;
+        double pe = 0.0, reg = 0.0;
+        if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &pe, s))) return rc;
+        if (pr.lambda != 0.0) {
+            if ((rc = launch_laplacian(d, a, nx, ny, nz, s))) return rc;
+            double sq = 0.0;
+            if ((rc = reduce_sq_norm(ctx, Buf{a, TOMO_F32, 1.0f}, nv, &sq, s))) return rc;
+            reg = 2.0 * sq;
+        }
+        st.norm_dj = gg;
+        st.norm_dj_proj = 2.0 * pe + pr.lambda * reg;
+        st.alpha = -gd / st.norm_dj_proj;
+        if ((rc = launch_axpy(vol, d, (float)st.alpha, pr.positivity, nv, s))) return rc;
+        if (stats) stats[k - 1] = st;
+    }
+    TOMO_HIP(hipStreamSynchronize(s));
+    return TOMO_OK;
+}
+
 // SIRT (defined by this library; the reference has none):
 //   R = 1/(P 1), C = relaxation/(B 1), x0 = 0,
 //   r = (y - P x) * R;  x = x + C * (B r)  (+ clamp at 0).
@@ -156,8 +222,11 @@ extern "C" int tomo_sirt_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_sirt_
     return TOMO_OK;
 }
 
-extern "C" int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram,
-                           float* volume, tomo_gd_iter_stat* stats, void* stream) {
+namespace {
+// Shared entry of tomo_gd_run / tomo_cg_run: argument checks and host staging.
+template <typename Run>
+int run_loop(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram, float* volume,
+             tomo_gd_iter_stat* stats, void* stream, Run run) {
     using namespace tomo;
     if (!ctx || !g || !params || !sinogram || !volume) return fail(TOMO_ERR_INVALID_ARG, "null argument");
     TOMO_HIP(hipSetDevice(ctx->device));
@@ -165,18 +234,27 @@ extern "C" int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_para
         return fail(TOMO_ERR_INVALID_ARG, "unknown pair");
     if (params->gradient_iteration_nb < 0) return fail(TOMO_ERR_INVALID_ARG, "negative iteration count");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
-    if (!(params->flags & TOMO_FLAG_HOST_PTR)) return gd_run(ctx, g, *params, sinogram, volume, stats, s);
-    // Host-resident data (the reference's mem_CPU storage): stage in, run,
-    // copy the volume back.
+    if (!(params->flags & TOMO_FLAG_HOST_PTR)) return run(ctx, g, *params, sinogram, volume, stats, s);
     const size_t nv = g->nvox(), ns = g->nsino();
     DevBuf ds, dv;
     TOMO_HIP(hipMalloc(&ds.p, ns * sizeof(float)));
     TOMO_HIP(hipMalloc(&dv.p, nv * sizeof(float)));
     TOMO_HIP(hipMemcpyAsync(ds.p, sinogram, ns * sizeof(float), hipMemcpyHostToDevice, s));
     TOMO_HIP(hipMemcpyAsync(dv.p, volume, nv * sizeof(float), hipMemcpyHostToDevice, s));
-    int rc = gd_run(ctx, g, *params, (const float*)ds.p, (float*)dv.p, stats, s);
+    int rc = run(ctx, g, *params, (const float*)ds.p, (float*)dv.p, stats, s);
     if (rc) return rc;
     TOMO_HIP(hipMemcpyAsync(volume, dv.p, nv * sizeof(float), hipMemcpyDeviceToHost, s));
     TOMO_HIP(hipStreamSynchronize(s));
     return TOMO_OK;
+}
+}  // namespace
+
+extern "C" int tomo_cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram,
+                           float* volume, tomo_gd_iter_stat* stats, void* stream) {
+    return run_loop(ctx, g, params, sinogram, volume, stats, stream, tomo::cg_run);
+}
+
+extern "C" int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram,
+                           float* volume, tomo_gd_iter_stat* stats, void* stream) {
+    return run_loop(ctx, g, params, sinogram, volume, stats, stream, tomo::gd_run);
 }

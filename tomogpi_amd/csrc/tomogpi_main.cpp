@@ -17,7 +17,8 @@
 //   --op=proj   loads configCT:realVolumeName, writes proj.s
 //   --op=back   loads <workdir>/<configTiff:tiffName>.s, writes back.v
 //   --op=back_fdk  same input, FDK-weighted backprojection, writes back_fdk.v
-//   (no --op)   simple-gradient reconstruction from the same sinogram, starting
+//   (no --op)   simple-gradient (configIteration gradient Simple) or conjugate
+//               gradient (gradient Conjugate) reconstruction from the same sinogram, starting
 //               from configCT:initVolumeName (or zero), writes
 //               <workdir>/TomoIter/Results/Result_<date>/final_Volume.v
 // Every failure prints a message and returns a nonzero status.
@@ -229,8 +230,8 @@ int main(int argc, char** argv) {
         const std::string grad = cfg(work, "configIteration", "gradient", "Simple");
         const std::string pen = cfg(work, "configIteration", "penaltyFunction", "GGaussian");
         tomo::config_double(ci, "beta", &beta);
-        if (grad != "Simple") {
-            std::fprintf(stderr, "tomogpi: gradient %s is not supported (Simple)\n", grad.c_str());
+        if (grad != "Simple" && grad != "Conjugate") {
+            std::fprintf(stderr, "tomogpi: gradient %s is not supported (Simple, Conjugate)\n", grad.c_str());
             return TOMO_ERR_UNSUPPORTED;
         }
         if (lambda != 0.0 && (pen != "GGaussian" || beta != 2.0)) {
@@ -263,7 +264,10 @@ int main(int argc, char** argv) {
         gp.alpha_min_init = -0.0347711;
         std::vector<tomo_gd_iter_stat> st(iters > 0 ? iters : 1);
         t0 = clk::now();
-        CHECK(tomo_gd_run(ctx, geom, &gp, sino.data(), vol.data(), st.data(), nullptr), "gradient descent");
+        if (grad == "Conjugate")
+            CHECK(tomo_cg_run(ctx, geom, &gp, sino.data(), vol.data(), st.data(), nullptr), "conjugate gradient");
+        else
+            CHECK(tomo_gd_run(ctx, geom, &gp, sino.data(), vol.data(), st.data(), nullptr), "gradient descent");
         t1 = clk::now();
         FILE* log = std::fopen((res + "Iteration.dat").c_str(), "w");
         if (log) std::fprintf(log, "IterNb residual_sq normdJ normdJProj alpha\n");
