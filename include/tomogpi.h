@@ -274,9 +274,10 @@ int tomo_sirt_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_sirt_params* 
  * in arrival order while the device memory they need fits the budget and a
  * slot is free; admitted requests keep volumes, sinograms, geometry and loop
  * scalars resident on the device.  The server advances every admitted request
- * by one iteration per round, each operator of the round being one shared
- * persistent launch over all of them (optionally replayed from a captured
- * hipGraph); requests join and leave between rounds.  A request's arithmetic
+ * by one or more iterations per round (more for cheaper requests), each
+ * operator of an iteration being one shared persistent launch over all of
+ * them (optionally replayed from captured hipGraphs); requests join and leave
+ * between rounds.  A request's arithmetic
  * does not depend on what shares its launches, so its result is the same
  * alone and in any mix.  A request is a simple-gradient reconstruction or a
  * SIRT reconstruction (tomo_sirt_run).  Storage is FP32.                     */
@@ -295,7 +296,12 @@ typedef struct tomo_serve_params {
     int32_t  max_active;        /* requests resident at once, 1..64         */
     int32_t  use_graph;         /* replay each round from a captured graph  */
     int32_t  schedule;          /* TOMO_SCHEDULE_*                          */
-    int32_t  reserved;
+    int32_t  sub_rounds;        /* iterations per round of the cheapest
+                                   requests, 1..8 (0 = 8): each round gives a
+                                   request about (work of the largest admitted
+                                   request) / (its work) iterations, so small
+                                   requests finish early; 1 = one iteration
+                                   per request per round                     */
 } tomo_serve_params;
 
 #define TOMO_ALGO_GD    0

@@ -2,8 +2,9 @@
 
 * Every request's result is the same alone and in a mix of uneven sizes
   (bit-identical: a problem's arithmetic does not depend on what shares its
-  launches), with and without graph replay, and under a memory budget that
-  admits only part of the stream at a time.
+  launches), with and without graph replay, with one or several iterations
+  per round, and under a memory budget that admits only part of the stream
+  at a time.
 * Direct schedule vs tomo_gd_run (the plain loop): the operators and vector
   ops are the same code; only the FP64 norms are summed in a different order
   (per-tile partials vs grid-stride partials), so volumes agree to rel-L2
@@ -81,10 +82,10 @@ def alone(ctx, problems):
     return out
 
 
-@pytest.mark.parametrize("use_graph", [True, False])
-def test_mix_equals_alone(ctx, problems, alone, use_graph):
+@pytest.mark.parametrize("use_graph,sub_rounds", [(True, 0), (False, 0), (True, 1)])
+def test_mix_equals_alone(ctx, problems, alone, use_graph, sub_rounds):
     idx = list(range(len(CASES)))
-    vols, res, hist, st = _run(ctx, problems, idx, max_active=8, use_graph=use_graph)
+    vols, res, hist, st = _run(ctx, problems, idx, max_active=8, use_graph=use_graph, sub_rounds=sub_rounds)
     assert st["completed"] == len(idx) and st["failed"] == 0
     assert st["peak_active"] == len(idx)
     for i in idx:

@@ -9,13 +9,14 @@
 // work volumes, z-pair copies, sinograms y / P x / residual / P dJ, the VI
 // tap-quad copy, FP64 partial sums, loop scalars) and stages its inputs.
 //
-// A round advances every admitted request by one iteration of the loop of
-// tomo_gd_run (C/Iter3D_Algo_Gradient.cu:25-223 with the beta=2 regulariser,
-// C/GeneralizedGaussianRegularizer_CPU.cu:158-226).  Each operator of the
-// round is one shared persistent launch over all requests that take part in
-// it; the launch reads its problem table from device memory, so the host
-// writes the tables (one copy) and replays the same launch sequence, captured
-// once as a hipGraph, every round.  Norms and the step size stay on the
+// A round advances every admitted request by one or more iterations of the
+// loop of tomo_gd_run (C/Iter3D_Algo_Gradient.cu:25-223 with the beta=2
+// regulariser, C/GeneralizedGaussianRegularizer_CPU.cu:158-226) or of
+// tomo_sirt_run; cheaper requests get more iterations per round.  Each
+// operator of an iteration is one shared persistent launch over all requests
+// that take part in it; the launch reads its problem table from device
+// memory, so the host writes the tables (one copy per round) and replays the
+// same launch sequences, captured once as hipGraphs, every round.  Norms and the step size stay on the
 // device (FP64 per-tile partials summed in a fixed order, then a step
 // kernel), so a round needs one host synchronisation, at its end, where
 // finished requests leave and queued ones join.
@@ -37,6 +38,7 @@ namespace tomo {
 namespace {
 
 constexpr size_t kAlign = 256;
+constexpr int kMaxSubRounds = 8;
 
 uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
@@ -93,9 +95,10 @@ struct tomo_server {
     tomo_serve_params params{};
     hipStream_t stream = nullptr;
     hipMemPool_t pool = nullptr;           // request memory (kept cached between requests)
-    tomo::BatchTable* d_tables = nullptr;  // OP_COUNT tables
+    tomo::BatchTable* d_tables = nullptr;  // kMaxSubRounds sets of OP_COUNT tables
     tomo::BatchTable* h_tables = nullptr;  // pinned mirror
-    hipGraphExec_t graph = nullptr;
+    hipGraphExec_t graph[8] = {};          // one captured round per table set
+    int sub_rounds = 8;
     std::thread worker;
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
@@ -251,19 +254,22 @@ struct TableBuilder {
     BatchItem& add_one(const DevGeom& g) { return add(g, 1, 1, 1); }
 };
 
-void build_tables(tomo_server* srv, const std::vector<std::shared_ptr<Request>>& running) {
+// Tables of table set `set` for one iteration of every request in `running`
+// (advances each request's iteration count).
+void build_tables(tomo_server* srv, int set, const std::vector<Request*>& running) {
     TableBuilder tb[OP_COUNT];
     for (int o = 0; o < OP_COUNT; ++o) {
-        tb[o].t = srv->h_tables + o;
+        tb[o].t = srv->h_tables + (size_t)set * OP_COUNT + o;
         tb[o].reset();
     }
     const bool recurrent_server = srv->params.schedule == TOMO_SCHEDULE_RECURRENT;
-    for (auto& qp : running) {
+    for (Request* qp : running) {
         Request& q = *qp;
+        ++q.k;  // this set runs iteration q.k (1-based) below
         const DevGeom& d = q.geom.dev;
         const tomo_gd_params& gd = q.req.gd;
         const size_t nv = q.geom.nvox(), ns = q.geom.nsino();
-        const int k = q.k + 1;  // 1-based iteration about to run
+        const int k = q.k;  // 1-based iteration of this set
         const uint32_t pgx = cdiv(d.nu, 64), pgz = cdiv(d.nv, 4);
         const bool sid = gd.pair == TOMO_PAIR_SIDDONVI;
         const int cube = (gd.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0;
@@ -429,9 +435,9 @@ void build_tables(tomo_server* srv, const std::vector<std::shared_ptr<Request>>&
     }
 }
 
-// The fixed launch sequence of a round (every table, empty or not).
-int launch_round(tomo_server* srv) {
-    const BatchTable* T = srv->d_tables;
+// The fixed launch sequence of one iteration (every table, empty or not).
+int launch_round(tomo_server* srv, int set) {
+    const BatchTable* T = srv->d_tables + (size_t)set * OP_COUNT;
     hipStream_t s = srv->stream;
     hipError_t e = hipSuccess;
 #define TOMO_STEP(call) \
@@ -466,31 +472,59 @@ int launch_round(tomo_server* srv) {
     return TOMO_OK;
 }
 
+// One round: request q runs m_q iterations, m_q = W_max / W_q clamped to
+// [1, sub_rounds] and to its remaining iterations, with W = n^3 Nphi (the
+// operators' work) and W_max over the admitted requests; iteration j of the
+// round is table set j, whose launches include the requests with m_q > j.
+// Cheap requests thus finish in few rounds beside expensive ones, and every
+// request's arithmetic is unchanged.
 int run_round(tomo_server* srv, const std::vector<std::shared_ptr<Request>>& running) {
-    build_tables(srv, running);
+    auto work = [](const Request& q) { return (double)q.geom.nvox() * (double)q.geom.info.nphi; };
+    double wmax = 0.0;
+    for (auto& q : running)
+        if (q->k < q->req.gd.gradient_iteration_nb) wmax = std::max(wmax, work(*q));
+    std::vector<int> m(running.size(), 0);
+    int sets = 0;
+    for (size_t i = 0; i < running.size(); ++i) {
+        const Request& q = *running[i];
+        const int left = q.req.gd.gradient_iteration_nb - q.k;
+        if (left <= 0) continue;
+        const double ratio = wmax / work(q);
+        const int want = ratio >= (double)srv->sub_rounds ? srv->sub_rounds : std::max(1, (int)ratio);
+        m[i] = std::min(want, left);
+        sets = std::max(sets, m[i]);
+    }
+    std::vector<Request*> part;
+    for (int j = 0; j < sets; ++j) {
+        part.clear();
+        for (size_t i = 0; i < running.size(); ++i)
+            if (m[i] > j) part.push_back(running[i].get());
+        build_tables(srv, j, part);
+    }
     hipStream_t s = srv->stream;
-    TOMO_HIP(hipMemcpyAsync(srv->d_tables, srv->h_tables, sizeof(BatchTable) * OP_COUNT, hipMemcpyHostToDevice, s));
-    if (srv->params.use_graph) {
-        if (!srv->graph) {
-            hipGraph_t g = nullptr;
-            TOMO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-            const int rc = launch_round(srv);
-            hipGraph_t captured = nullptr;
-            const hipError_t ee = hipStreamEndCapture(s, &captured);
-            if (rc) {
-                if (captured) (void)hipGraphDestroy(captured);
-                return rc;
+    TOMO_HIP(hipMemcpyAsync(srv->d_tables, srv->h_tables, sizeof(BatchTable) * OP_COUNT * (size_t)sets,
+                            hipMemcpyHostToDevice, s));
+    for (int j = 0; j < sets; ++j) {
+        if (srv->params.use_graph) {
+            if (!srv->graph[j]) {
+                TOMO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+                const int rc = launch_round(srv, j);
+                hipGraph_t captured = nullptr;
+                const hipError_t ee = hipStreamEndCapture(s, &captured);
+                if (rc) {
+                    if (captured) (void)hipGraphDestroy(captured);
+                    return rc;
+                }
+                TOMO_HIP(ee);
+                const hipError_t ie = hipGraphInstantiate(&srv->graph[j], captured, nullptr, nullptr, 0);
+                (void)hipGraphDestroy(captured);
+                TOMO_HIP(ie);
             }
-            TOMO_HIP(ee);
-            g = captured;
-            const hipError_t ie = hipGraphInstantiate(&srv->graph, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            TOMO_HIP(ie);
+            TOMO_HIP(hipGraphLaunch(srv->graph[j], s));
+        } else {
+            const int rc = launch_round(srv, j);
+            if (rc) return rc;
         }
-        TOMO_HIP(hipGraphLaunch(srv->graph, s));
-    } else {
-        const int rc = launch_round(srv);
-        if (rc) return rc;
     }
     TOMO_HIP(hipStreamSynchronize(s));
     return TOMO_OK;
@@ -599,7 +633,6 @@ void worker_main(tomo_server* srv) {
                 srv->cv_done.notify_all();
                 continue;
             }
-            for (auto& q : running) ++q->k;
         }
         // Finished requests leave: result copied out, memory released.
         for (size_t i = 0; i < running.size();) {
@@ -629,10 +662,13 @@ extern "C" int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params,
         return fail(TOMO_ERR_INVALID_ARG, "max_active must be in 1..64");
     if (params->schedule != TOMO_SCHEDULE_DIRECT && params->schedule != TOMO_SCHEDULE_RECURRENT)
         return fail(TOMO_ERR_INVALID_ARG, "unknown schedule");
+    if (params->sub_rounds < 0 || params->sub_rounds > kMaxSubRounds)
+        return fail(TOMO_ERR_INVALID_ARG, "sub_rounds must be in 0..8");
     TOMO_HIP(hipSetDevice(ctx->device));
     std::unique_ptr<tomo_server> srv(new tomo_server());
     srv->ctx = ctx;
     srv->params = *params;
+    srv->sub_rounds = params->sub_rounds ? params->sub_rounds : kMaxSubRounds;
     srv->t0 = std::chrono::steady_clock::now();
     size_t free_b = 0, total_b = 0;
     TOMO_HIP(hipMemGetInfo(&free_b, &total_b));
@@ -653,15 +689,16 @@ extern "C" int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params,
         (void)hipMemPoolDestroy(srv->pool);
         return hip_fail(e, "server pool / stream");
     }
-    e = hipMalloc((void**)&srv->d_tables, sizeof(BatchTable) * OP_COUNT);
-    if (e == hipSuccess) e = hipHostMalloc((void**)&srv->h_tables, sizeof(BatchTable) * OP_COUNT, hipHostMallocDefault);
+    const size_t tbytes = sizeof(BatchTable) * OP_COUNT * kMaxSubRounds;
+    e = hipMalloc((void**)&srv->d_tables, tbytes);
+    if (e == hipSuccess) e = hipHostMalloc((void**)&srv->h_tables, tbytes, hipHostMallocDefault);
     if (e != hipSuccess) {
         if (srv->d_tables) (void)hipFree(srv->d_tables);
         (void)hipStreamDestroy(srv->stream);
         (void)hipMemPoolDestroy(srv->pool);
         return hip_fail(e, "server tables");
     }
-    std::memset(srv->h_tables, 0, sizeof(BatchTable) * OP_COUNT);
+    std::memset(srv->h_tables, 0, tbytes);
     tomo_server* p = srv.release();
     p->worker = std::thread(worker_main, p);
     *out = p;
@@ -805,7 +842,8 @@ extern "C" int tomo_serve_destroy(tomo_server* srv) {
     }
     if (srv->worker.joinable()) srv->worker.join();
     (void)hipSetDevice(srv->ctx->device);
-    if (srv->graph) (void)hipGraphExecDestroy(srv->graph);
+    for (auto& gx : srv->graph)
+        if (gx) (void)hipGraphExecDestroy(gx);
     if (srv->d_tables) (void)hipFree(srv->d_tables);
     if (srv->h_tables) (void)hipHostFree(srv->h_tables);
     if (srv->stream) {

@@ -84,7 +84,7 @@ class GdIterStat(C.Structure):
 
 class ServeParams(C.Structure):
     _fields_ = [("hbm_budget_bytes", C.c_uint64), ("max_active", C.c_int32), ("use_graph", C.c_int32),
-                ("schedule", C.c_int32), ("reserved", C.c_int32)]
+                ("schedule", C.c_int32), ("sub_rounds", C.c_int32)]
 
 
 class SirtParams(C.Structure):
@@ -415,10 +415,10 @@ class Server:
     reconstructions of uneven sizes on one device."""
 
     def __init__(self, ctx: Context, budget_bytes: int = 0, max_active: int = 16, use_graph: bool = True,
-                 schedule: int = SCHEDULE_DIRECT):
+                 schedule: int = SCHEDULE_DIRECT, sub_rounds: int = 0):
         self.ctx, self.lib = ctx, ctx.lib
         p = ServeParams(hbm_budget_bytes=budget_bytes, max_active=max_active, use_graph=int(use_graph),
-                        schedule=schedule)
+                        schedule=schedule, sub_rounds=sub_rounds)
         h = C.c_void_p()
         check(self.lib.tomo_serve_create(ctx.h, C.byref(p), C.byref(h)))
         self.h = h
