@@ -9,7 +9,11 @@ volume followed by a VI backprojection of the sinogram, device-resident
 512x512 detector, FOD 1000 mm, FDD 2000 mm, 1 mm voxels, 2 mm pixels).
 GUPS = voxel-view updates / s = 2 N^3 Nphi / t_step.  The RS projector is
 timed alongside.  Kernel times come from HIP events recorded on the stream
-the kernels run on.  One JSON line is printed.
+the kernels run on.  One JSON line is printed, with these further legs:
+  mixed    FP16 / BF16 storage forms against FP32 (speed, error vs FP64)
+  serving  mixed-size reconstruction stream, served vs sequential, p50/p99
+  batched  uneven SIRT problems in shared launches vs one at a time
+  cpu_baseline  the FP32 oracle on the host cores (views 0..3)
 """
 from __future__ import annotations
 
@@ -39,6 +43,8 @@ def parse():
                     help="requests in the serving sample (0 skips the serving leg)")
     ap.add_argument("--serving-rates", default="2,10,50", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serving-schedule", choices=["direct", "recurrent"], default="recurrent")
+    ap.add_argument("--mixed-oracle-views", type=int, default=8,
+                    help="views of the FP64 oracle check of the mixed forms (0 skips the mixed leg)")
     ap.add_argument("--batch-problems", type=int, default=32,
                     help="uneven SIRT problems in the batched leg (0 skips it)")
     ap.add_argument("--calibration-copy", action="store_true",
@@ -188,6 +194,76 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
     return out
 
 
+def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
+    """Mixed-precision forms on the bench workload: volumes and sinograms
+    stored in FP16 (scaled so max|x| = 2^14) or BF16, FP32 accumulation,
+    FP64 geometry.  Speed of each operator against its FP32 time, and the
+    error of each 16-bit result against the FP64 oracle on views
+    0..oracle_views-1 (a CPU-bounded subset of the same acquisition)."""
+    import numpy as np
+    import torch
+    from tomogpi_amd import lib
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    from oracle_ffi import Oracle, scaled_geometry
+    n, nphi = G.info["nx"], G.info["nphi"]
+    out = {}
+    vmax = float(vol.abs().max())
+    # FP64 references on the view subset
+    gsub = scaled_geometry(n, oracle_views)
+    gsub.num_stop, gsub.den_stop = 2.0 * oracle_views, float(nphi)
+    o = Oracle(gsub)
+    vol_host = vol.cpu().numpy()
+    t0 = time.perf_counter()
+    ref_sid = o.siddon(vol_host.astype(np.float64), f64=True)
+    sino_sub = o.siddon(vol_host)  # FP32 sinogram subset, the backprojection input
+    ref_vi = o.vi(sino_sub.astype(np.float64), f64=True)
+    print(f"[mixed] FP64 oracle on {oracle_views} views: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+          flush=True)
+    dsub = lib.AcquisitionDesc.scaled(n, oracle_views)
+    dsub.num_stop_angle, dsub.den_stop_angle = 2.0 * oracle_views, float(nphi)
+    Gs = lib.Geometry(ctx, dsub)
+
+    def rel(a, b):
+        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+    for name, tdt, code in (("f16", torch.float16, lib.F16), ("bf16", torch.bfloat16, lib.BF16)):
+        sv = 2.0 ** 14 / vmax if name == "f16" else 1.0
+        v16 = (vol * sv).to(tdt)
+        s16 = torch.empty(G.sino_shape, dtype=tdt, device="cuda")
+        b16 = torch.empty(G.vol_shape, dtype=tdt, device="cuda")
+        ss = 2.0 ** 14 / (vmax * 2.0 * n) if name == "f16" else 1.0     # ray sums <= max * chord
+        sb = 2.0 ** 14 / (vmax * 2.0 * n * nphi * 2.0) if name == "f16" else 1.0
+        ms = {}
+        for op, fn in (("siddon", lambda: G.project(v16, s16, lib.PROJ_SIDDON, vol_scale=sv, sino_scale=ss,
+                                                     stream=stream)),
+                       ("vi", lambda: G.backproject(s16, b16, sino_scale=ss, vol_scale=sb, stream=stream)),
+                       ("rs", lambda: G.project(v16, s16, lib.PROJ_RS, vol_scale=sv, sino_scale=ss, stream=stream))):
+            fn()
+            ms[op], _ = timed(fn, 3, stream)
+        G.project(v16, s16, lib.PROJ_SIDDON, vol_scale=sv, sino_scale=ss, stream=stream)  # leave Siddon output
+        torch.cuda.synchronize()
+        # errors on the view subset
+        e_s = torch.empty(Gs.sino_shape, dtype=tdt, device="cuda")
+        Gs.project(v16, e_s, lib.PROJ_SIDDON, vol_scale=sv, sino_scale=ss)
+        sin16 = (torch.from_numpy(sino_sub).cuda() * ss).to(tdt)
+        e_b = torch.empty(Gs.vol_shape, dtype=tdt, device="cuda")
+        Gs.backproject(sin16, e_b, sino_scale=ss, vol_scale=sb)
+        torch.cuda.synchronize()
+        err_s = rel((e_s.float() / ss).cpu().numpy().astype(np.float64), ref_sid)
+        err_b = rel((e_b.float() / sb).cpu().numpy().astype(np.float64), ref_vi)
+        out[name] = {
+            "siddon_ms": round(ms["siddon"], 3), "vi_ms": round(ms["vi"], 3), "rs_ms": round(ms["rs"], 3),
+            "siddon_speedup": round(fp32_ms["siddon"] / ms["siddon"], 3),
+            "vi_speedup": round(fp32_ms["vi"] / ms["vi"], 3), "rs_speedup": round(fp32_ms["rs"] / ms["rs"], 3),
+            "pair_speedup": round((fp32_ms["siddon"] + fp32_ms["vi"]) / (ms["siddon"] + ms["vi"]), 3),
+            "siddon_rel_l2_vs_fp64": float(f"{err_s:.3e}"), "vi_rel_l2_vs_fp64": float(f"{err_b:.3e}"),
+            "error_sample": f"views 0..{oracle_views - 1}, 16-bit inputs and outputs"}
+        print(f"[mixed] {name}: {out[name]}", file=sys.stderr, flush=True)
+        del v16, s16, b16, e_s, e_b, sin16
+    Gs.close()
+    return out
+
+
 def batch_leg(ctx, n_prob, iters=20, seed=7):
     """Batched uneven SIRT reconstructions: problems with edge n uniform in
     {128,192,256,320,384}, views uniform in [90,360], n x n detector,
@@ -327,6 +403,11 @@ def main():
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
                 "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": measured.get(dom_kernel)}
 
+    mixed = None
+    if a.mixed_oracle_views > 0:
+        mixed = mixed_leg(ctx, G, desc, vol, {"siddon": sid_ms, "vi": vi_ms, "rs": rs_ms}, stream,
+                          a.mixed_oracle_views)
+
     serving = None
     del vol, sino, back
     torch.cuda.empty_cache()
@@ -363,6 +444,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": base,
         "operators": ops,
+        "mixed": mixed,
         "serving": serving,
         "batched": batched,
     }

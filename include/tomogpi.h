@@ -348,7 +348,9 @@ typedef struct tomo_serve_stats {
 } tomo_serve_stats;
 
 int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params, tomo_server** out);
-/* Queues a request; the pointers must stay valid until it is done.          */
+/* Queues a request; the pointers must stay valid until it is done.  The
+ * server remembers the latest 65536 requests for poll/wait/iter_stats; older
+ * finished ones are forgotten (TOMO_ERR_INVALID_ARG "unknown request id").  */
 int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req, uint64_t* id);
 int tomo_serve_poll(tomo_server* srv, uint64_t id, tomo_serve_result* res);
 /* Waits until the request is done or failed (TOMO_ERR_BUSY after timeout_s

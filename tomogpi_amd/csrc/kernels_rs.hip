@@ -211,8 +211,11 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __
     const int kend = wave_max(ok ? k0 + r.lr : 0);
     const int nx = g.nx, ny = g.ny, nz = g.nz;
     float acc;
-    if (mode >= 0 && mode < 2) acc = rs_samples<0>(r, qx, nx, ny, nz, vol_inv, k0, kend);
-    else acc = rs_samples<1>(r, qy, nx, ny, nz, vol_inv, k0, kend);
+    // 16-bit volumes are read unscaled; the sum is linear in them, so the
+    // storage scale applies once to it (FP32 storage has no scale).
+    if (mode >= 0 && mode < 2) acc = rs_samples<0>(r, qx, nx, ny, nz, 1.0f, k0, kend);
+    else acc = rs_samples<1>(r, qy, nx, ny, nz, 1.0f, k0, kend);
+    if (sizeof(TV) == 2) acc *= vol_inv;
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
         Storage<TS>::store(sino, o, r.lr > 0 ? acc * g.delta : 0.0f, sino_scale);

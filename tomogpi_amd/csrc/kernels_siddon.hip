@@ -361,7 +361,7 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
         // Per-lane traversal on the y-slab copy, where voxel (x, y, z) is
         // element 2(x+1) + 2(nx+2)(z+1) + 2(nx+2)(nz+1) y (z-pair index of
         // corner (x, z) on plane y, first of the pair).
-        const auto rd = make_reader<TV, BUF>(qy, qy_bytes, vol_inv);
+        const auto rd = make_reader<TV, BUF>(qy, qy_bytes, 1.0f);
         SidState s;
 #pragma unroll
         for (int a = 0; a < 3; ++a) {
@@ -384,14 +384,17 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
         while (s.live && guard-- > 0) sid_step(s, r, rd);
         acc = s.acc;
     } else if (mode < 2) {
-        const auto rq = make_quad_reader<TV, BUF>(qx, qx_bytes, vol_inv);
+        const auto rq = make_quad_reader<TV, BUF>(qx, qx_bytes, 1.0f);
         if (zneg_all && !zpos_all) sid_walk_quads<0, true>(r, rq, nx, ny, nz, ok, acc);
         else sid_walk_quads<0, false>(r, rq, nx, ny, nz, ok, acc);
     } else {
-        const auto rq = make_quad_reader<TV, BUF>(qy, qy_bytes, vol_inv);
+        const auto rq = make_quad_reader<TV, BUF>(qy, qy_bytes, 1.0f);
         if (zneg_all && !zpos_all) sid_walk_quads<1, true>(r, rq, ny, nx, nz, ok, acc);
         else sid_walk_quads<1, false>(r, rq, ny, nx, nz, ok, acc);
     }
+    // 16-bit volumes are read unscaled; the sum is linear in them, so the
+    // storage scale applies once here (FP32 storage has no scale).
+    if (sizeof(TV) == 2) acc *= vol_inv;
     if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), acc, sino_scale);
 }
 

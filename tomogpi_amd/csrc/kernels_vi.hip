@@ -12,6 +12,8 @@
 // thread owns a column of ZB voxels that share u and A; the per-(column,view)
 // terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
 // and the per-voxel v, bilinear weights and sum in FP32.
+#include <type_traits>
+
 #include "device_common.hpp"
 #include "batch.hpp"
 
@@ -123,18 +125,44 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
     constexpr uint32_t es = sizeof(TS);
     const uint64_t view_bytes = (uint64_t)qs.view * es;
     const bool fast = view_bytes < (1ull << 24) && view_bytes * (uint64_t)g.nphi < (1ull << 32);
-    // one voxel-view term from its taps
-    auto term = [&](const ColumnView& cvw, float fu, int k, float v, float vf, const float4& t) {
-        const float fv = v - vf;
-        const float a = fmaf(fu, t.z - t.x, t.x);
-        const float b = fmaf(fu, t.w - t.y, t.y);
-        const float r = fmaf(fv, b - a, a);
+    constexpr bool kF16 = std::is_same<TS, __half>::value;
+    constexpr bool kBF16 = std::is_same<TS, __hip_bfloat16>::value;
+    // one voxel-view term from its interpolated value r
+    auto add = [&](const ColumnView& cvw, int k, float r) {
         if (FDK) {
             const float vp = (zp0 + (float)k) * cvw.A;
             const float wt = cvw.num / sqrtf(fmaf(vp, vp, cvw.base));
             acc[k] += wt * r;
         } else {
             acc[k] += r;
+        }
+    };
+    // FP32 taps (FP32 storage: the oracle's arithmetic)
+    auto term = [&](const ColumnView& cvw, float fu, int k, float v, float vf, const float4& t) {
+        const float fv = v - vf;
+        const float a = fmaf(fu, t.z - t.x, t.x);
+        const float b = fmaf(fu, t.w - t.y, t.y);
+        add(cvw, k, fmaf(fv, b - a, a));
+    };
+    // 16-bit storage, raw taps (g(u,v), g(u,v+1)) in w0 and (g(u+1,v),
+    // g(u+1,v+1)) in w1; the storage scale is applied once at the store.
+    // FP16: both rows interpolated along u in one packed FP16 fma, the v
+    // step in FP32 (error well inside the mixed-form bound; accumulation
+    // stays FP32).  BF16: FP32 interpolation of the widened taps.
+    auto term16 = [&](const ColumnView& cvw, __half2 fu2, float fu, int k, float v, float vf, uint32_t w0,
+                      uint32_t w1) {
+        const float fv = v - vf;
+        if constexpr (kF16) {
+            const __half2 p0 = __builtin_bit_cast(__half2, w0), p1 = __builtin_bit_cast(__half2, w1);
+            const __half2 ab = __hfma2(fu2, __hsub2(p1, p0), p0);
+            const __half a = __low2half(ab);
+            add(cvw, k, fmaf(fv, __half2float(__hsub(__high2half(ab), a)), __half2float(a)));
+        } else {
+            const float t00 = __uint_as_float(w0 << 16), t10 = __uint_as_float(w0 & 0xffff0000u);
+            const float t01 = __uint_as_float(w1 << 16), t11 = __uint_as_float(w1 & 0xffff0000u);
+            const float a = fmaf(fu, t01 - t00, t00);
+            const float b = fmaf(fu, t11 - t10, t10);
+            add(cvw, k, fmaf(fv, b - a, a));
         }
     };
     if (fast) {
@@ -148,13 +176,19 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
             // byte offset of cell (uc, vc): ((vc + 2) row + 2 (uc + 2)) es
             const float base = fmaf(uc + 2.0f, (float)(2u * es), 2.0f * row_b);
             const uint32_t soff = (uint32_t)p * (uint32_t)view_bytes;
+            const __half2 fu2 = __float2half2_rn(fu);
 #pragma unroll
             for (int k = 0; k < ZB; ++k) {
                 const float v = fmaf(zp0 + (float)k, cvw.A, cv);
                 const float vf = floorf(v);
                 const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
                 const uint32_t off = (uint32_t)fmaf(vc, row_b, base);
-                term(cvw, fu, k, v, vf, BufLoad4<TS>::load(rsrc, off, soff, sino_inv));
+                if constexpr (kF16 || kBF16) {
+                    const auto w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, (int)soff, 0);
+                    term16(cvw, fu2, fu, k, v, vf, w[0], w[1]);
+                } else {
+                    term(cvw, fu, k, v, vf, BufLoad4<TS>::load(rsrc, off, soff, sino_inv));
+                }
             }
         }
     } else {
@@ -164,22 +198,32 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
             const float fu = cvw.u - uf;
             const int uc = (int)__builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
             const TS* gp = quads + (size_t)p * qs.view + 2u * (uint32_t)(uc + 2);
+            const __half2 fu2 = __float2half2_rn(fu);
 #pragma unroll
             for (int k = 0; k < ZB; ++k) {
                 const float v = fmaf(zp0 + (float)k, cvw.A, cv);
                 const float vf = floorf(v);
                 const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                term(cvw, fu, k, v, vf, Storage<TS>::load4(gp, (uint64_t)(uint32_t)(vc + 2) * qs.row, sino_inv));
+                const uint64_t e = (uint64_t)(uint32_t)(vc + 2) * qs.row;
+                if constexpr (kF16 || kBF16) {
+                    uint32_t w[2];
+                    __builtin_memcpy(w, gp + e, sizeof(w));
+                    term16(cvw, fu2, fu, k, v, vf, w[0], w[1]);
+                } else {
+                    term(cvw, fu, k, v, vf, Storage<TS>::load4(gp, e, sino_inv));
+                }
             }
         }
     }
     const size_t sxy = (size_t)g.nx * g.ny;
     const size_t base = (size_t)i + (size_t)g.nx * j;
+    const float in_scale = (kF16 || kBF16) ? sino_inv : 1.0f;  // 16-bit taps are read unscaled
 #pragma unroll
-    for (int k = 0; k < ZB; ++k)
+    for (int k = 0; k < ZB; ++k) {
+        const float a = (kF16 || kBF16) ? acc[k] * in_scale : acc[k];
         if (k0 + k < g.nz)
-            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? acc[k] : (acc[k] * g.delta) * post,
-                               vol_scale);
+            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? a : (a * g.delta) * post, vol_scale);
+    }
 }
 
 template <typename TS>

@@ -39,6 +39,7 @@ namespace {
 
 constexpr size_t kAlign = 256;
 constexpr int kMaxSubRounds = 8;
+constexpr size_t kMaxRecords = 65536;  // finished requests remembered for poll/wait
 
 uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
 
@@ -734,6 +735,12 @@ extern "C" int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req
     ++srv->submitted;
     srv->queue.push_back(q);
     srv->all[q->id] = q;
+    // Bounded history: past kMaxRecords, the oldest finished records go
+    // (ids are increasing, so the map's order is submission order).
+    for (auto it = srv->all.begin(); srv->all.size() > kMaxRecords && it != srv->all.end();) {
+        if (it->second->state >= TOMO_REQ_DONE) it = srv->all.erase(it);
+        else ++it;
+    }
     *id = q->id;
     srv->cv_work.notify_all();
     return TOMO_OK;
