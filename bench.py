@@ -48,8 +48,8 @@ def parse():
     ap.add_argument("--batch-problems", type=int, default=32,
                     help="uneven SIRT problems in the batched leg (0 skips it)")
     ap.add_argument("--calibration-copy", action="store_true",
-                    help="run one plain device copy of the volume (known bytes) before the timed steps, "
-                         "as the reference kernel for FETCH_SIZE / WRITE_SIZE counter scaling")
+                    help="run one element-wise copy of the volume (known bytes) before the timed steps, "
+                         "as a reference kernel for FETCH_SIZE / WRITE_SIZE counter scaling")
     return ap.parse_args()
 
 
@@ -353,7 +353,7 @@ def main():
         G.project(vol, sino, lib.PROJ_RS, stream=stream)
 
     if a.calibration_copy:
-        copy = vol.clone()
+        copy = vol + 0.0  # an element-wise kernel: reads and writes the volume once
         torch.cuda.synchronize()
         del copy
     for _ in range(a.warmup):

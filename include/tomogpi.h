@@ -347,6 +347,8 @@ typedef struct tomo_serve_stats {
     double   throughput_rps;    /* completed / (last_done - first_submit)   */
 } tomo_serve_stats;
 
+/* Creation captures the server's launch sequences as graphs on its stream;
+ * other threads must not synchronise the whole device during this call.     */
 int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params, tomo_server** out);
 /* Queues a request; the pointers must stay valid until it is done.  The
  * server remembers the latest 65536 requests for poll/wait/iter_stats; older

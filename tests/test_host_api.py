@@ -83,3 +83,24 @@ def test_status_strings():
     L = lib.load()
     assert L.tomo_status_string(0) == b"ok"
     assert L.tomo_status_string(6) == b"configuration error"
+
+
+def test_null_arguments_are_errors_without_a_device():
+    """Argument checks come before any device call: null handles and bad
+    enums return TOMO_ERR_INVALID_ARG (1) on a machine without a GPU."""
+    L = lib.load()
+    st = lib.GdIterStat()
+    assert L.tomo_gd_run(None, None, None, None, None, C.byref(st), None) == 1
+    assert L.tomo_cg_run(None, None, None, None, None, C.byref(st), None) == 1
+    assert L.tomo_sirt_run(None, None, None, None, None, None, None) == 1
+    srv = C.c_void_p()
+    p = lib.ServeParams(max_active=4)
+    assert L.tomo_serve_create(None, C.byref(p), C.byref(srv)) == 1
+    rid = C.c_uint64()
+    req = lib.ServeRequest()
+    assert L.tomo_serve_submit(None, C.byref(req), C.byref(rid)) == 1
+    res = lib.ServeResult()
+    assert L.tomo_serve_poll(None, 1, C.byref(res)) == 1
+    assert L.tomo_serve_wait(None, 1, 0.0, C.byref(res)) == 1
+    assert L.tomo_serve_destroy(None) == 0
+    assert b"null" in L.tomo_last_error()

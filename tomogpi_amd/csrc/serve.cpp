@@ -507,20 +507,6 @@ int run_round(tomo_server* srv, const std::vector<std::shared_ptr<Request>>& run
                             hipMemcpyHostToDevice, s));
     for (int j = 0; j < sets; ++j) {
         if (srv->params.use_graph) {
-            if (!srv->graph[j]) {
-                TOMO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
-                const int rc = launch_round(srv, j);
-                hipGraph_t captured = nullptr;
-                const hipError_t ee = hipStreamEndCapture(s, &captured);
-                if (rc) {
-                    if (captured) (void)hipGraphDestroy(captured);
-                    return rc;
-                }
-                TOMO_HIP(ee);
-                const hipError_t ie = hipGraphInstantiate(&srv->graph[j], captured, nullptr, nullptr, 0);
-                (void)hipGraphDestroy(captured);
-                TOMO_HIP(ie);
-            }
             TOMO_HIP(hipGraphLaunch(srv->graph[j], s));
         } else {
             const int rc = launch_round(srv, j);
@@ -528,6 +514,29 @@ int run_round(tomo_server* srv, const std::vector<std::shared_ptr<Request>>& run
         }
     }
     TOMO_HIP(hipStreamSynchronize(s));
+    return TOMO_OK;
+}
+
+// Captures the launch sequence of every table set as a graph.  Done once, by
+// the creating thread before the worker starts: a capture in progress makes
+// device-wide synchronisation by other threads fail, so none happens while
+// requests are served.
+int capture_graphs(tomo_server* srv) {
+    hipStream_t s = srv->stream;
+    for (int j = 0; j < kMaxSubRounds; ++j) {
+        TOMO_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        const int rc = launch_round(srv, j);
+        hipGraph_t captured = nullptr;
+        const hipError_t ee = hipStreamEndCapture(s, &captured);
+        if (rc) {
+            if (captured) (void)hipGraphDestroy(captured);
+            return rc;
+        }
+        TOMO_HIP(ee);
+        const hipError_t ie = hipGraphInstantiate(&srv->graph[j], captured, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(captured);
+        TOMO_HIP(ie);
+    }
     return TOMO_OK;
 }
 
@@ -700,6 +709,20 @@ extern "C" int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params,
         return hip_fail(e, "server tables");
     }
     std::memset(srv->h_tables, 0, tbytes);
+    // Empty tables on the device, then the captured rounds.
+    e = hipMemcpyAsync(srv->d_tables, srv->h_tables, tbytes, hipMemcpyHostToDevice, srv->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(srv->stream);
+    int rc = e == hipSuccess ? TOMO_OK : hip_fail(e, "server tables");
+    if (rc == TOMO_OK && params->use_graph) rc = capture_graphs(srv.get());
+    if (rc != TOMO_OK) {
+        for (auto& gx : srv->graph)
+            if (gx) (void)hipGraphExecDestroy(gx);
+        (void)hipFree(srv->d_tables);
+        (void)hipHostFree(srv->h_tables);
+        (void)hipStreamDestroy(srv->stream);
+        (void)hipMemPoolDestroy(srv->pool);
+        return rc;
+    }
     tomo_server* p = srv.release();
     p->worker = std::thread(worker_main, p);
     *out = p;
