@@ -2,13 +2,21 @@
 //
 // A shared launch runs one operator for up to kBatchMax independent problems
 // of different sizes.  Its grid is fixed (kBatchGrid workgroups) and
-// persistent: workgroup w takes tiles w, w + kBatchGrid, ... of the launch.
-// The tiles of problem q are [item[q].begin, item[q].begin + item[q].tiles);
-// a tile is one workgroup of the single-problem kernel's grid, numbered x
-// fastest, then y, then z, and runs the same code as there.  A problem's
-// results therefore do not depend on what else shares the launch.  Because
-// the grid and the table address are fixed, a captured hipGraph of shared
-// launches stays valid while the table contents change between replays.
+// persistent.  The tiles of problem q are [item[q].begin, item[q].begin +
+// item[q].tiles); a tile is one workgroup of the single-problem kernel's
+// grid, numbered x fastest, then y, then z, and runs the same code as there.
+// A problem's results therefore do not depend on what else shares the launch
+// or on which workgroup runs a tile.  Workgroup w starts with tile w.  The
+// element-wise operators then take tiles w + kBatchGrid, w + 2 kBatchGrid, ...
+// (equal-cost tiles); the projectors and the backprojector, whose tiles differ
+// in cost with the problem's views and the rays' lengths, claim their next
+// tile from the table's counter instead (one vector atomic per tile, on a
+// cache line of its own, broadcast through LDS), which balances them over the
+// chip as hardware dispatch would.  Because the grid and the table address
+// are fixed, a captured hipGraph of shared launches stays valid while the
+// table contents change between replays; the host's copy of the table before
+// each launch sequence resets the counter, so a table serves one launch per
+// copy.
 #pragma once
 
 #include "tomo_internal.hpp"
@@ -40,6 +48,9 @@ struct BatchTable {
     uint32_t total;
     uint32_t pad[2];
     BatchItem item[kBatchMax];
+    // Tiles claimed beyond the first kBatchGrid (0 when the host writes the
+    // table); its own cache line keeps the atomics off the items.
+    alignas(128) uint32_t next;
 };
 
 // Per-problem scalars of the gradient loop, FP64, in `slot`.
@@ -55,6 +66,22 @@ enum GdSlot {
 };
 
 #ifdef __HIP__
+// Static order: tiles blockIdx.x, blockIdx.x + gridDim.x, ... (`tile` in the body).
+#define TOMO_BATCH_STRIDED(t) for (uint32_t tile = blockIdx.x; tile < (t)->total; tile += gridDim.x)
+
+// The workgroup's next claimed tile (every thread of the workgroup calls it).
+__device__ __forceinline__ uint32_t batch_claim(uint32_t* next, uint32_t* slot) {
+    __syncthreads();  // the previous tile's readers of *slot are done
+    if (threadIdx.x == 0 && threadIdx.y == 0) *slot = gridDim.x + atomicAdd(next, 1u);
+    __syncthreads();
+    return *slot;
+}
+
+// Dynamic order: tile blockIdx.x, then tiles claimed from `next`.
+#define TOMO_BATCH_CLAIMED(t, next)                                                         \
+    __shared__ uint32_t tomo_batch_slot;                                                    \
+    for (uint32_t tile = blockIdx.x; tile < (t)->total; tile = ::tomo::batch_claim((next), &tomo_batch_slot))
+
 // Tile -> problem: the last item whose begin <= tile (block-uniform).
 __device__ __forceinline__ int batch_find(const BatchTable* t, uint32_t tile) {
     int lo = 0, hi = t->count - 1;
@@ -69,18 +96,18 @@ __device__ __forceinline__ int batch_find(const BatchTable* t, uint32_t tile) {
 
 // Shared launches (stream-ordered; `t` is a device copy of the table).
 // Storage is FP32 throughout.
-hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s);      // in volume -> out qx
-hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s);      // in volume -> out qy
-hipError_t launch_batch_siddon(const BatchTable* t, hipStream_t s);       // in qx, in2 qy -> out sinogram
-hipError_t launch_batch_rs(const BatchTable* t, hipStream_t s);           // in qx, in2 qy -> out; flags bit0 cube
-hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s);     // in sinogram -> out tap quads
-hipError_t launch_batch_vi(const BatchTable* t, hipStream_t s);           // in tap quads -> out volume * a
-hipError_t launch_batch_diff_sq(const BatchTable* t, hipStream_t s);      // out = in - in2; aux partials of out^2
-hipError_t launch_batch_sq(const BatchTable* t, hipStream_t s);           // aux partials of in^2
-hipError_t launch_batch_laplacian(const BatchTable* t, hipStream_t s);    // out = L in; aux partials of out^2 if set
-hipError_t launch_batch_axpy(const BatchTable* t, hipStream_t s);         // out += alpha * in (see kernels_vec.hip)
-hipError_t launch_batch_fill(const BatchTable* t, hipStream_t s);         // out = a
-hipError_t launch_batch_finish(const BatchTable* t, hipStream_t s);       // slot[flags] = sum of aux[0..n)
-hipError_t launch_batch_step(const BatchTable* t, hipStream_t s);         // step rule on slot
+hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s);   // in volume -> out qx
+hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s);   // in volume -> out qy
+hipError_t launch_batch_siddon(BatchTable* t, hipStream_t s);          // in qx, in2 qy -> out sinogram
+hipError_t launch_batch_rs(BatchTable* t, hipStream_t s);              // in qx, in2 qy -> out; flags bit0 cube
+hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s);  // in sinogram -> out tap quads
+hipError_t launch_batch_vi(BatchTable* t, hipStream_t s);              // in tap quads -> out volume * a
+hipError_t launch_batch_diff_sq(const BatchTable* t, hipStream_t s);   // out = in - in2; aux partials of out^2
+hipError_t launch_batch_sq(const BatchTable* t, hipStream_t s);        // aux partials of in^2
+hipError_t launch_batch_laplacian(const BatchTable* t, hipStream_t s); // out = L in; aux partials of out^2 if set
+hipError_t launch_batch_axpy(const BatchTable* t, hipStream_t s);      // out += alpha * in (see kernels_vec.hip)
+hipError_t launch_batch_fill(const BatchTable* t, hipStream_t s);      // out = a
+hipError_t launch_batch_finish(const BatchTable* t, hipStream_t s);    // slot[flags] = sum of aux[0..n)
+hipError_t launch_batch_step(const BatchTable* t, hipStream_t s);      // step rule on slot
 
 }  // namespace tomo

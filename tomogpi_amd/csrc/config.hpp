@@ -3,7 +3,7 @@
 
 #include <string>
 
-#include "../../include/tomogpi.h"
+#include "tomogpi.h"
 
 namespace tomo {
 

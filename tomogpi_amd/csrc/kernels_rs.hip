@@ -230,9 +230,8 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const T
 }
 
 // Shared launch (FP32 storage); item flags bit 0 selects the cube FOV.
-__global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __restrict__ t) {
-    const uint32_t total = t->total;
-    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+__global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+    TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
@@ -309,8 +308,8 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
     return TOMO_OK;
 }
 
-hipError_t launch_batch_rs(const BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(rs_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t);
+hipError_t launch_batch_rs(BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(rs_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t, &t->next);
     return hipGetLastError();
 }
 

@@ -409,14 +409,20 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
 
 // Shared launch: tiles are the single-problem grid's workgroups (column
 // group fastest, then view, then row group); FP32 storage.
-__global__ __launch_bounds__(BX* BY) void siddon_batch_kernel(const BatchTable* __restrict__ t) {
-    const uint32_t total = t->total;
-    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+__global__ __launch_bounds__(BX* BY) void siddon_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+    TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        siddon_tile<float, float, false>(it.g, (const float*)it.in, (const float*)it.in2, 0u, 0u, 1.0f,
-                                         (float*)it.out, 1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
+        const uint64_t qxb = zpair_elems_x(it.g.nx, it.g.ny, it.g.nz) * 4ull;
+        const uint64_t qyb = zpair_elems_y(it.g.nx, it.g.ny, it.g.nz) * 4ull;
+        if (qxb < (1ull << 32) && qyb < (1ull << 32))
+            siddon_tile<float, float, true>(it.g, (const float*)it.in, (const float*)it.in2, (unsigned)qxb,
+                                            (unsigned)qyb, 1.0f, (float*)it.out, 1.0f, (int)bx, (int)(rest % it.gy),
+                                            (int)(rest / it.gy));
+        else
+            siddon_tile<float, float, false>(it.g, (const float*)it.in, (const float*)it.in2, 0u, 0u, 1.0f,
+                                             (float*)it.out, 1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
     }
 }
 
@@ -572,8 +578,8 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     return TOMO_OK;
 }
 
-hipError_t launch_batch_siddon(const BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(siddon_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t);
+hipError_t launch_batch_siddon(BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(siddon_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t, &t->next);
     return hipGetLastError();
 }
 

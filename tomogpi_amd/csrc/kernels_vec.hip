@@ -246,9 +246,6 @@ __device__ __forceinline__ double block_sum_loop(double v) {
     return r;
 }
 
-#define TOMO_BATCH_LOOP(t)                                                    \
-    for (uint32_t tile = blockIdx.x; tile < (t)->total; tile += gridDim.x)
-
 __device__ __forceinline__ void batch_coords(const BatchItem& it, uint32_t tile, int& bx, int& by, int& bz) {
     const uint32_t local = tile - it.begin;
     bx = (int)(local % it.gx);
@@ -259,7 +256,7 @@ __device__ __forceinline__ void batch_coords(const BatchItem& it, uint32_t tile,
 
 __global__ __launch_bounds__(256) void zpair_x_batch_kernel(const BatchTable* __restrict__ t) {
     __shared__ uint32_t tile_lds[2][32][33];
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
@@ -270,7 +267,7 @@ __global__ __launch_bounds__(256) void zpair_x_batch_kernel(const BatchTable* __
 }
 
 __global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
@@ -280,7 +277,7 @@ __global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __
 
 // d = in - in2; out = d (or d * in3 when in3 is set); aux[tile] = sum d^2.
 __global__ __launch_bounds__(kBlock) void diff_sq_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const float* a = (const float*)it.in;
@@ -305,7 +302,7 @@ __global__ __launch_bounds__(kBlock) void diff_sq_batch_kernel(const BatchTable*
 
 // aux[tile] = sum in^2.
 __global__ __launch_bounds__(kBlock) void sq_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const float* a = (const float*)it.in;
@@ -327,7 +324,7 @@ __global__ __launch_bounds__(kBlock) void sq_batch_kernel(const BatchTable* __re
 // out = L in over tiles of 64 x 4 voxels of one plane; aux[tile] = sum out^2
 // when aux is set.
 __global__ __launch_bounds__(kBlock) void laplacian_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
@@ -344,7 +341,7 @@ __global__ __launch_bounds__(kBlock) void laplacian_batch_kernel(const BatchTabl
 // flags bit 1 clamps the result at 0; flags bit 2 writes out = a instead;
 // with in3 set, a is the element in3[e] (the SIRT volume weight).
 __global__ __launch_bounds__(kBlock) void axpy_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const float a = (it.flags & 1) ? (float)it.slot[kSlotAlpha] : it.a;
@@ -366,7 +363,7 @@ __global__ __launch_bounds__(kBlock) void axpy_batch_kernel(const BatchTable* __
 }
 
 __global__ __launch_bounds__(kBlock) void fill_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         float* y = (float*)it.out;
@@ -380,7 +377,7 @@ __global__ __launch_bounds__(kBlock) void fill_batch_kernel(const BatchTable* __
 
 // slot[flags] = sum of the n partials in aux (one tile per item).
 __global__ __launch_bounds__(kBlock) void finish_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const double* part = (const double*)it.aux;
         double s = 0.0;
@@ -394,7 +391,7 @@ __global__ __launch_bounds__(kBlock) void finish_batch_kernel(const BatchTable* 
 // per item; d = lambda, flags bit 0 = optimal step this iteration).  aux,
 // when set, receives (residual, norm dJ, norm dJ proj, alpha) at entry n.
 __global__ __launch_bounds__(64) void step_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_LOOP(t) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         if (threadIdx.x != 0) continue;
         double* sl = it.slot;

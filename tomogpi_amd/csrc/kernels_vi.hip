@@ -242,8 +242,7 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
 // ceil((nv+3)/4), nphi).  VI: tile grid (ceil(nx/16), ceil(ny/16),
 // ceil(nz/16)); the stored value is (delta * sum) * a.
 __global__ __launch_bounds__(256) void vi_quads_batch_kernel(const BatchTable* __restrict__ t) {
-    const uint32_t total = t->total;
-    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+    TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
@@ -252,9 +251,10 @@ __global__ __launch_bounds__(256) void vi_quads_batch_kernel(const BatchTable* _
     }
 }
 
-__global__ __launch_bounds__(TX* TY) void vi_batch_kernel(const BatchTable* __restrict__ t) {
-    const uint32_t total = t->total;
-    for (uint32_t tile = blockIdx.x; tile < total; tile += gridDim.x) {
+// Register budget for 7 waves per SIMD (72 VGPRs, no spills); unbounded, the
+// tile loop around both address paths takes 129 VGPRs and 3 waves.
+__global__ __launch_bounds__(TX* TY, 7) void vi_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+    TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
@@ -347,8 +347,8 @@ hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s) {
     return hipGetLastError();
 }
 
-hipError_t launch_batch_vi(const BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(TX * TY), 0, s, t);
+hipError_t launch_batch_vi(BatchTable* t, hipStream_t s) {
+    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(TX * TY), 0, s, t, &t->next);
     return hipGetLastError();
 }
 

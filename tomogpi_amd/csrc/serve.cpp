@@ -235,6 +235,7 @@ struct TableBuilder {
     void reset() {
         t->count = 0;
         t->total = 0;
+        t->next = 0;
     }
     BatchItem& add(const DevGeom& g, uint32_t gx, uint32_t gy, uint32_t gz) {
         BatchItem& it = t->item[t->count++];
@@ -438,7 +439,7 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
 
 // The fixed launch sequence of one iteration (every table, empty or not).
 int launch_round(tomo_server* srv, int set) {
-    const BatchTable* T = srv->d_tables + (size_t)set * OP_COUNT;
+    BatchTable* T = srv->d_tables + (size_t)set * OP_COUNT;
     hipStream_t s = srv->stream;
     hipError_t e = hipSuccess;
 #define TOMO_STEP(call) \

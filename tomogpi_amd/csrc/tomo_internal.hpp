@@ -9,7 +9,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tomogpi.h"
+#include "tomogpi.h"
 
 namespace tomo {
 
@@ -71,8 +71,8 @@ namespace tomo {
 
 // Element counts of the z-pair copies of a volume for x slabs (qx) and y
 // slabs (qy): nA * (nz+1) * (nB+2) pairs.
-inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
-inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
+__host__ __device__ inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
+__host__ __device__ inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
 
 // Elements of the VI tap-quad copy of a sinogram: nphi views of (nv + 3)
 // rows of (nu + 4) pairs; its fill kernel's grid is (ceil((nu+4)/64),

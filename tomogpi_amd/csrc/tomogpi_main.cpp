@@ -34,7 +34,7 @@
 #include <string>
 #include <vector>
 
-#include "../../include/tomogpi.h"
+#include "tomogpi.h"
 #include "config.hpp"
 
 namespace {
