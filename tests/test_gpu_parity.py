@@ -113,6 +113,64 @@ def test_siddon_bit_exact(setup):
     assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
 
 
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+@pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon"])
+def test_16bit_volume_bit_exact(setup, dt, op):
+    """16-bit volumes at storage scale 1 projected into FP32 sinograms: the
+    sums are the FP32 oracle's on the rounded voxel values, bit for bit.  RS
+    reads such volumes through octet copies (one 16-byte cell per sample),
+    Siddon through 16-bit z-pair copies."""
+    g, o, G = setup
+    rng = np.random.default_rng(9)
+    vol = ellipsoids(g.nx, g.ny, g.nz, count=12, seed=4, amin=2.0, amax=12.0)
+    vol = vol + rng.random(g.vol_shape, dtype=np.float32)
+    v16 = dev(vol).to(torch.float16 if dt == "f16" else torch.bfloat16)
+    rounded = v16.float().cpu().numpy()
+    ref = o.siddon(rounded) if op == "siddon" else o.rs(rounded, cube=op == "rs_cube")
+    out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+    G.project(v16, out, lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS,
+              lib.FLAG_RS_FOV_CUBE if op == "rs_cube" else 0)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+# Detector rows of a quarter voxel at magnification 2: the row step A of a
+# column is about 4, so 16-bit VI reads one tap-column load per voxel.
+FINE_ROWS = Geometry(fod=200.0, fdd=400.0, nphi=12, su=40 * 0.5, sv=40 * 0.5, nu=40, nv=40, sx=24.0, sy=24.0,
+                     sz=24.0, nx=24, ny=24, nz=24, num_start=0.0, den_start=1.0, num_stop=2.0, den_stop=1.0)
+
+
+def _vi16_case(g, o, G, dt):
+    """16-bit sinogram at storage scale 1 into an FP32 volume: FP16 against
+    the oracle's FP16-arithmetic VI, BF16 (FP32 arithmetic on the widened
+    taps) against the FP32 oracle, both on the rounded values, bit for bit."""
+    x = o.rs(ellipsoids(g.nx, g.ny, g.nz, count=12, seed=6, amin=2.0, amax=12.0))
+    s16 = dev(x).to(torch.float16 if dt == "f16" else torch.bfloat16)
+    rounded = s16.float().cpu().numpy()
+    ref = o.vi_h16(rounded) if dt == "f16" else o.vi(rounded)
+    out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+    G.backproject(s16, out)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+def test_16bit_vi_bit_exact(setup, dt):
+    """Row steps below 2 (every geometry here): one load per voxel pair."""
+    g, o, G = setup
+    _vi16_case(g, o, G, dt)
+
+
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+def test_16bit_vi_bit_exact_fine_rows(ctx, dt):
+    """Row steps near 4: one load per voxel."""
+    G = lib.Geometry(ctx, to_desc(FINE_ROWS))
+    try:
+        _vi16_case(FINE_ROWS, Oracle(FINE_ROWS), G, dt)
+    finally:
+        G.close()
+
+
 @pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "vi"])
 def test_adjoint_dot(setup, ctx, op):
     """<A x, y> = <x, A^T y> with the GPU operator and its GPU transpose (rel 1e-5)."""

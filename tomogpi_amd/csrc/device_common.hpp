@@ -97,6 +97,21 @@ template <> struct BufLoad4<__hip_bfloat16> {
     }
 };
 
+// The two 16-bit values of a packed word as float, low half first.
+template <typename T> struct Unpack16;
+template <> struct Unpack16<__half> {
+    __device__ __forceinline__ static float lo(uint32_t w) {
+        return __half2float(__builtin_bit_cast(__half, (unsigned short)(w & 0xffffu)));
+    }
+    __device__ __forceinline__ static float hi(uint32_t w) {
+        return __half2float(__builtin_bit_cast(__half, (unsigned short)(w >> 16)));
+    }
+};
+template <> struct Unpack16<__hip_bfloat16> {
+    __device__ __forceinline__ static float lo(uint32_t w) { return __uint_as_float(w << 16); }
+    __device__ __forceinline__ static float hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
+};
+
 // Predicated read: clamped address (always in bounds) then select.
 template <typename T>
 __device__ __forceinline__ float tap(const T* p, int64_t idx, bool ok, float inv) {

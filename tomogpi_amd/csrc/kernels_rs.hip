@@ -133,6 +133,30 @@ __device__ __forceinline__ float trilinear_quads(const TV* __restrict__ q, int n
     return fmaf(fz, c1 - c0, c0);
 }
 
+// The same interpolation from one element of the octet copy of A (16-bit
+// storage, values unscaled): the eight taps, the weights and the order of
+// the operations are those of trilinear_quads, so results are identical.
+// Cells with a corner index n (or beyond) have no voxel inside the grid and
+// give +0, as the zero taps do there.
+template <int A, typename TV>
+__device__ __forceinline__ float trilinear_octet(const uint4* __restrict__ q, int nx, int ny, int nz, float x,
+                                                 float y, float z) {
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    if (!(xf >= -1.0f && xf <= (float)(nx - 1) && yf >= -1.0f && yf <= (float)(ny - 1) && zf >= -1.0f &&
+          zf <= (float)(nz - 1)))
+        return 0.0f;
+    const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
+    const float fx = x - xf, fy = y - yf, fz = z - zf;
+    const uint4 e = A == 0 ? q[octet_index(ix, iz, iy, nz, ny)] : q[octet_index(iy, iz, ix, nz, nx)];
+    using U = Unpack16<TV>;
+    const float t000 = U::lo(e.x), t100 = U::hi(e.x), t010 = U::lo(e.y), t110 = U::hi(e.y);
+    const float t001 = U::lo(e.z), t101 = U::hi(e.z), t011 = U::lo(e.w), t111 = U::hi(e.w);
+    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
+    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
+    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
+    return fmaf(fz, c1 - c0, c0);
+}
+
 __device__ __forceinline__ int wave_min(int v) {
     for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
     return v;
@@ -150,16 +174,20 @@ __device__ __forceinline__ float wave_minf(float v) {
 // Samples of one lane, taken one per iteration k of the wave's loop: sample
 // s = k - k0 for k0 <= k < k0 + lr.  Same positions (fma(s, d, entry)) and
 // the same order of the sum as the per-ray loop.
+// q is the z-pair copy of A for FP32 storage, its octet copy for 16-bit.
 template <int A, typename TV>
-__device__ __forceinline__ float rs_samples(const RsRay& r, const TV* __restrict__ q, int nx, int ny, int nz,
-                                            float inv, int k0, int kend) {
+__device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restrict__ q, int nx, int ny, int nz,
+                                            int k0, int kend) {
     float acc = 0.0f;
     for (int k = 0; k < kend; ++k) {
         const int s = k - k0;
         if (s >= 0 && s < r.lr) {
             const float sf = (float)s;
-            acc += trilinear_quads<A>(q, nx, ny, nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey), fmaf(sf, r.dz, r.ez),
-                                      inv);
+            const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
+            if constexpr (sizeof(TV) == 2)
+                acc += trilinear_octet<A, TV>((const uint4*)q, nx, ny, nz, x, y, z);
+            else
+                acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
         }
     }
     return acc;
@@ -174,12 +202,14 @@ __device__ __forceinline__ float rs_samples(const RsRay& r, const TV* __restrict
 // behind that one in samples (|d_A| per sample): at iteration k every lane
 // samples near the same A plane, one sample per lane per iteration.  Its 64
 // rays (adjacent detector columns) then read neighbouring quads of the
-// z-pair copy of A, two 16-byte loads per sample.  Other waves sample from
-// k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
+// z-pair copy of A, two 16-byte loads per sample (FP32 storage), or the
+// neighbouring cells of its octet copy, one 16-byte load per sample (16-bit
+// storage).  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
 // (every view of a row band before the next band).
 template <typename TV, typename TS>
-__device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __restrict__ qx, const TV* __restrict__ qy,
-                                        float vol_inv, TS* __restrict__ sino, float sino_scale, int bx, int p, int bz) {
+__device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* __restrict__ qx,
+                                        const void* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
+                                        float sino_scale, int bx, int p, int bz) {
     const int m = bx * BX + threadIdx.x;
     const int n = bz * BY + threadIdx.y;
     const bool inside = m < g.nu && n < g.nv;
@@ -213,8 +243,8 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __
     float acc;
     // 16-bit volumes are read unscaled; the sum is linear in them, so the
     // storage scale applies once to it (FP32 storage has no scale).
-    if (mode >= 0 && mode < 2) acc = rs_samples<0>(r, qx, nx, ny, nz, 1.0f, k0, kend);
-    else acc = rs_samples<1>(r, qy, nx, ny, nz, 1.0f, k0, kend);
+    if (mode >= 0 && mode < 2) acc = rs_samples<0, TV>(r, qx, nx, ny, nz, k0, kend);
+    else acc = rs_samples<1, TV>(r, qy, nx, ny, nz, k0, kend);
     if (sizeof(TV) == 2) acc *= vol_inv;
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
@@ -223,8 +253,8 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const TV* __
 }
 
 template <typename TV, typename TS>
-__global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const TV* __restrict__ qx,
-                                                  const TV* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
+__global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const void* __restrict__ qx,
+                                                  const void* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
                                                   float sino_scale) {
     rs_tile<TV, TS>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
 }
@@ -291,18 +321,25 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
     const DevGeom& d = g->dev;
     const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     const float inv = 1.0f / vol.scale;
+    // FP32 volumes are read through z-pair copies, 16-bit ones through octet copies.
     ZPairs zp;
-    hipError_t le = zp.alloc(vol.p, vol.dtype == TOMO_F32 ? 4 : 2, d.nx, d.ny, d.nz, s);
+    Octets oc;
+    hipError_t le = vol.dtype == TOMO_F32 ? zp.alloc(vol.p, 4, d.nx, d.ny, d.nz, s)
+                                          : oc.alloc(vol.p, d.nx, d.ny, d.nz, s);
+    const void* cx = vol.dtype == TOMO_F32 ? zp.qx : oc.ox;
+    const void* cy = vol.dtype == TOMO_F32 ? zp.qy : oc.oy;
     if (le == hipSuccess) {
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-                hipLaunchKernelGGL((rs_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, cube, (const TV*)zp.qx,
-                                   (const TV*)zp.qy, inv, (TS*)sino.p, sino.scale);
+                hipLaunchKernelGGL((rs_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv, (TS*)sino.p,
+                                   sino.scale);
             });
         });
         le = hipGetLastError();
     }
-    const hipError_t fe = zp.release(s);
+    hipError_t fe = zp.release(s);
+    const hipError_t fe2 = oc.release(s);
+    if (fe == hipSuccess) fe = fe2;
     TOMO_HIP(le);
     TOMO_HIP(fe);
     return TOMO_OK;

@@ -12,6 +12,8 @@
 // thread owns a column of ZB voxels that share u and A; the per-(column,view)
 // terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
 // and the per-voxel v, bilinear weights and sum in FP32.
+#include <algorithm>
+#include <cmath>
 #include <type_traits>
 
 #include "device_common.hpp"
@@ -58,22 +60,25 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
     return r;
 }
 
-// Tap-quad copy of the sinogram: element (p, vv, uu), vv in [0, nv+2],
+// Tap-quad copy of an FP32 sinogram: element (p, vv, uu), vv in [0, nv+2],
 // uu in [0, nu+3], holds the detector values (u, v) and (u, v+1) with
 // u = uu-2, v = vv-2 (0 off the detector).  Two neighbouring elements are the
 // four bilinear taps of the cell with lower corner (u, v), so one 16-byte
-// (FP32) or 8-byte (16-bit) load serves one voxel-view update.  Cells with a
-// corner in [-2, nu] x [-2, nv] are in the copy; a cell whose corner is
+// load serves one voxel-view update.  A 16-bit sinogram gets the tap-column
+// copy instead: element (p, vv, uu) holds (u, v..v+3), and two neighbouring
+// elements (16 bytes) are the taps of columns u, u+1 on rows v..v+3, enough
+// for two voxels of a column whose rows differ by at most 2.  Cells with a
+// corner in [-2, nu] x [-2, nv] are in the copies; a cell whose corner is
 // outside [-1, n-1] on either axis has no tap on the detector, and clamping
 // its corner into [-2, n] lands on another such cell, all zeros, which
 // replaces the per-tap range tests.  Values are copied as stored.
 struct QuadShape {
-    uint32_t row;        // elements per row of the copy: 2 (nu + 4)
+    uint32_t row;        // elements per row of the copy: per (nu + 4)
     uint32_t view;       // elements per view: row (nv + 3)
 };
-__host__ __device__ __forceinline__ QuadShape quad_shape(int nu, int nv) {
+__host__ __device__ __forceinline__ QuadShape quad_shape(int nu, int nv, uint32_t per) {
     QuadShape q;
-    q.row = 2u * (uint32_t)(nu + 4);
+    q.row = per * (uint32_t)(nu + 4);
     q.view = q.row * (uint32_t)(nv + 3);
     return q;
 }
@@ -92,13 +97,42 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
     __builtin_memset(&b, 0, sizeof(b));
     if (uok && v >= 0 && v < nv) a = gp[(size_t)v * nu + u];
     if (uok && v + 1 >= 0 && v + 1 < nv) b = gp[(size_t)(v + 1) * nu + u];
-    const QuadShape qs = quad_shape(nu, nv);
+    const QuadShape qs = quad_shape(nu, nv, 2);
     const size_t o = (size_t)p * qs.view + (size_t)vv * qs.row + 2u * (uint32_t)uu;
     out[o] = a;
     out[o + 1] = b;
 }
 
-// Forward VI on the tap-quad copy.  Bilinear interpolation at (u, v): the
+// Tap-column copy of a 16-bit sinogram (same grid as the tap quads).
+__device__ __forceinline__ void vi_columns_tile(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int nu,
+                                                int nv, int bx, int by, int p) {
+    const int uu = bx * 64 + (threadIdx.x & 63);
+    const int vv = by * 4 + (threadIdx.x >> 6);
+    if (uu > nu + 3 || vv > nv + 2) return;
+    const int u = uu - 2, v = vv - 2;
+    const bool uok = u >= 0 && u < nu;
+    const uint16_t* gp = in + (size_t)p * nu * nv;
+    uint32_t t[4];
+#pragma unroll
+    for (int r = 0; r < 4; ++r) t[r] = (uok && v + r >= 0 && v + r < nv) ? gp[(size_t)(v + r) * nu + u] : 0u;
+    const QuadShape qs = quad_shape(nu, nv, 4);
+    const uint2 w = make_uint2(t[0] | (t[1] << 16), t[2] | (t[3] << 16));
+    __builtin_memcpy(out + (size_t)p * qs.view + (size_t)vv * qs.row + 4u * (uint32_t)uu, &w, sizeof(w));
+}
+
+// Raw taps of the second voxel of a pair from the first one's tap-column
+// load: rows d, d+1 of columns u (words c0 = rows 0,1 and c1 = rows 2,3) and
+// u+1 (e0, e1), d in {0, 1, 2} its clamped row less the first's.  When the
+// row step A of the column is in [0, 1.99], consecutive voxels' v differ by
+// less than 2, their floors (and the clamped rows) by 0, 1 or 2.
+__device__ __forceinline__ void pair_rows(float d, uint32_t c0, uint32_t c1, uint32_t e0, uint32_t e1, uint32_t& b0,
+                                          uint32_t& b1) {
+    const uint32_t cm = __builtin_amdgcn_alignbyte(c1, c0, 2), em = __builtin_amdgcn_alignbyte(e1, e0, 2);
+    b0 = d == 0.0f ? c0 : (d == 1.0f ? cm : c1);
+    b1 = d == 0.0f ? e0 : (d == 1.0f ? em : e1);
+}
+
+// Forward VI on the tap-quad copy (FP32) or the tap-column copy (16-bit).  Bilinear interpolation at (u, v): the
 // taps (u,v), (u,v+1), (u+1,v), (u+1,v+1) of the cell with lower corner
 // (floor u, floor v), 0 off the detector, interpolated along u, then v, with
 // explicit fmaf.  The cell corner is clamped into the copy (a cell off the
@@ -106,8 +140,11 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
 // below 16 MiB): buffer loads at a byte offset computed as one exact FP32
 // fma of small integers; otherwise element indices in integers.
 // VI stores delta * sum; FDK weights each term by W = num / sqrt(fma(v', v',
-// base)), v' = z' A, and stores the sum itself.
-template <typename TS, typename TV, bool FDK>
+// base)), v' = z' A, and stores the sum itself.  16-bit storage with PAIRS
+// (the host has shown A in [0, 1.99] for every column and view) reads the
+// taps of voxels k and k+1 with one tap-column load; without it, one load per
+// voxel.  The taps and the arithmetic of each term are the same either way.
+template <typename TS, typename TV, bool FDK, bool PAIRS = false>
 __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__ quads, float sino_inv,
                                         TV* __restrict__ vol, float vol_scale, float post, int bx, int by, int bz) {
     const int i = bx * TX + (threadIdx.x % TX);
@@ -120,13 +157,14 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
     float acc[ZB];
 #pragma unroll
     for (int k = 0; k < ZB; ++k) acc[k] = 0.0f;
-    const QuadShape qs = quad_shape(g.nu, g.nv);
+    constexpr bool kF16 = std::is_same<TS, __half>::value;
+    constexpr bool kBF16 = std::is_same<TS, __hip_bfloat16>::value;
+    constexpr uint32_t kPer = (kF16 || kBF16) ? 4u : 2u;  // values per element: tap columns / tap quads
+    const QuadShape qs = quad_shape(g.nu, g.nv, kPer);
     const float fnu = (float)g.nu, fnv = (float)g.nv;
     constexpr uint32_t es = sizeof(TS);
     const uint64_t view_bytes = (uint64_t)qs.view * es;
     const bool fast = view_bytes < (1ull << 24) && view_bytes * (uint64_t)g.nphi < (1ull << 32);
-    constexpr bool kF16 = std::is_same<TS, __half>::value;
-    constexpr bool kBF16 = std::is_same<TS, __hip_bfloat16>::value;
     // one voxel-view term from its interpolated value r
     auto add = [&](const ColumnView& cvw, int k, float r) {
         if (FDK) {
@@ -173,20 +211,43 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
             const float uf = floorf(cvw.u);
             const float fu = cvw.u - uf;
             const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
-            // byte offset of cell (uc, vc): ((vc + 2) row + 2 (uc + 2)) es
-            const float base = fmaf(uc + 2.0f, (float)(2u * es), 2.0f * row_b);
+            // byte offset of cell (uc, vc): ((vc + 2) row + per (uc + 2)) es
+            const float base = fmaf(uc + 2.0f, (float)(kPer * es), 2.0f * row_b);
             const uint32_t soff = (uint32_t)p * (uint32_t)view_bytes;
-            const __half2 fu2 = __float2half2_rn(fu);
+            if constexpr (kF16 || kBF16) {
+                const __half2 fu2 = __float2half2_rn(fu);
+                if constexpr (PAIRS) {
 #pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                const float vf = floorf(v);
-                const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                const uint32_t off = (uint32_t)fmaf(vc, row_b, base);
-                if constexpr (kF16 || kBF16) {
-                    const auto w = __builtin_amdgcn_raw_buffer_load_b64(rsrc, (int)off, (int)soff, 0);
-                    term16(cvw, fu2, fu, k, v, vf, w[0], w[1]);
+                    for (int k = 0; k < ZB; k += 2) {
+                        const float va = fmaf(zp0 + (float)k, cvw.A, cv), vb = fmaf(zp0 + (float)(k + 1), cvw.A, cv);
+                        const float vfa = floorf(va), vfb = floorf(vb);
+                        const float vca = __builtin_amdgcn_fmed3f(vfa, -2.0f, fnv);
+                        const float vcb = __builtin_amdgcn_fmed3f(vfb, -2.0f, fnv);
+                        const auto w = __builtin_amdgcn_raw_buffer_load_b128(
+                            rsrc, (int)(uint32_t)fmaf(vca, row_b, base), (int)soff, 0);
+                        term16(cvw, fu2, fu, k, va, vfa, w[0], w[2]);
+                        uint32_t b0, b1;
+                        pair_rows(vcb - vca, w[0], w[1], w[2], w[3], b0, b1);
+                        term16(cvw, fu2, fu, k + 1, vb, vfb, b0, b1);
+                    }
                 } else {
+#pragma unroll
+                    for (int k = 0; k < ZB; ++k) {
+                        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                        const float vf = floorf(v);
+                        const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
+                        const auto w = __builtin_amdgcn_raw_buffer_load_b96(
+                            rsrc, (int)(uint32_t)fmaf(vc, row_b, base), (int)soff, 0);
+                        term16(cvw, fu2, fu, k, v, vf, w[0], w[2]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < ZB; ++k) {
+                    const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                    const float vf = floorf(v);
+                    const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
+                    const uint32_t off = (uint32_t)fmaf(vc, row_b, base);
                     term(cvw, fu, k, v, vf, BufLoad4<TS>::load(rsrc, off, soff, sino_inv));
                 }
             }
@@ -197,19 +258,41 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
             const float uf = floorf(cvw.u);
             const float fu = cvw.u - uf;
             const int uc = (int)__builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
-            const TS* gp = quads + (size_t)p * qs.view + 2u * (uint32_t)(uc + 2);
-            const __half2 fu2 = __float2half2_rn(fu);
+            const TS* gp = quads + (size_t)p * qs.view + kPer * (uint32_t)(uc + 2);
+            if constexpr (kF16 || kBF16) {
+                const __half2 fu2 = __float2half2_rn(fu);
+                if constexpr (PAIRS) {
 #pragma unroll
-            for (int k = 0; k < ZB; ++k) {
-                const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                const float vf = floorf(v);
-                const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                const uint64_t e = (uint64_t)(uint32_t)(vc + 2) * qs.row;
-                if constexpr (kF16 || kBF16) {
-                    uint32_t w[2];
-                    __builtin_memcpy(w, gp + e, sizeof(w));
-                    term16(cvw, fu2, fu, k, v, vf, w[0], w[1]);
+                    for (int k = 0; k < ZB; k += 2) {
+                        const float va = fmaf(zp0 + (float)k, cvw.A, cv), vb = fmaf(zp0 + (float)(k + 1), cvw.A, cv);
+                        const float vfa = floorf(va), vfb = floorf(vb);
+                        const float vca = __builtin_amdgcn_fmed3f(vfa, -2.0f, fnv);
+                        const float vcb = __builtin_amdgcn_fmed3f(vfb, -2.0f, fnv);
+                        uint32_t w[4];
+                        __builtin_memcpy(w, gp + (uint64_t)(uint32_t)((int)vca + 2) * qs.row, sizeof(w));
+                        term16(cvw, fu2, fu, k, va, vfa, w[0], w[2]);
+                        uint32_t b0, b1;
+                        pair_rows(vcb - vca, w[0], w[1], w[2], w[3], b0, b1);
+                        term16(cvw, fu2, fu, k + 1, vb, vfb, b0, b1);
+                    }
                 } else {
+#pragma unroll
+                    for (int k = 0; k < ZB; ++k) {
+                        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                        const float vf = floorf(v);
+                        const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
+                        uint32_t w[3];
+                        __builtin_memcpy(w, gp + (uint64_t)(uint32_t)(vc + 2) * qs.row, sizeof(w));
+                        term16(cvw, fu2, fu, k, v, vf, w[0], w[2]);
+                    }
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < ZB; ++k) {
+                    const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                    const float vf = floorf(v);
+                    const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
+                    const uint64_t e = (uint64_t)(uint32_t)(vc + 2) * qs.row;
                     term(cvw, fu, k, v, vf, Storage<TS>::load4(gp, e, sino_inv));
                 }
             }
@@ -229,13 +312,16 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
 template <typename TS>
 __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
                                                      int nv) {
-    vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
+    if constexpr (sizeof(TS) == 2)
+        vi_columns_tile((const uint16_t*)in, (uint16_t*)out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
+    else
+        vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <typename TS, typename TV, bool FDK>
+template <typename TS, typename TV, bool FDK, bool PAIRS>
 __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
                                                   TV* __restrict__ vol, float vol_scale) {
-    vi_tile<TS, TV, FDK>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
+    vi_tile<TS, TV, FDK, PAIRS>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Shared launches (FP32 storage).  Tap quads: tile grid (ceil((nu+4)/64),
@@ -310,17 +396,46 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 
 }  // namespace
 
+// True when every column-view's row step A = delta FDD/dv / w is in
+// [0, 1.98] (1.99 in the kernel, less a margin far above A's FP32 rounding).
+// w = delta (cos x' + sin y') + FOD is linear in (x', y'), so its minimum over
+// the voxel centres is at a corner of their rectangle.
+static bool vi_rows_pairable(const tomo_geom* g) {
+    const DevGeom& d = g->dev;
+    const double xs[2] = {-(double)d.cx, (double)(d.nx - 1) - (double)d.cx};
+    const double ys[2] = {-(double)d.cy, (double)(d.ny - 1) - (double)d.cy};
+    for (int p = 0; p < d.nphi; ++p) {
+        const double* c = &g->view[(size_t)p * kViewStride];
+        double wmin = 1.0e300;
+        for (double x : xs)
+            for (double y : ys) wmin = std::min(wmin, std::fma(c[2], x, std::fma(c[3], y, d.fod)));
+        if (!(wmin > 0.0) || !(c[6] >= 0.0) || !(c[6] / wmin <= 1.98)) return false;
+    }
+    return true;
+}
+
+template <typename TS, typename TV, bool FDK>
+static void launch_vi_kernel(const DevGeom& d, dim3 grid, bool pairs, const void* quads, float inv, void* vol,
+                             float vol_scale, hipStream_t s) {
+    if (sizeof(TS) == 2 && pairs)
+        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, sizeof(TS) == 2>), grid, dim3(TX * TY), 0, s, d,
+                           (const TS*)quads, inv, (TV*)vol, vol_scale);
+    else
+        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, false>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
+                           (TV*)vol, vol_scale);
+}
+
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) {
     const DevGeom& d = g->dev;
-    const size_t es = sino.dtype == TOMO_F32 ? 4 : 2;
-    const size_t quad_elems = vi_quad_elems(d.nu, d.nv, d.nphi);
-    // Tap-quad copy of the sinogram from the stream-ordered pool, released
-    // after the backprojection on the same stream.
+    // Tap-quad (FP32) or tap-column (16-bit) copy of the sinogram from the
+    // stream-ordered pool, released after the backprojection on the same
+    // stream; both take 4 bytes per tap-quad element.
     void* quads = nullptr;
-    TOMO_HIP(hipMallocAsync(&quads, quad_elems * es, s));
+    TOMO_HIP(hipMallocAsync(&quads, vi_quad_elems(d.nu, d.nv, d.nphi) * 4, s));
     const dim3 qgrid(ceil_div(d.nu + 4, 64), ceil_div(d.nv + 3, 4), d.nphi);
     const dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
     const float inv = 1.0f / sino.scale;
+    const bool pairs = sino.dtype != TOMO_F32 && vi_rows_pairable(g);
     hipError_t le = hipSuccess;
     TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
         hipLaunchKernelGGL((vi_quads_kernel<TS>), qgrid, dim3(256), 0, s, (const TS*)sino.p, (TS*)quads, d.nu, d.nv);
@@ -328,11 +443,9 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
         if (le == hipSuccess) {
             TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
                 if (fdk)
-                    hipLaunchKernelGGL((vi_kernel<TS, TV, true>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
-                                       (TV*)vol.p, vol.scale);
+                    launch_vi_kernel<TS, TV, true>(d, grid, pairs, quads, inv, vol.p, vol.scale, s);
                 else
-                    hipLaunchKernelGGL((vi_kernel<TS, TV, false>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads,
-                                       inv, (TV*)vol.p, vol.scale);
+                    launch_vi_kernel<TS, TV, false>(d, grid, pairs, quads, inv, vol.p, vol.scale, s);
             });
             le = hipGetLastError();
         }
