@@ -30,7 +30,8 @@ struct SidRay {
     bool ok;
 };
 
-// FP64 ray setup.  tmax is the FOV chord, shortened to the first crossing
+// FP64 ray setup (first crossings at +0 or later: "+ 0.0f" turns a -0 into
+// +0).  tmax is the FOV chord, shortened to the first crossing
 // that takes a voxel index out of the grid (the n-th crossing of axis a is
 // at fma(n, inc_a, first_a), the value the traversal computes).  A traversal
 // that stops at that crossing takes exactly the segments of one that steps
@@ -75,12 +76,12 @@ __device__ __forceinline__ SidRay sid_setup(const DevGeom& g, int p, int m, int 
         r.idx[ax] = (int)fi;
         if (d[ax] > 0.0) {
             r.step[ax] = 1;
-            r.first[ax] = (float)fmax(((fi + 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0);
+            r.first[ax] = (float)fmax(((fi + 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0) + 0.0f;
             r.inc[ax] = (float)(dl / d[ax]);
             tmax = fminf(tmax, fmaf((float)(nn[ax] - 1 - r.idx[ax]), r.inc[ax], r.first[ax]));
         } else if (d[ax] < 0.0) {
             r.step[ax] = -1;
-            r.first[ax] = (float)fmax(((fi - 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0);
+            r.first[ax] = (float)fmax(((fi - 0.5 - cen[ax]) * dl - S[ax]) / d[ax] - l0, 0.0) + 0.0f;
             r.inc[ax] = (float)(-dl / d[ax]);
             tmax = fminf(tmax, fmaf((float)r.idx[ax], r.inc[ax], r.first[ax]));
         } else {
@@ -180,6 +181,18 @@ __device__ __forceinline__ VolReader<TV, BUF> make_reader(const TV* p, unsigned 
     return v;
 }
 
+// Minimum and maximum of two times of the walk.  Every such time is +0 or
+// positive (the first crossings are clamped at +0, increments and tmax are
+// positive), and non-negative floats order as their bit patterns do as
+// unsigned integers, so one integer min / max gives the exact result without
+// the NaN canonicalisation of fminf / fmaxf.
+__device__ __forceinline__ float min_nn(float a, float b) {
+    return __builtin_bit_cast(float, min(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b)));
+}
+__device__ __forceinline__ float max_nn(float a, float b) {
+    return __builtin_bit_cast(float, max(__builtin_bit_cast(uint32_t, a), __builtin_bit_cast(uint32_t, b)));
+}
+
 // Per-lane state of the general forward traversal: next crossing and
 // crossings taken per axis, the linear index of the current voxel, the
 // running sum.
@@ -202,8 +215,8 @@ __device__ __forceinline__ void sid_advance(SidState& s, const SidRay& r) {
 // the nearest crossing; ties step every tied axis.
 template <typename RD>
 __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD& rd) {
-    const float tn = fminf(s.tn[0], fminf(s.tn[1], s.tn[2]));
-    const float te = fminf(tn, r.tmax);
+    const float tn = min_nn(s.tn[0], min_nn(s.tn[1], s.tn[2]));
+    const float te = min_nn(tn, r.tmax);
     s.acc = fmaf(te - s.t, rd(s.lin), s.acc);
     s.t = te;
     if (s.tn[0] == tn) sid_advance<0>(s, r);
@@ -222,20 +235,21 @@ __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD&
 // tmax.  These are the segments of the general step in the same order; an
 // absent one has zero length and adds exactly 0.  ZNEG: the wave's rays go
 // down in z (one detector row, one sign).
-// Quad reads of a z-pair copy by element index: a buffer resource with a
-// 32-bit byte offset (copies below 4 GiB) or a plain global load.
+// Quad reads of a z-pair copy: a buffer resource with a 32-bit byte offset
+// (copies below 4 GiB; the walk carries the offset in bytes) or a plain
+// global load by element index.
 template <typename TV, bool BUF> struct QuadReader;
 template <typename TV> struct QuadReader<TV, false> {
+    static constexpr uint32_t kUnit = 1;  // element index
     const TV* p;
     float inv;
     __device__ __forceinline__ float4 operator()(uint32_t e) const { return Storage<TV>::load4(p, e, inv); }
 };
 template <typename TV> struct QuadReader<TV, true> {
+    static constexpr uint32_t kUnit = sizeof(TV);  // byte offset
     __amdgpu_buffer_rsrc_t r;
     float inv;
-    __device__ __forceinline__ float4 operator()(uint32_t e) const {
-        return BufLoad4<TV>::load(r, e * (uint32_t)sizeof(TV), 0u, inv);
-    }
+    __device__ __forceinline__ float4 operator()(uint32_t off) const { return BufLoad4<TV>::load(r, off, 0u, inv); }
 };
 template <typename TV, bool BUF>
 __device__ __forceinline__ QuadReader<TV, BUF> make_quad_reader(const TV* p, uint32_t bytes, float inv) {
@@ -246,28 +260,31 @@ __device__ __forceinline__ QuadReader<TV, BUF> make_quad_reader(const TV* p, uin
     return q;
 }
 
-template <int A, bool ZNEG, typename QR>
+// BNEG: 0 / 1 when every ray of the wave steps up / down along B, 2 when
+// the wave mixes both (per-lane selects).
+template <int A, bool ZNEG, int BNEG, typename QR>
 __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, int nA, int nB, int nz, bool live,
                                                float& acc_out) {
     constexpr int B = 1 - A;
     const int sA = r.step[A], sB = r.step[B];
-    const bool bneg = sB < 0;
+    const bool bneg = BNEG == 2 ? sB < 0 : BNEG == 1;
     float tA = r.first[A], tB = r.first[B], tz = r.first[2];
     float cA = 0.0f, cB = 0.0f, cz = 0.0f;
     int iA = r.idx[A];
     float t = 0.0f, acc = 0.0f;
-    // Element index of the block corner (A = iA, B = iB or iB - 1, z = iz or
+    // Offset of the block corner (A = iA, B = iB or iB - 1, z = iz or
     // iz - 1), carried by the steps.  A live lane's voxel is inside the grid
     // (tmax stops at the first crossing that leaves it), so the corner is in
-    // the copy; the index of a finished lane is never used.
+    // the copy; the offset of a finished lane is never used.
+    constexpr uint32_t u = QR::kUnit;
     const uint32_t sz_e = 2u * (uint32_t)(nB + 2), sa_e = sz_e * (uint32_t)(nz + 1);
     uint32_t e = 0u;
     if (live) {
         const int bb = (bneg ? r.idx[B] - 1 : r.idx[B]) + 1;
         const int zz = (ZNEG ? r.idx[2] - 1 : r.idx[2]) + 1;
-        e = (uint32_t)iA * sa_e + (uint32_t)zz * sz_e + 2u * (uint32_t)bb;
+        e = ((uint32_t)iA * sa_e + (uint32_t)zz * sz_e + 2u * (uint32_t)bb) * u;
     }
-    const uint32_t dA = (uint32_t)sA * sa_e, dB = 2u * (uint32_t)sB, dZ = (uint32_t)r.step[2] * sz_e;
+    const uint32_t dA = (uint32_t)sA * sa_e * u, dB = 2u * (uint32_t)sB * u, dZ = (uint32_t)r.step[2] * sz_e * u;
     int I = sA > 0 ? wave_min(live ? iA : 0x7fffffff) : wave_max(live ? iA : -0x7fffffff);
     for (int outer = 0; outer <= nA + 1 && __any(live); ++outer) {
         if (live && iA == I) {
@@ -279,10 +296,10 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
             const float Vbz = bneg ? (ZNEG ? v.x : v.y) : (ZNEG ? v.z : v.w);
             const bool hb = tB < tA, hz = tz < tA;
             const float tb = hb ? tB : tA, tzz = hz ? tz : tA;
-            const float e1 = fminf(tb, tzz), e2 = fmaxf(tb, tzz);
+            const float e1 = min_nn(tb, tzz), e2 = max_nn(tb, tzz);
             const float V1 = tb < tzz ? Vb : (tzz < tb ? Vz : Vbz);
             const float V2 = hb ? (hz ? Vbz : Vb) : (hz ? Vz : V0);
-            const float c1 = fminf(e1, r.tmax), c2 = fminf(e2, r.tmax), c3 = fminf(tA, r.tmax);
+            const float c1 = min_nn(e1, r.tmax), c2 = min_nn(e2, r.tmax), c3 = min_nn(tA, r.tmax);
             acc = fmaf(c1 - t, V0, acc);
             acc = fmaf(c2 - c1, V1, acc);
             acc = fmaf(c3 - c2, V2, acc);
@@ -299,6 +316,16 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
         I += sA;
     }
     acc_out = acc;
+}
+
+template <int A, bool ZNEG, typename QR>
+__device__ __forceinline__ void sid_walk_quads_b(const SidRay& r, const QR& rd, int nA, int nB, int nz, bool live,
+                                                 float& acc) {
+    const bool pos_all = __ballot(live && r.step[1 - A] < 0) == 0ull;
+    const bool neg_all = __ballot(live && r.step[1 - A] >= 0) == 0ull;
+    if (pos_all) sid_walk_quads<A, ZNEG, 0>(r, rd, nA, nB, nz, live, acc);
+    else if (neg_all) sid_walk_quads<A, ZNEG, 1>(r, rd, nA, nB, nz, live, acc);
+    else sid_walk_quads<A, ZNEG, 2>(r, rd, nA, nB, nz, live, acc);
 }
 
 // Forward clean Siddon.  Each lane takes the segments of the per-ray
@@ -385,12 +412,12 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
         acc = s.acc;
     } else if (mode < 2) {
         const auto rq = make_quad_reader<TV, BUF>(qx, qx_bytes, 1.0f);
-        if (zneg_all && !zpos_all) sid_walk_quads<0, true>(r, rq, nx, ny, nz, ok, acc);
-        else sid_walk_quads<0, false>(r, rq, nx, ny, nz, ok, acc);
+        if (zneg_all && !zpos_all) sid_walk_quads_b<0, true>(r, rq, nx, ny, nz, ok, acc);
+        else sid_walk_quads_b<0, false>(r, rq, nx, ny, nz, ok, acc);
     } else {
         const auto rq = make_quad_reader<TV, BUF>(qy, qy_bytes, 1.0f);
-        if (zneg_all && !zpos_all) sid_walk_quads<1, true>(r, rq, ny, nx, nz, ok, acc);
-        else sid_walk_quads<1, false>(r, rq, ny, nx, nz, ok, acc);
+        if (zneg_all && !zpos_all) sid_walk_quads_b<1, true>(r, rq, ny, nx, nz, ok, acc);
+        else sid_walk_quads_b<1, false>(r, rq, ny, nx, nz, ok, acc);
     }
     // 16-bit volumes are read unscaled; the sum is linear in them, so the
     // storage scale applies once here (FP32 storage has no scale).
