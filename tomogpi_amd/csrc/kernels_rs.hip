@@ -157,6 +157,82 @@ __device__ __forceinline__ float trilinear_octet(const uint4* __restrict__ q, in
     return fmaf(fz, c1 - c0, c0);
 }
 
+// Buffer-path samplers, for copies below 4 GiB whose A-plane stride is
+// below 2^24 bytes: 32-bit byte offsets from 24-bit multiply-adds, loads
+// issued unconditionally.  A buffer load at an offset past the copy's end
+// returns 0 and never faults, so an absent plane (outside the grid along A),
+// a sample whose cell has no voxel in the grid, or an inactive one (act
+// false) reads from kOutside: its taps are 0 and the interpolation gives +0,
+// the value the index forms return for it.  Corners are clamped before the integer conversion, which
+// keeps every offset well defined.  Same taps and arithmetic as above.
+constexpr uint32_t kOutside = 0xfffffff0u;
+
+struct CopyBuf {
+    __amdgpu_buffer_rsrc_t r;
+    uint32_t sa, sz;  // bytes per A plane and per z row
+};
+
+template <int A>
+__device__ __forceinline__ float trilinear_quads_buf(const CopyBuf& q, int nx, int ny, int nz, float x, float y,
+                                                     float z, bool act) {
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    const float xc = __builtin_amdgcn_fmed3f(xf, -1.0f, (float)nx);
+    const float yc = __builtin_amdgcn_fmed3f(yf, -1.0f, (float)ny);
+    const float zc = __builtin_amdgcn_fmed3f(zf, -1.0f, (float)nz);
+    const bool in = act & (xc == xf) & (yc == yf) & (zc == zf);
+    const int ix = (int)xc, iy = (int)yc, iz = (int)zc;
+    const float fx = x - xf, fy = y - yf, fz = z - zf;
+    const int ia = A == 0 ? ix : iy, ib = A == 0 ? iy : ix;
+    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    const bool blk = in & (ib < nB) & (iz < nz);
+    const bool p0 = blk & (ia >= 0) & (ia < nA), p1 = blk & (ia + 1 < nA);
+    const uint32_t bq = (uint32_t)(min(ib, nB - 1) + 1), zq = (uint32_t)(min(iz, nz - 1) + 1);
+    const uint32_t inner = __umul24(zq, q.sz) + bq * 8u;
+    const uint32_t o0 = p0 ? __umul24((uint32_t)max(ia, 0), q.sa) + inner : kOutside;
+    const uint32_t o1 = p1 ? __umul24((uint32_t)(ia + 1), q.sa) + inner : kOutside;
+    const auto u = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o0, 0, 0);
+    const auto w = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o1, 0, 0);
+    // u, w = (b, z), (b, z+1), (b+1, z), (b+1, z+1) on planes ia, ia+1
+    const float u0 = __uint_as_float(u[0]), u1 = __uint_as_float(u[1]), u2 = __uint_as_float(u[2]),
+                u3 = __uint_as_float(u[3]);
+    const float w0 = __uint_as_float(w[0]), w1 = __uint_as_float(w[1]), w2 = __uint_as_float(w[2]),
+                w3 = __uint_as_float(w[3]);
+    float t000, t100, t010, t110, t001, t101, t011, t111;
+    if (A == 0) {
+        t000 = u0; t001 = u1; t010 = u2; t011 = u3;
+        t100 = w0; t101 = w1; t110 = w2; t111 = w3;
+    } else {
+        t000 = u0; t001 = u1; t100 = u2; t101 = u3;
+        t010 = w0; t011 = w1; t110 = w2; t111 = w3;
+    }
+    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
+    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
+    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
+    return fmaf(fz, c1 - c0, c0);
+}
+
+template <int A, typename TV>
+__device__ __forceinline__ float trilinear_octet_buf(const CopyBuf& q, int nx, int ny, int nz, float x, float y,
+                                                     float z, bool act) {
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    const float xc = __builtin_amdgcn_fmed3f(xf, -1.0f, (float)(nx - 1));
+    const float yc = __builtin_amdgcn_fmed3f(yf, -1.0f, (float)(ny - 1));
+    const float zc = __builtin_amdgcn_fmed3f(zf, -1.0f, (float)(nz - 1));
+    const bool in = act & (xc == xf) & (yc == yf) & (zc == zf);
+    const int ix = (int)xc, iy = (int)yc, iz = (int)zc;
+    const float fx = x - xf, fy = y - yf, fz = z - zf;
+    const uint32_t a = (uint32_t)((A == 0 ? ix : iy) + 1), b = (uint32_t)((A == 0 ? iy : ix) + 1);
+    const uint32_t o = in ? __umul24(a, q.sa) + __umul24((uint32_t)(iz + 1), q.sz) + b * 16u : kOutside;
+    const auto e = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o, 0, 0);
+    using U = Unpack16<TV>;
+    const float t000 = U::lo(e[0]), t100 = U::hi(e[0]), t010 = U::lo(e[1]), t110 = U::hi(e[1]);
+    const float t001 = U::lo(e[2]), t101 = U::hi(e[2]), t011 = U::lo(e[3]), t111 = U::hi(e[3]);
+    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
+    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
+    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
+    return fmaf(fz, c1 - c0, c0);
+}
+
 __device__ __forceinline__ int wave_min(int v) {
     for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
     return v;
@@ -174,23 +250,73 @@ __device__ __forceinline__ float wave_minf(float v) {
 // Samples of one lane, taken one per iteration k of the wave's loop: sample
 // s = k - k0 for k0 <= k < k0 + lr.  Same positions (fma(s, d, entry)) and
 // the same order of the sum as the per-ray loop.
-// q is the z-pair copy of A for FP32 storage, its octet copy for 16-bit.
-template <int A, typename TV>
-__device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restrict__ q, int nx, int ny, int nz,
-                                            int k0, int kend) {
+// q is the z-pair copy of A for FP32 storage, its octet copy for 16-bit;
+// BUF reads it through qb (see the buffer-path samplers).  The buffer path
+// has no branch: RS_UNROLL samples per iteration, their loads in flight
+// together, an inactive sample (s outside [0, lr)) adding +0 in its place,
+// which leaves the sum unchanged (it starts at +0 and is never -0).
+#ifndef RS_UNROLL
+#define RS_UNROLL 4
+#endif
+template <int A, typename TV, bool BUF>
+__device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restrict__ q, const CopyBuf& qb, int nx,
+                                            int ny, int nz, int k0, int kend) {
     float acc = 0.0f;
-    for (int k = 0; k < kend; ++k) {
-        const int s = k - k0;
-        if (s >= 0 && s < r.lr) {
-            const float sf = (float)s;
-            const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
-            if constexpr (sizeof(TV) == 2)
-                acc += trilinear_octet<A, TV>((const uint4*)q, nx, ny, nz, x, y, z);
-            else
-                acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
+    if constexpr (BUF) {
+        for (int k = 0; k < kend; k += RS_UNROLL) {
+            float v[RS_UNROLL];
+#pragma unroll
+            for (int j = 0; j < RS_UNROLL; ++j) {
+                const int s = k + j - k0;
+                const float sf = (float)s;
+                const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
+                const bool act = (s >= 0) & (s < r.lr);
+                if constexpr (sizeof(TV) == 2) v[j] = trilinear_octet_buf<A, TV>(qb, nx, ny, nz, x, y, z, act);
+                else v[j] = trilinear_quads_buf<A>(qb, nx, ny, nz, x, y, z, act);
+            }
+#pragma unroll
+            for (int j = 0; j < RS_UNROLL; ++j) acc += v[j];
+        }
+    } else {
+        for (int k = 0; k < kend; ++k) {
+            const int s = k - k0;
+            if (s >= 0 && s < r.lr) {
+                const float sf = (float)s;
+                const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
+                if constexpr (sizeof(TV) == 2)
+                    acc += trilinear_octet<A, TV>((const uint4*)q, nx, ny, nz, x, y, z);
+                else
+                    acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
+            }
         }
     }
     return acc;
+}
+
+// Byte strides of the copy read by the buffer path, and whether it applies.
+struct CopyShape {
+    uint64_t bytes_x, bytes_y;
+    uint32_t sa_x, sz_x, sa_y, sz_y;
+    bool buf;
+};
+__host__ __device__ __forceinline__ CopyShape rs_copy_shape(int nx, int ny, int nz, bool octet) {
+    CopyShape c;
+    if (octet) {
+        c.sz_x = (uint32_t)(ny + 1) * 16u;
+        c.sz_y = (uint32_t)(nx + 1) * 16u;
+        c.bytes_x = octet_elems(nx, ny, nz) * 16ull;
+        c.bytes_y = octet_elems(ny, nx, nz) * 16ull;
+    } else {
+        c.sz_x = (uint32_t)(ny + 2) * 8u;
+        c.sz_y = (uint32_t)(nx + 2) * 8u;
+        c.bytes_x = zpair_elems_x(nx, ny, nz) * 4ull;
+        c.bytes_y = zpair_elems_y(nx, ny, nz) * 4ull;
+    }
+    const uint64_t sax = (uint64_t)c.sz_x * (uint64_t)(nz + 1), say = (uint64_t)c.sz_y * (uint64_t)(nz + 1);
+    c.sa_x = (uint32_t)sax;
+    c.sa_y = (uint32_t)say;
+    c.buf = c.bytes_x < kOutside && c.bytes_y < kOutside && sax < (1ull << 24) && say < (1ull << 24);
+    return c;
 }
 
 // Forward RS projector.  Samples, taps and the order of the sum are those of
@@ -206,7 +332,7 @@ __device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restri
 // neighbouring cells of its octet copy, one 16-byte load per sample (16-bit
 // storage).  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
 // (every view of a row band before the next band).
-template <typename TV, typename TS>
+template <typename TV, typename TS, bool BUF>
 __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* __restrict__ qx,
                                         const void* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
                                         float sino_scale, int bx, int p, int bz) {
@@ -243,8 +369,16 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* 
     float acc;
     // 16-bit volumes are read unscaled; the sum is linear in them, so the
     // storage scale applies once to it (FP32 storage has no scale).
-    if (mode >= 0 && mode < 2) acc = rs_samples<0, TV>(r, qx, nx, ny, nz, k0, kend);
-    else acc = rs_samples<1, TV>(r, qy, nx, ny, nz, k0, kend);
+    CopyBuf qb{};
+    if constexpr (BUF) {
+        const CopyShape cs = rs_copy_shape(nx, ny, nz, sizeof(TV) == 2);
+        const bool xs = mode >= 0 && mode < 2;
+        qb.r = buffer_rsrc(xs ? qx : qy, (uint32_t)(xs ? cs.bytes_x : cs.bytes_y));
+        qb.sa = xs ? cs.sa_x : cs.sa_y;
+        qb.sz = xs ? cs.sz_x : cs.sz_y;
+    }
+    if (mode >= 0 && mode < 2) acc = rs_samples<0, TV, BUF>(r, qx, qb, nx, ny, nz, k0, kend);
+    else acc = rs_samples<1, TV, BUF>(r, qy, qb, nx, ny, nz, k0, kend);
     if (sizeof(TV) == 2) acc *= vol_inv;
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
@@ -252,11 +386,11 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* 
     }
 }
 
-template <typename TV, typename TS>
+template <typename TV, typename TS, bool BUF>
 __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const void* __restrict__ qx,
                                                   const void* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
                                                   float sino_scale) {
-    rs_tile<TV, TS>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
+    rs_tile<TV, TS, BUF>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Shared launch (FP32 storage); item flags bit 0 selects the cube FOV.
@@ -265,8 +399,12 @@ __global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __re
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        rs_tile<float, float>(it.g, it.flags & 1, (const float*)it.in, (const float*)it.in2, 1.0f, (float*)it.out,
-                              1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
+        if (rs_copy_shape(it.g.nx, it.g.ny, it.g.nz, false).buf)
+            rs_tile<float, float, true>(it.g, it.flags & 1, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx,
+                                        (int)(rest % it.gy), (int)(rest / it.gy));
+        else
+            rs_tile<float, float, false>(it.g, it.flags & 1, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx,
+                                         (int)(rest % it.gy), (int)(rest / it.gy));
     }
 }
 
@@ -329,10 +467,15 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
     const void* cx = vol.dtype == TOMO_F32 ? zp.qx : oc.ox;
     const void* cy = vol.dtype == TOMO_F32 ? zp.qy : oc.oy;
     if (le == hipSuccess) {
+        const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, vol.dtype != TOMO_F32).buf;
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-                hipLaunchKernelGGL((rs_kernel<TV, TS>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv, (TS*)sino.p,
-                                   sino.scale);
+                if (buf)
+                    hipLaunchKernelGGL((rs_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv,
+                                       (TS*)sino.p, sino.scale);
+                else
+                    hipLaunchKernelGGL((rs_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv,
+                                       (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();
