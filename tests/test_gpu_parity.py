@@ -117,9 +117,8 @@ def test_siddon_bit_exact(setup):
 @pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon"])
 def test_16bit_volume_bit_exact(setup, dt, op):
     """16-bit volumes at storage scale 1 projected into FP32 sinograms: the
-    sums are the FP32 oracle's on the rounded voxel values, bit for bit.  RS
-    reads such volumes through octet copies (one 16-byte cell per sample),
-    Siddon through 16-bit z-pair copies."""
+    sums are the FP32 oracle's on the rounded voxel values, bit for bit (both
+    projectors read 16-bit z-pair copies)."""
     g, o, G = setup
     rng = np.random.default_rng(9)
     vol = ellipsoids(g.nx, g.ny, g.nz, count=12, seed=4, amin=2.0, amax=12.0)

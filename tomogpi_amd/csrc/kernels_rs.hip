@@ -133,32 +133,8 @@ __device__ __forceinline__ float trilinear_quads(const TV* __restrict__ q, int n
     return fmaf(fz, c1 - c0, c0);
 }
 
-// The same interpolation from one element of the octet copy of A (16-bit
-// storage, values unscaled): the eight taps, the weights and the order of
-// the operations are those of trilinear_quads, so results are identical.
-// Cells with a corner index n (or beyond) have no voxel inside the grid and
-// give +0, as the zero taps do there.
-template <int A, typename TV>
-__device__ __forceinline__ float trilinear_octet(const uint4* __restrict__ q, int nx, int ny, int nz, float x,
-                                                 float y, float z) {
-    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
-    if (!(xf >= -1.0f && xf <= (float)(nx - 1) && yf >= -1.0f && yf <= (float)(ny - 1) && zf >= -1.0f &&
-          zf <= (float)(nz - 1)))
-        return 0.0f;
-    const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
-    const float fx = x - xf, fy = y - yf, fz = z - zf;
-    const uint4 e = A == 0 ? q[octet_index(ix, iz, iy, nz, ny)] : q[octet_index(iy, iz, ix, nz, nx)];
-    using U = Unpack16<TV>;
-    const float t000 = U::lo(e.x), t100 = U::hi(e.x), t010 = U::lo(e.y), t110 = U::hi(e.y);
-    const float t001 = U::lo(e.z), t101 = U::hi(e.z), t011 = U::lo(e.w), t111 = U::hi(e.w);
-    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
-    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
-    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
-    return fmaf(fz, c1 - c0, c0);
-}
-
-// Buffer-path samplers, for copies below 4 GiB whose A-plane stride is
-// below 2^24 bytes: 32-bit byte offsets from 24-bit multiply-adds, loads
+// Buffer-path sampler, for copies below 4 GiB whose A-plane stride is below
+// 2^24 bytes: 32-bit byte offsets from 24-bit multiply-adds, loads
 // issued unconditionally.  A buffer load at an offset past the copy's end
 // returns 0 and never faults, so an absent plane (outside the grid along A),
 // a sample whose cell has no voxel in the grid, or an inactive one (act
@@ -172,9 +148,10 @@ struct CopyBuf {
     uint32_t sa, sz;  // bytes per A plane and per z row
 };
 
-template <int A>
+template <int A, typename TV>
 __device__ __forceinline__ float trilinear_quads_buf(const CopyBuf& q, int nx, int ny, int nz, float x, float y,
                                                      float z, bool act) {
+    constexpr uint32_t es = sizeof(TV);  // a z-pair element is 2 es bytes
     const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
     const float xc = __builtin_amdgcn_fmed3f(xf, -1.0f, (float)nx);
     const float yc = __builtin_amdgcn_fmed3f(yf, -1.0f, (float)ny);
@@ -187,16 +164,23 @@ __device__ __forceinline__ float trilinear_quads_buf(const CopyBuf& q, int nx, i
     const bool blk = in & (ib < nB) & (iz < nz);
     const bool p0 = blk & (ia >= 0) & (ia < nA), p1 = blk & (ia + 1 < nA);
     const uint32_t bq = (uint32_t)(min(ib, nB - 1) + 1), zq = (uint32_t)(min(iz, nz - 1) + 1);
-    const uint32_t inner = __umul24(zq, q.sz) + bq * 8u;
+    const uint32_t inner = __umul24(zq, q.sz) + bq * (2u * es);
     const uint32_t o0 = p0 ? __umul24((uint32_t)max(ia, 0), q.sa) + inner : kOutside;
     const uint32_t o1 = p1 ? __umul24((uint32_t)(ia + 1), q.sa) + inner : kOutside;
-    const auto u = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o0, 0, 0);
-    const auto w = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o1, 0, 0);
     // u, w = (b, z), (b, z+1), (b+1, z), (b+1, z+1) on planes ia, ia+1
-    const float u0 = __uint_as_float(u[0]), u1 = __uint_as_float(u[1]), u2 = __uint_as_float(u[2]),
-                u3 = __uint_as_float(u[3]);
-    const float w0 = __uint_as_float(w[0]), w1 = __uint_as_float(w[1]), w2 = __uint_as_float(w[2]),
-                w3 = __uint_as_float(w[3]);
+    float u0, u1, u2, u3, w0, w1, w2, w3;
+    if constexpr (es == 4) {
+        const auto u = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o0, 0, 0);
+        const auto w = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o1, 0, 0);
+        u0 = __uint_as_float(u[0]); u1 = __uint_as_float(u[1]); u2 = __uint_as_float(u[2]); u3 = __uint_as_float(u[3]);
+        w0 = __uint_as_float(w[0]); w1 = __uint_as_float(w[1]); w2 = __uint_as_float(w[2]); w3 = __uint_as_float(w[3]);
+    } else {
+        using U = Unpack16<TV>;
+        const auto u = __builtin_amdgcn_raw_buffer_load_b64(q.r, (int)o0, 0, 0);
+        const auto w = __builtin_amdgcn_raw_buffer_load_b64(q.r, (int)o1, 0, 0);
+        u0 = U::lo(u[0]); u1 = U::hi(u[0]); u2 = U::lo(u[1]); u3 = U::hi(u[1]);
+        w0 = U::lo(w[0]); w1 = U::hi(w[0]); w2 = U::lo(w[1]); w3 = U::hi(w[1]);
+    }
     float t000, t100, t010, t110, t001, t101, t011, t111;
     if (A == 0) {
         t000 = u0; t001 = u1; t010 = u2; t011 = u3;
@@ -205,28 +189,6 @@ __device__ __forceinline__ float trilinear_quads_buf(const CopyBuf& q, int nx, i
         t000 = u0; t001 = u1; t100 = u2; t101 = u3;
         t010 = w0; t011 = w1; t110 = w2; t111 = w3;
     }
-    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
-    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
-    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
-    return fmaf(fz, c1 - c0, c0);
-}
-
-template <int A, typename TV>
-__device__ __forceinline__ float trilinear_octet_buf(const CopyBuf& q, int nx, int ny, int nz, float x, float y,
-                                                     float z, bool act) {
-    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
-    const float xc = __builtin_amdgcn_fmed3f(xf, -1.0f, (float)(nx - 1));
-    const float yc = __builtin_amdgcn_fmed3f(yf, -1.0f, (float)(ny - 1));
-    const float zc = __builtin_amdgcn_fmed3f(zf, -1.0f, (float)(nz - 1));
-    const bool in = act & (xc == xf) & (yc == yf) & (zc == zf);
-    const int ix = (int)xc, iy = (int)yc, iz = (int)zc;
-    const float fx = x - xf, fy = y - yf, fz = z - zf;
-    const uint32_t a = (uint32_t)((A == 0 ? ix : iy) + 1), b = (uint32_t)((A == 0 ? iy : ix) + 1);
-    const uint32_t o = in ? __umul24(a, q.sa) + __umul24((uint32_t)(iz + 1), q.sz) + b * 16u : kOutside;
-    const auto e = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o, 0, 0);
-    using U = Unpack16<TV>;
-    const float t000 = U::lo(e[0]), t100 = U::hi(e[0]), t010 = U::lo(e[1]), t110 = U::hi(e[1]);
-    const float t001 = U::lo(e[2]), t101 = U::hi(e[2]), t011 = U::lo(e[3]), t111 = U::hi(e[3]);
     const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
     const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
     const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
@@ -250,8 +212,8 @@ __device__ __forceinline__ float wave_minf(float v) {
 // Samples of one lane, taken one per iteration k of the wave's loop: sample
 // s = k - k0 for k0 <= k < k0 + lr.  Same positions (fma(s, d, entry)) and
 // the same order of the sum as the per-ray loop.
-// q is the z-pair copy of A for FP32 storage, its octet copy for 16-bit;
-// BUF reads it through qb (see the buffer-path samplers).  The buffer path
+// q is the z-pair copy of A (FP32 or 16-bit, as stored); BUF reads it
+// through qb (see the buffer-path samplers).  The buffer path
 // has no branch: RS_UNROLL samples per iteration, their loads in flight
 // together, an inactive sample (s outside [0, lr)) adding +0 in its place,
 // which leaves the sum unchanged (it starts at +0 and is never -0).
@@ -271,8 +233,7 @@ __device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restri
                 const float sf = (float)s;
                 const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
                 const bool act = (s >= 0) & (s < r.lr);
-                if constexpr (sizeof(TV) == 2) v[j] = trilinear_octet_buf<A, TV>(qb, nx, ny, nz, x, y, z, act);
-                else v[j] = trilinear_quads_buf<A>(qb, nx, ny, nz, x, y, z, act);
+                v[j] = trilinear_quads_buf<A, TV>(qb, nx, ny, nz, x, y, z, act);
             }
 #pragma unroll
             for (int j = 0; j < RS_UNROLL; ++j) acc += v[j];
@@ -283,10 +244,7 @@ __device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restri
             if (s >= 0 && s < r.lr) {
                 const float sf = (float)s;
                 const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
-                if constexpr (sizeof(TV) == 2)
-                    acc += trilinear_octet<A, TV>((const uint4*)q, nx, ny, nz, x, y, z);
-                else
-                    acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
+                acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
             }
         }
     }
@@ -299,19 +257,12 @@ struct CopyShape {
     uint32_t sa_x, sz_x, sa_y, sz_y;
     bool buf;
 };
-__host__ __device__ __forceinline__ CopyShape rs_copy_shape(int nx, int ny, int nz, bool octet) {
+__host__ __device__ __forceinline__ CopyShape rs_copy_shape(int nx, int ny, int nz, uint32_t es) {
     CopyShape c;
-    if (octet) {
-        c.sz_x = (uint32_t)(ny + 1) * 16u;
-        c.sz_y = (uint32_t)(nx + 1) * 16u;
-        c.bytes_x = octet_elems(nx, ny, nz) * 16ull;
-        c.bytes_y = octet_elems(ny, nx, nz) * 16ull;
-    } else {
-        c.sz_x = (uint32_t)(ny + 2) * 8u;
-        c.sz_y = (uint32_t)(nx + 2) * 8u;
-        c.bytes_x = zpair_elems_x(nx, ny, nz) * 4ull;
-        c.bytes_y = zpair_elems_y(nx, ny, nz) * 4ull;
-    }
+    c.sz_x = (uint32_t)(ny + 2) * 2u * es;
+    c.sz_y = (uint32_t)(nx + 2) * 2u * es;
+    c.bytes_x = zpair_elems_x(nx, ny, nz) * es;
+    c.bytes_y = zpair_elems_y(nx, ny, nz) * es;
     const uint64_t sax = (uint64_t)c.sz_x * (uint64_t)(nz + 1), say = (uint64_t)c.sz_y * (uint64_t)(nz + 1);
     c.sa_x = (uint32_t)sax;
     c.sa_y = (uint32_t)say;
@@ -328,9 +279,8 @@ __host__ __device__ __forceinline__ CopyShape rs_copy_shape(int nx, int ny, int 
 // behind that one in samples (|d_A| per sample): at iteration k every lane
 // samples near the same A plane, one sample per lane per iteration.  Its 64
 // rays (adjacent detector columns) then read neighbouring quads of the
-// z-pair copy of A, two 16-byte loads per sample (FP32 storage), or the
-// neighbouring cells of its octet copy, one 16-byte load per sample (16-bit
-// storage).  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
+// z-pair copy of A, two loads per sample (16 bytes each for FP32 storage,
+// 8 for 16-bit).  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
 // (every view of a row band before the next band).
 template <typename TV, typename TS, bool BUF>
 __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* __restrict__ qx,
@@ -371,7 +321,7 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* 
     // storage scale applies once to it (FP32 storage has no scale).
     CopyBuf qb{};
     if constexpr (BUF) {
-        const CopyShape cs = rs_copy_shape(nx, ny, nz, sizeof(TV) == 2);
+        const CopyShape cs = rs_copy_shape(nx, ny, nz, sizeof(TV));
         const bool xs = mode >= 0 && mode < 2;
         qb.r = buffer_rsrc(xs ? qx : qy, (uint32_t)(xs ? cs.bytes_x : cs.bytes_y));
         qb.sa = xs ? cs.sa_x : cs.sa_y;
@@ -399,7 +349,7 @@ __global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __re
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        if (rs_copy_shape(it.g.nx, it.g.ny, it.g.nz, false).buf)
+        if (rs_copy_shape(it.g.nx, it.g.ny, it.g.nz, 4).buf)
             rs_tile<float, float, true>(it.g, it.flags & 1, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx,
                                         (int)(rest % it.gy), (int)(rest / it.gy));
         else
@@ -459,15 +409,13 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
     const DevGeom& d = g->dev;
     const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     const float inv = 1.0f / vol.scale;
-    // FP32 volumes are read through z-pair copies, 16-bit ones through octet copies.
+    const uint32_t es = vol.dtype == TOMO_F32 ? 4u : 2u;
     ZPairs zp;
-    Octets oc;
-    hipError_t le = vol.dtype == TOMO_F32 ? zp.alloc(vol.p, 4, d.nx, d.ny, d.nz, s)
-                                          : oc.alloc(vol.p, d.nx, d.ny, d.nz, s);
-    const void* cx = vol.dtype == TOMO_F32 ? zp.qx : oc.ox;
-    const void* cy = vol.dtype == TOMO_F32 ? zp.qy : oc.oy;
+    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s);
+    const void* cx = zp.qx;
+    const void* cy = zp.qy;
     if (le == hipSuccess) {
-        const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, vol.dtype != TOMO_F32).buf;
+        const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, es).buf;
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)
@@ -480,9 +428,7 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s)
         });
         le = hipGetLastError();
     }
-    hipError_t fe = zp.release(s);
-    const hipError_t fe2 = oc.release(s);
-    if (fe == hipSuccess) fe = fe2;
+    const hipError_t fe = zp.release(s);
     TOMO_HIP(le);
     TOMO_HIP(fe);
     return TOMO_OK;

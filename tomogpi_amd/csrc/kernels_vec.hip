@@ -233,42 +233,6 @@ __global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, 
     zpair_x_tile<T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
 }
 
-// Octet copies of a 16-bit volume: a workgroup makes the 32 x 32 cells
-// (cx, cy) = (ix+1, iy+1) of one zz = iz+1 for both copies from the 33 x 33
-// x 2 voxels it stages in LDS (x fastest, coalesced).  Each thread then
-// writes 16-byte elements: along B = y for ox, along B = x for oy, so both
-// stores are contiguous.  Grid (ceil((nx+1)/32), ceil((ny+1)/32), nz+1).
-__device__ __forceinline__ uint4 octet_pack(const uint16_t (*v)[33][34], int i, int j) {
-    uint4 e;
-    e.x = (uint32_t)v[0][j][i] | ((uint32_t)v[0][j][i + 1] << 16);          // (0,0,0) (1,0,0)
-    e.y = (uint32_t)v[0][j + 1][i] | ((uint32_t)v[0][j + 1][i + 1] << 16);  // (0,1,0) (1,1,0)
-    e.z = (uint32_t)v[1][j][i] | ((uint32_t)v[1][j][i + 1] << 16);          // (0,0,1) (1,0,1)
-    e.w = (uint32_t)v[1][j + 1][i] | ((uint32_t)v[1][j + 1][i + 1] << 16);  // (0,1,1) (1,1,1)
-    return e;
-}
-
-__global__ __launch_bounds__(256) void octet_kernel(const uint16_t* __restrict__ in, uint4* __restrict__ ox,
-                                                    uint4* __restrict__ oy, int nx, int ny, int nz) {
-    __shared__ uint16_t v[2][33][34];  // [z offset][y - (cy0-1)][x - (cx0-1)]
-    const int cx0 = blockIdx.x * 32, cy0 = blockIdx.y * 32, zz = blockIdx.z;
-    for (int e = threadIdx.x; e < 2 * 33 * 33; e += 256) {
-        const int i = e % 33, r = e / 33, j = r % 33, c = r / 33;
-        const int x = cx0 - 1 + i, y = cy0 - 1 + j, z = zz - 1 + c;
-        uint16_t val = 0;
-        if (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-            val = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
-        v[c][j][i] = val;
-    }
-    __syncthreads();
-    const int t = threadIdx.x & 31, w = threadIdx.x >> 5;
-    for (int r = w; r < 32; r += 8) {
-        if (cx0 + r <= nx && cy0 + t <= ny)
-            ox[((size_t)(cx0 + r) * (nz + 1) + zz) * (ny + 1) + (cy0 + t)] = octet_pack(v, r, t);
-        if (cx0 + t <= nx && cy0 + r <= ny)
-            oy[((size_t)(cy0 + r) * (nz + 1) + zz) * (nx + 1) + (cx0 + t)] = octet_pack(v, t, r);
-    }
-}
-
 // ---- shared launches of the gradient loop (FP32).  Element-wise tiles
 // cover kVecTile consecutive elements, 16 per thread at stride 256; FP64
 // partial sums are per tile, in that fixed order, so a problem's sums do not
@@ -470,12 +434,6 @@ unsigned grid_for(size_t n) {
 }
 
 }  // namespace
-
-hipError_t launch_octet(const void* vol, void* ox, void* oy, int nx, int ny, int nz, hipStream_t s) {
-    const dim3 grid(ceil_div((size_t)nx + 1, 32), ceil_div((size_t)ny + 1, 32), nz + 1);
-    hipLaunchKernelGGL(octet_kernel, grid, dim3(256), 0, s, (const uint16_t*)vol, (uint4*)ox, (uint4*)oy, nx, ny, nz);
-    return hipGetLastError();
-}
 
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s) {
     const dim3 gx(ceil_div(nx, 32), ceil_div((size_t)ny + 2, 32), nz + 1);

@@ -74,12 +74,6 @@ namespace tomo {
 __host__ __device__ inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
 __host__ __device__ inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
 
-// Elements (16 bytes each) of the octet copy of a 16-bit volume for slabs
-// along A: (nA+1) * (nz+1) * (nB+1) cells.
-__host__ __device__ inline size_t octet_elems(int nA, int nB, int nz) {
-    return (size_t)(nA + 1) * (nz + 1) * (nB + 1);
-}
-
 // Elements of the VI tap-quad copy of a sinogram: nphi views of (nv + 3)
 // rows of (nu + 4) pairs; its fill kernel's grid is (ceil((nu+4)/64),
 // ceil((nv+3)/4), nphi).
