@@ -18,6 +18,7 @@ the kernels run on.  One JSON line is printed, with these further legs:
 from __future__ import annotations
 
 import argparse
+import dataclasses
 import json
 import os
 import sys
@@ -45,7 +46,9 @@ def parse():
     ap.add_argument("--serving-schedule", choices=["direct", "recurrent"], default="recurrent")
     ap.add_argument("--mixed-oracle-views", type=int, default=8,
                     help="views of the FP64 oracle check of the mixed forms (0 skips the mixed leg)")
-    ap.add_argument("--batch-problems", type=int, default=32,
+    ap.add_argument("--c2-sizes", default="512,1024",
+                    help="edges of the mixed-precision workload (720 views, 768x512 detector); empty skips it")
+    ap.add_argument("--batch-problems", type=int, default=64,
                     help="uneven SIRT problems in the batched leg (0 skips it)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one element-wise copy of the volume (known bytes) before the timed steps, "
@@ -195,32 +198,35 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
 
 
 def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
-    """Mixed-precision forms on the bench workload: volumes and sinograms
-    stored in FP16 (scaled so max|x| = 2^14) or BF16, FP32 accumulation,
-    FP64 geometry.  Speed of each operator against its FP32 time, and the
-    error of each 16-bit result against the FP64 oracle on views
-    0..oracle_views-1 (a CPU-bounded subset of the same acquisition)."""
+    """Mixed-precision forms of one acquisition: volumes and sinograms stored
+    in FP16 (scaled so max|x| = 2^14) or BF16, FP32 accumulation, FP64
+    geometry.  Speed of each operator against its FP32 time, and the error of
+    each 16-bit result against the FP64 oracle on views 0..oracle_views-1 (a
+    CPU-bounded subset of the same acquisition)."""
     import numpy as np
     import torch
     from tomogpi_amd import lib
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    from oracle_ffi import Oracle, scaled_geometry
+    from oracle_ffi import Geometry, Oracle
     n, nphi = G.info["nx"], G.info["nphi"]
+    nu, nv = G.info["nu"], G.info["nv"]
     out = {}
     vmax = float(vol.abs().max())
-    # FP64 references on the view subset
-    gsub = scaled_geometry(n, oracle_views)
-    gsub.num_stop, gsub.den_stop = 2.0 * oracle_views, float(nphi)
+    # FP64 references on the view subset (same angles: the stop angle keeps the step)
+    gsub = Geometry(fod=desc.fod_mm, fdd=desc.fdd_mm, nphi=oracle_views, su=desc.u_detector_size_mm,
+                    sv=desc.v_detector_size_mm, nu=nu, nv=nv, sx=desc.x_volume_size_mm, sy=desc.y_volume_size_mm,
+                    sz=desc.z_volume_size_mm, nx=n, ny=n, nz=n, num_stop=2.0 * oracle_views, den_stop=float(nphi))
     o = Oracle(gsub)
     vol_host = vol.cpu().numpy()
     t0 = time.perf_counter()
     ref_sid = o.siddon(vol_host.astype(np.float64), f64=True)
     sino_sub = o.siddon(vol_host)  # FP32 sinogram subset, the backprojection input
     ref_vi = o.vi(sino_sub.astype(np.float64), f64=True)
-    print(f"[mixed] FP64 oracle on {oracle_views} views: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+    del vol_host
+    print(f"[mixed] n={n}: FP64 oracle on {oracle_views} views: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
-    dsub = lib.AcquisitionDesc.scaled(n, oracle_views)
-    dsub.num_stop_angle, dsub.den_stop_angle = 2.0 * oracle_views, float(nphi)
+    dsub = dataclasses.replace(desc, projection_nb=oracle_views, num_stop_angle=2.0 * oracle_views,
+                               den_stop_angle=float(nphi))
     Gs = lib.Geometry(ctx, dsub)
 
     def rel(a, b):
@@ -240,8 +246,8 @@ def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
                        ("rs", lambda: G.project(v16, s16, lib.PROJ_RS, vol_scale=sv, sino_scale=ss, stream=stream))):
             fn()
             ms[op], _ = timed(fn, 3, stream)
-        G.project(v16, s16, lib.PROJ_SIDDON, vol_scale=sv, sino_scale=ss, stream=stream)  # leave Siddon output
         torch.cuda.synchronize()
+        del b16, s16
         # errors on the view subset
         e_s = torch.empty(Gs.sino_shape, dtype=tdt, device="cuda")
         Gs.project(v16, e_s, lib.PROJ_SIDDON, vol_scale=sv, sino_scale=ss)
@@ -255,12 +261,50 @@ def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
             "siddon_ms": round(ms["siddon"], 3), "vi_ms": round(ms["vi"], 3), "rs_ms": round(ms["rs"], 3),
             "siddon_speedup": round(fp32_ms["siddon"] / ms["siddon"], 3),
             "vi_speedup": round(fp32_ms["vi"] / ms["vi"], 3), "rs_speedup": round(fp32_ms["rs"] / ms["rs"], 3),
-            "pair_speedup": round((fp32_ms["siddon"] + fp32_ms["vi"]) / (ms["siddon"] + ms["vi"]), 3),
+            "siddonvi_pair_speedup": round((fp32_ms["siddon"] + fp32_ms["vi"]) / (ms["siddon"] + ms["vi"]), 3),
+            "rsvi_pair_speedup": round((fp32_ms["rs"] + fp32_ms["vi"]) / (ms["rs"] + ms["vi"]), 3),
             "siddon_rel_l2_vs_fp64": float(f"{err_s:.3e}"), "vi_rel_l2_vs_fp64": float(f"{err_b:.3e}"),
             "error_sample": f"views 0..{oracle_views - 1}, 16-bit inputs and outputs"}
-        print(f"[mixed] {name}: {out[name]}", file=sys.stderr, flush=True)
-        del v16, s16, b16, e_s, e_b, sin16
+        print(f"[mixed] n={n} {name}: {out[name]}", file=sys.stderr, flush=True)
+        del v16, e_s, e_b, sin16
+        torch.cuda.empty_cache()
     Gs.close()
+    return out
+
+
+def c2_leg(ctx, sizes, stream, oracle_views):
+    """The mixed-precision workload at each edge n: 720 views over 360 deg,
+    768 x 512 detector with pixels of 2 n / 512 mm (full vertical coverage),
+    FOD / FDD 500 / 1000 mm scaled by n / 256, 1 mm voxels; volume: 64
+    ellipsoids, seed 7, semi-axes 5..120 voxels scaled by n / 512.  FP32
+    times of the three operators, then the 16-bit forms against them."""
+    import torch
+    from tomogpi_amd import lib
+    out = {}
+    for n in sizes:
+        desc = lib.AcquisitionDesc.scaled(n, 720, nu=768, nv=512, pixel_mm=2.0 * n / 512.0)
+        G = lib.Geometry(ctx, desc)
+        vol = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+        G.ellipsoids(vol, count=64, seed=7, amin=5.0 * n / 512.0, amax=120.0 * n / 512.0)
+        sino = torch.empty(G.sino_shape, dtype=torch.float32, device="cuda")
+        back = torch.empty(G.vol_shape, dtype=torch.float32, device="cuda")
+        fp32 = {}
+        for op, fn in (("siddon", lambda: G.project(vol, sino, lib.PROJ_SIDDON, stream=stream)),
+                       ("vi", lambda: G.backproject(sino, back, stream=stream)),
+                       ("rs", lambda: G.project(vol, sino, lib.PROJ_RS, stream=stream))):
+            fn()
+            fp32[op], _ = timed(fn, 3, stream)
+        upd = float(n) ** 3 * 720
+        print(f"[c2] n={n} FP32: " + ", ".join(f"{k} {v:.2f} ms" for k, v in fp32.items()), file=sys.stderr,
+              flush=True)
+        del sino, back
+        torch.cuda.empty_cache()
+        res = mixed_leg(ctx, G, desc, vol, fp32, stream, oracle_views if n <= 512 else max(1, oracle_views // 4))
+        res["fp32"] = {k: {"ms": round(v, 3), "gups": round(upd / v / 1e6, 2)} for k, v in fp32.items()}
+        out[str(n)] = res
+        del vol
+        G.close()
+        torch.cuda.empty_cache()
     return out
 
 
@@ -268,9 +312,10 @@ def batch_leg(ctx, n_prob, iters=20, seed=7):
     """Batched uneven SIRT reconstructions: problems with edge n uniform in
     {128,192,256,320,384}, views uniform in [90,360], n x n detector,
     16-ellipsoid phantoms seeded per problem, pairs alternating RSVI /
-    SIDDONVI, `iters` SIRT iterations each, FP32.  One at a time
-    (tomo_sirt_run) against all at once in the server's shared launches;
-    the volumes must agree bit for bit."""
+    SIDDONVI, `iters` SIRT iterations each.  One at a time (tomo_sirt_run,
+    FP32) against all at once in the server's shared launches, with FP32
+    operator copies (the volumes must agree bit for bit) and with BF16
+    operator copies (error against the FP32 volumes reported)."""
     import numpy as np
     import torch
     from tomogpi_amd import lib
@@ -297,28 +342,39 @@ def batch_leg(ctx, n_prob, iters=20, seed=7):
     torch.cuda.synchronize()
     t_seq = time.perf_counter() - t0
     print(f"[batch] one at a time: {t_seq:.2f} s", file=sys.stderr, flush=True)
-    outs = [torch.zeros(p["G"].vol_shape, dtype=torch.float32, device="cuda") for p in probs]
-    srv = lib.Server(ctx, max_active=min(64, n_prob), use_graph=True)
-    torch.cuda.synchronize()
-    t0 = time.perf_counter()
-    ids = [srv.submit(p["desc"], p["y"], o, pair=p["pair"], iters=iters, algorithm=lib.ALGO_SIRT)
-           for p, o in zip(probs, outs)]
-    for rid in ids:
-        srv.wait(rid)
-    t_batch = time.perf_counter() - t0
-    st = srv.stats()
-    srv.close()
-    torch.cuda.synchronize()
+
+    def batched(storage):
+        outs = [torch.zeros(p["G"].vol_shape, dtype=torch.float32, device="cuda") for p in probs]
+        srv = lib.Server(ctx, max_active=min(64, n_prob), use_graph=True)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        ids = [srv.submit(p["desc"], p["y"], o, pair=p["pair"], iters=iters, algorithm=lib.ALGO_SIRT,
+                          storage=storage) for p, o in zip(probs, outs)]
+        for rid in ids:
+            srv.wait(rid)
+        t = time.perf_counter() - t0
+        st = srv.stats()
+        srv.close()
+        torch.cuda.synchronize()
+        return outs, t, st
+
+    outs, t_batch, st = batched(lib.F32)
     same = all(bool(torch.equal(o, p["ref"])) for o, p in zip(outs, probs))
+    print(f"[batch] batched: {t_batch:.2f} s, identical={same}", file=sys.stderr, flush=True)
+    outs16, t_bf16, _ = batched(lib.BF16)
+    err16 = max(float(torch.linalg.vector_norm(a - b) / torch.linalg.vector_norm(b)) for a, b in zip(outs16, outs))
+    print(f"[batch] batched, BF16 copies: {t_bf16:.2f} s, max rel-L2 vs FP32 {err16:.3e}", file=sys.stderr,
+          flush=True)
     upd = sum(2.0 * iters * p["n"] ** 3 * p["nphi"] for p in probs)
     for p in probs:
         p["G"].close()
-    print(f"[batch] batched: {t_batch:.2f} s, identical={same}", file=sys.stderr, flush=True)
     return {"problems": n_prob, "iterations": iters, "algorithm": "SIRT", "storage": "f32",
             "one_at_a_time_s": round(t_seq, 3), "batched_s": round(t_batch, 3),
             "speedup": round(t_seq / t_batch, 3), "batched_gups": round(upd / t_batch / 1e9, 2),
             "peak_active": st["peak_active"], "rounds": st["rounds"], "identical_to_one_at_a_time": same,
-            "note": "BF16 storage is not offered by the shared launches (FP32 only)"}
+            "bf16_copies": {"batched_s": round(t_bf16, 3), "speedup_vs_one_at_a_time_fp32": round(t_seq / t_bf16, 3),
+                            "max_rel_l2_vs_fp32_batched": float(f"{err16:.3e}"),
+                            "note": "BF16 z-pair and tap copies rounded from the FP32 state; FP32 sums and state"}}
 
 
 def main():
@@ -411,6 +467,9 @@ def main():
     serving = None
     del vol, sino, back
     torch.cuda.empty_cache()
+    mixed_c2 = None
+    if a.c2_sizes.strip() and a.mixed_oracle_views > 0:
+        mixed_c2 = c2_leg(ctx, [int(x) for x in a.c2_sizes.split(",")], stream, a.mixed_oracle_views)
     if a.serving_requests > 0:
         sched = lib.SCHEDULE_RECURRENT if a.serving_schedule == "recurrent" else lib.SCHEDULE_DIRECT
         serving = serving_leg(ctx, a.serving_requests, [float(x) for x in a.serving_rates.split(",")], sched)
@@ -445,6 +504,7 @@ def main():
         "cpu_baseline": base,
         "operators": ops,
         "mixed": mixed,
+        "mixed_c2": mixed_c2,
         "serving": serving,
         "batched": batched,
     }

@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TOMOGPI_ABI_VERSION 2
+#define TOMOGPI_ABI_VERSION 3
 
 /* ---- status codes ------------------------------------------------------ */
 #define TOMO_OK                0
@@ -317,7 +317,12 @@ typedef struct tomo_serve_request {
     const float* sinogram;      /* measured projections; read at admission  */
     float*       volume;        /* result, written when the request is done */
     int32_t      algorithm;     /* TOMO_ALGO_*                              */
-    int32_t      reserved;
+    int32_t      storage;       /* element type of the operator copies (the
+                                   z-pair copies the projectors read, the
+                                   tap copy VI reads): TOMO_F32 (0) or
+                                   TOMO_BF16, rounded from the FP32 state
+                                   each time they are made; the state, the
+                                   sums and the result stay FP32            */
     double       relaxation;    /* SIRT relaxation (1.0 for plain SIRT)     */
 } tomo_serve_request;
 

@@ -205,3 +205,48 @@ def test_sirt_served_in_a_mix_equals_plain_sirt(ctx, problems):
     for i in (2, 4):
         assert np.array_equal(outs[i].cpu().numpy(), plain[i].cpu().numpy()), i
     assert hist[-1]["residual_sq"] < hist[0]["residual_sq"]
+
+
+def test_bf16_copies_alone_equals_mix_and_close_to_fp32(ctx):
+    """BF16 operator copies (storage=BF16): the projectors read BF16 z-pair
+    copies and VI BF16 tap columns, rounded from the FP32 state each time
+    they are made; sums, state and result stay FP32.  Each result is the
+    same alone and in the mix (bit-identical), and within rel-L2 2e-2 of
+    the same request with FP32 copies (6 iterations).  The mix has both
+    pairs, GD and SIRT, and a geometry whose row steps are near 4 (VI reads
+    one tap-column load per voxel there, one per voxel pair elsewhere)."""
+    from test_gpu_parity import FINE_ROWS
+    specs = [  # (geometry, pair, algorithm, lam, kopt)
+        (scaled_geometry(32, 24), lib.PAIR_SIDDONVI, lib.ALGO_GD, 0.01, 10),
+        (scaled_geometry(40, 30), lib.PAIR_RSVI, lib.ALGO_GD, 0.01, 3),
+        (scaled_geometry(36, 20), lib.PAIR_SIDDONVI, lib.ALGO_SIRT, 0.0, 0),
+        (FINE_ROWS, lib.PAIR_RSVI, lib.ALGO_GD, 0.01, 10),
+    ]
+    ys = []
+    for i, (g, pair, algo, lam, kopt) in enumerate(specs):
+        o = Oracle(g)
+        truth = ellipsoids(g.nx, g.ny, g.nz, count=5, seed=300 + i, amin=2.0, amax=g.nx / 3)
+        ys.append(torch.from_numpy(o.rs(truth) if pair == lib.PAIR_RSVI else o.siddon(truth)).cuda())
+
+    def run(idx, storage, **kw):
+        print(f"[case] serve bf16 {idx} storage={storage} {kw}", flush=True)
+        srv = lib.Server(ctx, **kw)
+        outs = {i: torch.zeros(specs[i][0].vol_shape, dtype=torch.float32, device="cuda") for i in idx}
+        torch.cuda.synchronize()
+        ids = {i: srv.submit(to_desc(specs[i][0]), ys[i], outs[i], pair=specs[i][1], iters=ITERS, kopt=specs[i][4],
+                             lam=specs[i][3], algorithm=specs[i][2], storage=storage) for i in idx}
+        for i in idx:
+            assert srv.wait(ids[i], timeout=120.0)["state"] == lib.REQ_DONE, i
+        srv.close()
+        torch.cuda.synchronize()
+        return {i: outs[i].cpu().numpy() for i in idx}
+
+    idx = list(range(len(specs)))
+    alone16 = {i: run([i], lib.BF16, max_active=1)[i] for i in idx}
+    alone32 = {i: run([i], lib.F32, max_active=1)[i] for i in idx}
+    mix16 = run(idx, lib.BF16, max_active=8)
+    for i in idx:
+        assert np.array_equal(mix16[i], alone16[i]), i
+        err = rel_l2(alone16[i], alone32[i])
+        print(f"[bf16] request {i}: rel_l2 vs FP32 copies {err:.3e}", flush=True)
+        assert 0 < err <= 2e-2, (i, err)

@@ -53,6 +53,14 @@ struct BatchTable {
     alignas(128) uint32_t next;
 };
 
+// Item flag bits of the operator launches (copies, projectors, VI): the
+// operator copies (z-pairs, tap columns) are BF16, converted from the FP32
+// state by the copy launches; VI on BF16 tap columns may read one load per
+// voxel pair (row steps below 2, shown on the host).  RS keeps bit 0 for the
+// cube FOV.
+constexpr int32_t kItemBF16 = 0x10;
+constexpr int32_t kItemPairs = 0x20;
+
 // Per-problem scalars of the gradient loop, FP64, in `slot`.
 enum GdSlot {
     kSlotResidual = 0,  // sum (y - P x)^2
@@ -95,12 +103,12 @@ __device__ __forceinline__ int batch_find(const BatchTable* t, uint32_t tile) {
 #endif
 
 // Shared launches (stream-ordered; `t` is a device copy of the table).
-// Storage is FP32 throughout.
-hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s);   // in volume -> out qx
-hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s);   // in volume -> out qy
+// The state is FP32 throughout; operator copies are FP32 or BF16 (kItemBF16).
+hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s);   // in volume -> out qx (FP32/BF16)
+hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s);   // in volume -> out qy (FP32/BF16)
 hipError_t launch_batch_siddon(BatchTable* t, hipStream_t s);          // in qx, in2 qy -> out sinogram
 hipError_t launch_batch_rs(BatchTable* t, hipStream_t s);              // in qx, in2 qy -> out; flags bit0 cube
-hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s);  // in sinogram -> out tap quads
+hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s);  // in sinogram -> tap quads / BF16 columns
 hipError_t launch_batch_vi(BatchTable* t, hipStream_t s);              // in tap quads -> out volume * a
 hipError_t launch_batch_diff_sq(const BatchTable* t, hipStream_t s);   // out = in - in2; aux partials of out^2
 hipError_t launch_batch_sq(const BatchTable* t, hipStream_t s);        // aux partials of in^2

@@ -343,18 +343,27 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const v
     rs_tile<TV, TS, BUF>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Shared launch (FP32 storage); item flags bit 0 selects the cube FOV.
-__global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+// Shared launch (FP32 or BF16 (kItemBF16) z-pair copies, FP32 sinograms);
+// item flags bit 0 selects the cube FOV.
+// 8 waves per SIMD (52 VGPRs, no spills; unbounded, its four sampler
+// variants take 86 and 5 waves).
+__global__ __launch_bounds__(BX* BY, 8) void rs_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
     TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        if (rs_copy_shape(it.g.nx, it.g.ny, it.g.nz, 4).buf)
-            rs_tile<float, float, true>(it.g, it.flags & 1, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx,
-                                        (int)(rest % it.gy), (int)(rest / it.gy));
+        const int by = (int)(rest % it.gy), bz = (int)(rest / it.gy), cube = it.flags & 1;
+        const bool bf = (it.flags & kItemBF16) != 0;
+        const bool buf = rs_copy_shape(it.g.nx, it.g.ny, it.g.nz, bf ? 2u : 4u).buf;
+        using BF = __hip_bfloat16;
+        if (bf && buf)
+            rs_tile<BF, float, true>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
+        else if (bf)
+            rs_tile<BF, float, false>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
+        else if (buf)
+            rs_tile<float, float, true>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
         else
-            rs_tile<float, float, false>(it.g, it.flags & 1, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx,
-                                         (int)(rest % it.gy), (int)(rest / it.gy));
+            rs_tile<float, float, false>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
     }
 }
 

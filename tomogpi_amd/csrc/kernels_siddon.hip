@@ -435,21 +435,32 @@ __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __r
 }
 
 // Shared launch: tiles are the single-problem grid's workgroups (column
-// group fastest, then view, then row group); FP32 storage.
+// group fastest, then view, then row group); FP32 or BF16 (kItemBF16) z-pair
+// copies, FP32 sinograms.
 __global__ __launch_bounds__(BX* BY) void siddon_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
     TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        const uint64_t qxb = zpair_elems_x(it.g.nx, it.g.ny, it.g.nz) * 4ull;
-        const uint64_t qyb = zpair_elems_y(it.g.nx, it.g.ny, it.g.nz) * 4ull;
-        if (qxb < (1ull << 32) && qyb < (1ull << 32))
+        const int by = (int)(rest % it.gy), bz = (int)(rest / it.gy);
+        const bool bf = (it.flags & kItemBF16) != 0;
+        const uint64_t es = bf ? 2ull : 4ull;
+        const uint64_t qxb = zpair_elems_x(it.g.nx, it.g.ny, it.g.nz) * es;
+        const uint64_t qyb = zpair_elems_y(it.g.nx, it.g.ny, it.g.nz) * es;
+        const bool buf = qxb < (1ull << 32) && qyb < (1ull << 32);
+        using BF = __hip_bfloat16;
+        if (bf && buf)
+            siddon_tile<BF, float, true>(it.g, (const BF*)it.in, (const BF*)it.in2, (unsigned)qxb, (unsigned)qyb, 1.0f,
+                                         (float*)it.out, 1.0f, (int)bx, by, bz);
+        else if (bf)
+            siddon_tile<BF, float, false>(it.g, (const BF*)it.in, (const BF*)it.in2, 0u, 0u, 1.0f, (float*)it.out,
+                                          1.0f, (int)bx, by, bz);
+        else if (buf)
             siddon_tile<float, float, true>(it.g, (const float*)it.in, (const float*)it.in2, (unsigned)qxb,
-                                            (unsigned)qyb, 1.0f, (float*)it.out, 1.0f, (int)bx, (int)(rest % it.gy),
-                                            (int)(rest / it.gy));
+                                            (unsigned)qyb, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
         else
             siddon_tile<float, float, false>(it.g, (const float*)it.in, (const float*)it.in2, 0u, 0u, 1.0f,
-                                             (float*)it.out, 1.0f, (int)bx, (int)(rest % it.gy), (int)(rest / it.gy));
+                                             (float*)it.out, 1.0f, (int)bx, by, bz);
     }
 }
 

@@ -7,6 +7,8 @@
 // as the reference host loops).  Squared norms are sum double(x)^2
 // (C/Image3D_CPU.cu:363-390; "L2Norm" without the square root), reduced in a
 // fixed order: per-block partials, then one block.
+#include <type_traits>
+
 #include "zpair.hpp"
 #include "batch.hpp"
 
@@ -170,10 +172,18 @@ __global__ __launch_bounds__(kBlock) void ellipsoid_kernel(int nx, int ny, int n
     vol[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * k)] = (float)acc;
 }
 
+// Element conversion of the copies: bit copy, or FP32 to BF16 (nearest
+// even) when the copy is BF16.
+template <typename TO, typename TI>
+__device__ __forceinline__ TO copy_cvt(TI v) {
+    if constexpr (std::is_same<TO, TI>::value) return v;
+    else return __bfloat16_as_ushort(__float2bfloat16(v));
+}
+
 // z-pair copy for y slabs (A = y, B = x): B is also the input's fastest axis,
 // a direct pass.  Grid (ceil((nx+2)/256), ny, nz+1).
-template <typename T>
-__device__ __forceinline__ void zpair_y_tile(const T* __restrict__ in, T* __restrict__ out, int nx, int ny, int nz,
+template <typename TI, typename T>
+__device__ __forceinline__ void zpair_y_tile(const TI* __restrict__ in, T* __restrict__ out, int nx, int ny, int nz,
                                              int bx, int y, int zz) {
     const int bb = bx * 256 + threadIdx.x;
     if (bb > nx + 1) return;
@@ -183,7 +193,8 @@ __device__ __forceinline__ void zpair_y_tile(const T* __restrict__ in, T* __rest
     for (int c = 0; c < 2; ++c) {
         const int z = zz - 1 + c;
         __builtin_memset(&v[c], 0, sizeof(T));
-        if (x >= 0 && x < nx && z >= 0 && z < nz) v[c] = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+        if (x >= 0 && x < nx && z >= 0 && z < nz)
+            v[c] = copy_cvt<T>(in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]);
     }
     const size_t o = zpair_index(y, zz, bb, nz, nx);
     out[o] = v[0];
@@ -192,8 +203,8 @@ __device__ __forceinline__ void zpair_y_tile(const T* __restrict__ in, T* __rest
 
 // z-pair copy for x slabs (A = x, B = y): a 32x32 (x, B) tile transpose of
 // two z planes through LDS.  Grid (ceil(nx/32), ceil((ny+2)/32), nz+1).
-template <typename T>
-__device__ __forceinline__ void zpair_x_tile(const T* __restrict__ in, T* __restrict__ out, int nx, int ny, int nz,
+template <typename TI, typename T>
+__device__ __forceinline__ void zpair_x_tile(const TI* __restrict__ in, T* __restrict__ out, int nx, int ny, int nz,
                                              int bx, int by, int zz, T (*tile)[32][33]) {
     const int x0 = bx * 32, b0 = by * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
@@ -205,7 +216,7 @@ __device__ __forceinline__ void zpair_x_tile(const T* __restrict__ in, T* __rest
             T v;
             __builtin_memset(&v, 0, sizeof(T));
             if (x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-                v = in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+                v = copy_cvt<T>(in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]);
             tile[c][r][tx] = v;
         }
     }
@@ -223,14 +234,14 @@ __device__ __forceinline__ void zpair_x_tile(const T* __restrict__ in, T* __rest
 template <typename T>
 __global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
                                                     int nz) {
-    zpair_y_tile<T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
+    zpair_y_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 template <typename T>
 __global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
                                                     int nz) {
     __shared__ T tile[2][32][33];
-    zpair_x_tile<T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
+    zpair_x_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
 }
 
 // ---- shared launches of the gradient loop (FP32).  Element-wise tiles
@@ -254,14 +265,20 @@ __device__ __forceinline__ void batch_coords(const BatchItem& it, uint32_t tile,
     bz = (int)(rest / it.gy);
 }
 
+// z-pair copies of FP32 volumes: FP32 (bit copy) or BF16 (kItemBF16).
 __global__ __launch_bounds__(256) void zpair_x_batch_kernel(const BatchTable* __restrict__ t) {
     __shared__ uint32_t tile_lds[2][32][33];
+    __shared__ uint16_t tile_lds16[2][32][33];
     TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
-        zpair_x_tile<uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by, bz,
-                               tile_lds);
+        if (it.flags & kItemBF16)
+            zpair_x_tile<float, uint16_t>((const float*)it.in, (uint16_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by,
+                                          bz, tile_lds16);
+        else
+            zpair_x_tile<uint32_t, uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx,
+                                             by, bz, tile_lds);
         __syncthreads();
     }
 }
@@ -271,7 +288,12 @@ __global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
-        zpair_y_tile<uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by, bz);
+        if (it.flags & kItemBF16)
+            zpair_y_tile<float, uint16_t>((const float*)it.in, (uint16_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by,
+                                          bz);
+        else
+            zpair_y_tile<uint32_t, uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx,
+                                             by, bz);
     }
 }
 

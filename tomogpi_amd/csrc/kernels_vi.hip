@@ -103,18 +103,26 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
     out[o + 1] = b;
 }
 
-// Tap-column copy of a 16-bit sinogram (same grid as the tap quads).
-__device__ __forceinline__ void vi_columns_tile(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int nu,
+// Tap-column copy of a 16-bit sinogram (same grid as the tap quads), or
+// BF16 tap columns of an FP32 sinogram (TI = float, nearest even).
+template <typename TI>
+__device__ __forceinline__ uint32_t column_bits(TI v) {
+    if constexpr (std::is_same<TI, float>::value) return __bfloat16_as_ushort(__float2bfloat16(v));
+    else return v;
+}
+template <typename TI>
+__device__ __forceinline__ void vi_columns_tile(const TI* __restrict__ in, uint16_t* __restrict__ out, int nu,
                                                 int nv, int bx, int by, int p) {
     const int uu = bx * 64 + (threadIdx.x & 63);
     const int vv = by * 4 + (threadIdx.x >> 6);
     if (uu > nu + 3 || vv > nv + 2) return;
     const int u = uu - 2, v = vv - 2;
     const bool uok = u >= 0 && u < nu;
-    const uint16_t* gp = in + (size_t)p * nu * nv;
+    const TI* gp = in + (size_t)p * nu * nv;
     uint32_t t[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r) t[r] = (uok && v + r >= 0 && v + r < nv) ? gp[(size_t)(v + r) * nu + u] : 0u;
+    for (int r = 0; r < 4; ++r)
+        t[r] = (uok && v + r >= 0 && v + r < nv) ? column_bits(gp[(size_t)(v + r) * nu + u]) : 0u;
     const QuadShape qs = quad_shape(nu, nv, 4);
     const uint2 w = make_uint2(t[0] | (t[1] << 16), t[2] | (t[3] << 16));
     __builtin_memcpy(out + (size_t)p * qs.view + (size_t)vv * qs.row + 4u * (uint32_t)uu, &w, sizeof(w));
@@ -313,7 +321,7 @@ template <typename TS>
 __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
                                                      int nv) {
     if constexpr (sizeof(TS) == 2)
-        vi_columns_tile((const uint16_t*)in, (uint16_t*)out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
+        vi_columns_tile<uint16_t>((const uint16_t*)in, (uint16_t*)out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
     else
         vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
@@ -324,16 +332,21 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
     vi_tile<TS, TV, FDK, PAIRS>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Shared launches (FP32 storage).  Tap quads: tile grid (ceil((nu+4)/64),
-// ceil((nv+3)/4), nphi).  VI: tile grid (ceil(nx/16), ceil(ny/16),
-// ceil(nz/16)); the stored value is (delta * sum) * a.
+// Shared launches (FP32 sinograms and volumes; kItemBF16: BF16 tap columns,
+// kItemPairs: their pair reads).  Tap quads / columns: tile grid
+// (ceil((nu+4)/64), ceil((nv+3)/4), nphi).  VI: tile grid (ceil(nx/16),
+// ceil(ny/16), ceil(nz/16)); the stored value is (delta * sum) * a.
 __global__ __launch_bounds__(256) void vi_quads_batch_kernel(const BatchTable* __restrict__ t) {
     TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        vi_quads_tile<float>((const float*)it.in, (float*)it.out, it.g.nu, it.g.nv, (int)bx, (int)(rest % it.gy),
-                             (int)(rest / it.gy));
+        if (it.flags & kItemBF16)
+            vi_columns_tile<float>((const float*)it.in, (uint16_t*)it.out, it.g.nu, it.g.nv, (int)bx,
+                                   (int)(rest % it.gy), (int)(rest / it.gy));
+        else
+            vi_quads_tile<float>((const float*)it.in, (float*)it.out, it.g.nu, it.g.nv, (int)bx, (int)(rest % it.gy),
+                                 (int)(rest / it.gy));
     }
 }
 
@@ -344,8 +357,15 @@ __global__ __launch_bounds__(TX* TY, 7) void vi_batch_kernel(const BatchTable* _
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
-        vi_tile<float, float, false>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx,
-                              (int)(rest % it.gy), (int)(rest / it.gy));
+        const int by = (int)(rest % it.gy), bz = (int)(rest / it.gy);
+        if (!(it.flags & kItemBF16))
+            vi_tile<float, float, false>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx, by, bz);
+        else if (it.flags & kItemPairs)
+            vi_tile<__hip_bfloat16, float, false, true>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out, 1.0f,
+                                                        it.a, (int)bx, by, bz);
+        else
+            vi_tile<__hip_bfloat16, float, false, false>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out,
+                                                         1.0f, it.a, (int)bx, by, bz);
     }
 }
 
@@ -400,7 +420,7 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 // [0, 1.98] (1.99 in the kernel, less a margin far above A's FP32 rounding).
 // w = delta (cos x' + sin y') + FOD is linear in (x', y'), so its minimum over
 // the voxel centres is at a corner of their rectangle.
-static bool vi_rows_pairable(const tomo_geom* g) {
+bool vi_rows_pairable(const tomo_geom* g) {
     const DevGeom& d = g->dev;
     const double xs[2] = {-(double)d.cx, (double)(d.nx - 1) - (double)d.cx};
     const double ys[2] = {-(double)d.cy, (double)(d.ny - 1) - (double)d.cy};

@@ -76,6 +76,8 @@ struct Request {
     float *qx = nullptr, *qy = nullptr, *y = nullptr, *px = nullptr, *r = nullptr, *pd = nullptr, *tq = nullptr;
     double *part_res = nullptr, *part_vol = nullptr, *part_reg = nullptr, *part_pd = nullptr;
     double *slot = nullptr, *hist = nullptr;
+    bool bf16 = false;      // BF16 operator copies
+    bool vi_pairs = false;  // BF16 VI reads one tap-column load per voxel pair
     int k = 0;              // iterations done
     bool prev_opt = false;  // the previous iteration computed P dJ (and L(L dJ))
     std::vector<double> hist_host;
@@ -127,7 +129,7 @@ namespace {
 uint32_t vec_tiles(uint64_t n) { return cdiv(n, kVecTile); }
 uint32_t lap_tiles(const DevGeom& d) { return cdiv(d.nx, 64) * cdiv(d.ny, 4) * (uint32_t)d.nz; }
 
-Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo) {
+Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo, size_t ces) {
     const DevGeom& d = g.dev;
     const size_t nv = g.nvox(), ns = g.nsino();
     const bool lam = gd.lambda != 0.0 && algo == TOMO_ALGO_GD;
@@ -141,13 +143,13 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo) {
     sizes[i++] = (lam || sirt) ? nv * 4 : 0;                             // 4 t1 (SIRT: volume weight C)
     sizes[i++] = lam ? nv * 4 : 0;                                       // 5 t2
     sizes[i++] = lam ? nv * 4 : 0;                                       // 6 t3
-    sizes[i++] = zpair_elems_x(d.nx, d.ny, d.nz) * 4;                    // 7 qx
-    sizes[i++] = zpair_elems_y(d.nx, d.ny, d.nz) * 4;                    // 8 qy
+    sizes[i++] = zpair_elems_x(d.nx, d.ny, d.nz) * ces;                  // 7 qx (FP32 or BF16)
+    sizes[i++] = zpair_elems_y(d.nx, d.ny, d.nz) * ces;                  // 8 qy
     sizes[i++] = ns * 4;                                                 // 9 y
     sizes[i++] = ns * 4;                                                 // 10 px
     sizes[i++] = ns * 4;                                                 // 11 r
     sizes[i++] = ns * 4;                                                 // 12 pd (SIRT: sinogram weight R)
-    sizes[i++] = vi_quad_elems(d.nu, d.nv, d.nphi) * 4;                  // 13 tq
+    sizes[i++] = vi_quad_elems(d.nu, d.nv, d.nphi) * 4;                  // 13 tq (FP32 quads or BF16 columns)
     sizes[i++] = (size_t)vec_tiles(ns) * 8;                              // 14 part_res
     sizes[i++] = (size_t)vec_tiles(nv) * 8;                              // 15 part_vol
     sizes[i++] = lam ? (size_t)lap_tiles(d) * 8 : 0;                     // 16 part_reg
@@ -167,7 +169,9 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo) {
 // Runs on the worker without the lock.  Returns TOMO_OK or a failure (the
 // request is then marked failed by the caller).
 int admit(tomo_server* srv, Request& q) {
-    const Layout L = layout_of(q.geom, q.req.gd, q.req.algorithm);
+    q.bf16 = q.req.storage == TOMO_BF16;
+    q.vi_pairs = q.bf16 && vi_rows_pairable(&q.geom);
+    const Layout L = layout_of(q.geom, q.req.gd, q.req.algorithm, q.bf16 ? 2 : 4);
     TOMO_HIP(hipMallocFromPoolAsync(&q.block, L.total, srv->pool, srv->stream));
     char* b = (char*)q.block;
     auto at = [&](int j) { return (void*)(b + L.off[j]); };
@@ -275,19 +279,23 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
         const uint32_t pgx = cdiv(d.nu, 64), pgz = cdiv(d.nv, 4);
         const bool sid = gd.pair == TOMO_PAIR_SIDDONVI;
         const int cube = (gd.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0;
+        const int32_t cflag = q.bf16 ? kItemBF16 : 0;  // operator copies in BF16
         auto add_projection = [&](const float* vol, float* out, int opx, int opy, int ops, int opr) {
             BatchItem& zx = tb[opx].add(d, cdiv(d.nx, 32), cdiv(d.ny + 2, 32), d.nz + 1);
             zx.in = vol;
             zx.out = q.qx;
+            zx.flags = cflag;
             BatchItem& zy = tb[opy].add(d, cdiv(d.nx + 2, 256), d.ny, d.nz + 1);
             zy.in = vol;
             zy.out = q.qy;
+            zy.flags = cflag;
             BatchItem& p = tb[sid ? ops : opr].add(d, pgx, d.nphi, pgz);
             p.in = q.qx;
             p.in2 = q.qy;
             p.out = out;
-            p.flags = cube;
+            p.flags = cube | cflag;
         };
+        const int32_t vflag = cflag | (q.vi_pairs ? kItemPairs : 0);
         if (q.req.algorithm == TOMO_ALGO_SIRT) {
             add_projection(q.x, q.px, OP_ZPX_X, OP_ZPY_X, OP_SID_X, OP_RS_X);
             BatchItem& df = tb[OP_DIFF].add_vec(d, ns);
@@ -304,10 +312,12 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
             BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 4, 64), cdiv(d.nv + 3, 4), d.nphi);
             vq.in = q.r;
             vq.out = q.tq;
+            vq.flags = cflag;
             BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, 16), cdiv(d.ny, 16), cdiv(d.nz, 16));
             vi.in = q.tq;
             vi.out = q.dj;
             vi.a = 1.0f;
+            vi.flags = vflag;
             BatchItem& st = tb[OP_STEP].add_one(d);
             st.slot = q.slot;
             st.flags = 2;
@@ -377,10 +387,12 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
         BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 4, 64), cdiv(d.nv + 3, 4), d.nphi);
         vq.in = q.r;
         vq.out = q.tq;
+        vq.flags = cflag;
         BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, 16), cdiv(d.ny, 16), cdiv(d.nz, 16));
         vi.in = q.tq;
         vi.out = q.dj;
         vi.a = -2.0f;
+        vi.flags = vflag;
         if (lam) {
             BatchItem& ar = tb[OP_AXPY_REG].add_vec(d, nv);
             ar.in = q.t2;
@@ -586,7 +598,8 @@ void worker_main(tomo_server* srv) {
                 // Admission: the head of the queue while it fits.
                 while (!srv->queue.empty() && !srv->broken) {
                     auto q = srv->queue.front();
-                    const uint64_t need = layout_of(q->geom, q->req.gd, q->req.algorithm).total;
+                    const uint64_t need =
+                        layout_of(q->geom, q->req.gd, q->req.algorithm, q->req.storage == TOMO_BF16 ? 2 : 4).total;
                     if ((int)(running.size() + admitted.size()) >= srv->params.max_active) break;
                     if (srv->bytes_in_use + need > srv->budget) {
                         if (!running.empty() || !admitted.empty()) break;
@@ -742,6 +755,8 @@ extern "C" int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req
         return fail(TOMO_ERR_INVALID_ARG, "unknown algorithm");
     if (req->algorithm == TOMO_ALGO_SIRT && !(req->relaxation > 0.0))
         return fail(TOMO_ERR_INVALID_ARG, "SIRT relaxation must be positive");
+    if (req->storage != TOMO_F32 && req->storage != TOMO_BF16)
+        return fail(TOMO_ERR_UNSUPPORTED, "operator copies are FP32 or BF16");
     auto q = std::make_shared<Request>();
     q->req = *req;
     q->host = (gd.flags & TOMO_FLAG_HOST_PTR) != 0;
@@ -779,7 +794,7 @@ void fill_result(const Request& q, tomo_serve_result* res) {
     res->done_s = q.done_s;
     res->latency_s = q.state >= TOMO_REQ_DONE ? q.done_s - q.submit_s : 0.0;
     res->residual_sq = q.residual_sq;
-    res->device_bytes = layout_of(q.geom, q.req.gd, q.req.algorithm).total;
+    res->device_bytes = layout_of(q.geom, q.req.gd, q.req.algorithm, q.req.storage == TOMO_BF16 ? 2 : 4).total;
 }
 }  // namespace
 

@@ -94,7 +94,7 @@ class SirtParams(C.Structure):
 
 class ServeRequest(C.Structure):
     _fields_ = [("geom", GeomDesc), ("gd", GdParams), ("sinogram", C.c_void_p), ("volume", C.c_void_p),
-                ("algorithm", C.c_int32), ("reserved", C.c_int32), ("relaxation", C.c_double)]
+                ("algorithm", C.c_int32), ("storage", C.c_int32), ("relaxation", C.c_double)]
 
 
 class ServeResult(C.Structure):
@@ -425,14 +425,17 @@ class Server:
         self._keep = {}  # request id -> arrays that must outlive the request
 
     def submit(self, desc: AcquisitionDesc, sino, vol, pair=PAIR_RSVI, flags=0, iters=20, kopt=21, lam=0.01,
-               positivity=0, alpha_min_init=-0.0347711, algorithm=ALGO_GD, relaxation=1.0) -> int:
+               positivity=0, alpha_min_init=-0.0347711, algorithm=ALGO_GD, relaxation=1.0, storage=F32) -> int:
+        """Queue one reconstruction; storage F32 or BF16 sets the element type
+        of its operator copies (the state and the result stay FP32)."""
         ps, hs = _ptr(sino)
         pv, hv = _ptr(vol)
         if hs != hv:
             raise ValueError("sinogram and volume must both be host or both be device arrays")
         gd = GdParams(pair=pair, flags=flags | (FLAG_HOST_PTR if hs else 0), gradient_iteration_nb=iters,
                       optimal_step_iteration_nb=kopt, lam=lam, positivity=positivity, alpha_min_init=alpha_min_init)
-        r = ServeRequest(geom=desc.to_c(), gd=gd, sinogram=ps, volume=pv, algorithm=algorithm, relaxation=relaxation)
+        r = ServeRequest(geom=desc.to_c(), gd=gd, sinogram=ps, volume=pv, algorithm=algorithm, relaxation=relaxation,
+                         storage=storage)
         rid = C.c_uint64()
         check(self.lib.tomo_serve_submit(self.h, C.byref(r), C.byref(rid)))
         self._keep[rid.value] = (sino, vol)
