@@ -121,7 +121,10 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
     gradient iterations with the quadratic regulariser (lambda 0.01, every
     step optimal), pairs alternating RSVI / SIDDONVI, 16-ellipsoid phantom
     seeded by the request id.  Sinograms y = P(phantom) are made on the
-    device before the clock starts (excluded from latency)."""
+    device before the clock starts (excluded from latency).  The serving
+    loop runs with FP32 operator copies at every rate, and once more with
+    BF16 copies at the highest rate (FP32 state and sums; the error against
+    the sequential FP32 results is reported)."""
     import numpy as np
     import torch
     from tomogpi_amd import lib
@@ -155,8 +158,10 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
     seq_total = sum(service)
     out = {"requests": n_req, "sizes": {str(k): int((edges == k).sum()) for k in (128, 256, 384, 512)},
            "schedule": "recurrent" if schedule == lib.SCHEDULE_RECURRENT else "direct",
-           "sequential_rps": round(n_req / seq_total, 4), "sequential_busy_s": round(seq_total, 3), "rates": []}
-    for rate in rates:
+           "sequential_rps": round(n_req / seq_total, 4), "sequential_busy_s": round(seq_total, 3), "rates": [],
+           "bf16_copies": []}
+    runs = [(rate, lib.F32) for rate in rates] + [(max(rates), lib.BF16)]
+    for rate, storage in runs:
         arrivals = np.cumsum(np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, size=n_req))
         # sequential latency at these arrivals (FIFO, one at a time)
         t_free, lat_seq = 0.0, []
@@ -173,7 +178,7 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
             dt = arrivals[i] - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
-            ids.append(srv.submit(r["desc"], r["y"], outs[i], pair=r["pair"], **gd))
+            ids.append(srv.submit(r["desc"], r["y"], outs[i], pair=r["pair"], storage=storage, **gd))
         for rid in ids:
             srv.wait(rid)
         st = srv.stats()
@@ -182,15 +187,17 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
         err = max(float(torch.linalg.vector_norm(o - r["ref"]) / torch.linalg.vector_norm(r["ref"]))
                   for o, r in zip(outs, reqs))
         lat = np.array(lat_seq)
-        out["rates"].append({
-            "rate_rps": rate, "serving_rps": round(st["throughput_rps"], 4),
+        rec = {
+            "rate_rps": rate, "copies": "bf16" if storage == lib.BF16 else "f32",
+            "serving_rps": round(st["throughput_rps"], 4),
             "speedup_vs_sequential": round(st["throughput_rps"] * seq_total / n_req, 3),
             "p50_s": round(st["latency_p50_s"], 3), "p99_s": round(st["latency_p99_s"], 3),
             "sequential_p50_s": round(float(np.percentile(lat, 50, method="inverted_cdf")), 3),
             "sequential_p99_s": round(float(np.percentile(lat, 99, method="inverted_cdf")), 3),
             "peak_active": st["peak_active"], "peak_gib": round(st["peak_bytes"] / 2**30, 2),
-            "max_rel_l2_vs_sequential": float(f"{err:.3e}")})
-        print(f"[serving] rate {rate}: {out['rates'][-1]}", file=sys.stderr, flush=True)
+            "max_rel_l2_vs_sequential": float(f"{err:.3e}")}
+        out["rates" if storage == lib.F32 else "bf16_copies"].append(rec)
+        print(f"[serving] rate {rate} copies {rec['copies']}: {rec}", file=sys.stderr, flush=True)
         del outs
     for r in reqs:
         r["G"].close()
