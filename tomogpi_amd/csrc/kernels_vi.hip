@@ -128,16 +128,57 @@ __device__ __forceinline__ void vi_columns_tile(const TI* __restrict__ in, uint1
     __builtin_memcpy(out + (size_t)p * qs.view + (size_t)vv * qs.row + 4u * (uint32_t)uu, &w, sizeof(w));
 }
 
-// Raw taps of the second voxel of a pair from the first one's tap-column
-// load: rows d, d+1 of columns u (words c0 = rows 0,1 and c1 = rows 2,3) and
-// u+1 (e0, e1), d in {0, 1, 2} its clamped row less the first's.  When the
-// row step A of the column is in [0, 1.99], consecutive voxels' v differ by
-// less than 2, their floors (and the clamped rows) by 0, 1 or 2.
+// Raw taps of a later voxel of a group from the group's tap-column load:
+// rows d, d+1 of columns u (words c0 = rows 0,1 and c1 = rows 2,3) and u+1
+// (e0, e1), d in {0, 1, 2} its clamped row less the first voxel's.  When the
+// voxels' v differ by less than 2 (A below 1.99 for neighbours, below 0.995
+// two apart), their floors (and the clamped rows) differ by 0, 1 or 2.
 __device__ __forceinline__ void pair_rows(float d, uint32_t c0, uint32_t c1, uint32_t e0, uint32_t e1, uint32_t& b0,
                                           uint32_t& b1) {
     const uint32_t cm = __builtin_amdgcn_alignbyte(c1, c0, 2), em = __builtin_amdgcn_alignbyte(e1, e0, 2);
     b0 = d == 0.0f ? c0 : (d == 1.0f ? cm : c1);
     b1 = d == 0.0f ? e0 : (d == 1.0f ? em : e1);
+}
+
+// Four words of a tap-column load (the non-buffer path's stand-in for the
+// buffer load's vector).
+struct w4 {
+    uint32_t v[4];
+    __device__ __forceinline__ uint32_t operator[](int i) const { return v[i]; }
+};
+
+// The 16 voxels of a column for one view on 16-bit tap columns, GROUP
+// voxels per load: the group's first voxel reads rows 0,1 of the load, the
+// others rows d, d+1 with d their clamped row less the first's (0..2).
+// col4 / col3 load 4 / 3 words at a clamped row; term16 adds one term.
+template <int GROUP, typename C4, typename C3, typename T16, typename CV>
+__device__ __forceinline__ void vi_column16(const C4& col4, const C3& col3, const T16& term16, const CV& cvw,
+                                            __half2 fu2, float fu, float zp0, float cv, float fnv) {
+    constexpr int kFull = GROUP == 1 ? 0 : ZB / GROUP * GROUP;
+#pragma unroll
+    for (int k = 0; k < kFull; k += GROUP) {
+        const float va = fmaf(zp0 + (float)k, cvw.A, cv);
+        const float vfa = floorf(va);
+        const float vca = __builtin_amdgcn_fmed3f(vfa, -2.0f, fnv);
+        const auto w = col4(vca);
+        term16(cvw, fu2, fu, k, va, vfa, w[0], w[2]);
+#pragma unroll
+        for (int j = 1; j < GROUP; ++j) {
+            const float vb = fmaf(zp0 + (float)(k + j), cvw.A, cv);
+            const float vfb = floorf(vb);
+            const float vcb = __builtin_amdgcn_fmed3f(vfb, -2.0f, fnv);
+            uint32_t b0, b1;
+            pair_rows(vcb - vca, w[0], w[1], w[2], w[3], b0, b1);
+            term16(cvw, fu2, fu, k + j, vb, vfb, b0, b1);
+        }
+    }
+#pragma unroll
+    for (int k = kFull; k < ZB; ++k) {
+        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+        const float vf = floorf(v);
+        const auto w = col3(__builtin_amdgcn_fmed3f(vf, -2.0f, fnv));
+        term16(cvw, fu2, fu, k, v, vf, w[0], w[2]);
+    }
 }
 
 // Forward VI on the tap-quad copy (FP32) or the tap-column copy (16-bit).  Bilinear interpolation at (u, v): the
@@ -148,11 +189,14 @@ __device__ __forceinline__ void pair_rows(float d, uint32_t c0, uint32_t c1, uin
 // below 16 MiB): buffer loads at a byte offset computed as one exact FP32
 // fma of small integers; otherwise element indices in integers.
 // VI stores delta * sum; FDK weights each term by W = num / sqrt(fma(v', v',
-// base)), v' = z' A, and stores the sum itself.  16-bit storage with PAIRS
-// (the host has shown A in [0, 1.99] for every column and view) reads the
-// taps of voxels k and k+1 with one tap-column load; without it, one load per
-// voxel.  The taps and the arithmetic of each term are the same either way.
-template <typename TS, typename TV, bool FDK, bool PAIRS = false>
+// base)), v' = z' A, and stores the sum itself.  16-bit storage reads the
+// taps of GROUP consecutive voxels of the column with one tap-column load:
+// GROUP 2 when the host has shown A in [0, 1.99] for every column and view
+// (rows of voxels k, k+1 differ by at most 2), GROUP 3 when A is in
+// [0, 0.995] (rows of k, k+2 differ by at most 2), else 1; the voxels left
+// after the last full group load alone.  The taps and the arithmetic of each
+// term are the same for every GROUP.
+template <typename TS, typename TV, bool FDK, int GROUP = 1>
 __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__ quads, float sino_inv,
                                         TV* __restrict__ vol, float vol_scale, float post, int bx, int by, int bz) {
     const int i = bx * TX + (threadIdx.x % TX);
@@ -224,31 +268,15 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
             const uint32_t soff = (uint32_t)p * (uint32_t)view_bytes;
             if constexpr (kF16 || kBF16) {
                 const __half2 fu2 = __float2half2_rn(fu);
-                if constexpr (PAIRS) {
-#pragma unroll
-                    for (int k = 0; k < ZB; k += 2) {
-                        const float va = fmaf(zp0 + (float)k, cvw.A, cv), vb = fmaf(zp0 + (float)(k + 1), cvw.A, cv);
-                        const float vfa = floorf(va), vfb = floorf(vb);
-                        const float vca = __builtin_amdgcn_fmed3f(vfa, -2.0f, fnv);
-                        const float vcb = __builtin_amdgcn_fmed3f(vfb, -2.0f, fnv);
-                        const auto w = __builtin_amdgcn_raw_buffer_load_b128(
-                            rsrc, (int)(uint32_t)fmaf(vca, row_b, base), (int)soff, 0);
-                        term16(cvw, fu2, fu, k, va, vfa, w[0], w[2]);
-                        uint32_t b0, b1;
-                        pair_rows(vcb - vca, w[0], w[1], w[2], w[3], b0, b1);
-                        term16(cvw, fu2, fu, k + 1, vb, vfb, b0, b1);
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < ZB; ++k) {
-                        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                        const float vf = floorf(v);
-                        const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                        const auto w = __builtin_amdgcn_raw_buffer_load_b96(
-                            rsrc, (int)(uint32_t)fmaf(vc, row_b, base), (int)soff, 0);
-                        term16(cvw, fu2, fu, k, v, vf, w[0], w[2]);
-                    }
-                }
+                const auto col4 = [&](float vc) {
+                    return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(uint32_t)fmaf(vc, row_b, base),
+                                                                 (int)soff, 0);
+                };
+                const auto col3 = [&](float vc) {
+                    return __builtin_amdgcn_raw_buffer_load_b96(rsrc, (int)(uint32_t)fmaf(vc, row_b, base),
+                                                                (int)soff, 0);
+                };
+                vi_column16<GROUP>(col4, col3, term16, cvw, fu2, fu, zp0, cv, fnv);
             } else {
 #pragma unroll
                 for (int k = 0; k < ZB; ++k) {
@@ -269,31 +297,17 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
             const TS* gp = quads + (size_t)p * qs.view + kPer * (uint32_t)(uc + 2);
             if constexpr (kF16 || kBF16) {
                 const __half2 fu2 = __float2half2_rn(fu);
-                if constexpr (PAIRS) {
-#pragma unroll
-                    for (int k = 0; k < ZB; k += 2) {
-                        const float va = fmaf(zp0 + (float)k, cvw.A, cv), vb = fmaf(zp0 + (float)(k + 1), cvw.A, cv);
-                        const float vfa = floorf(va), vfb = floorf(vb);
-                        const float vca = __builtin_amdgcn_fmed3f(vfa, -2.0f, fnv);
-                        const float vcb = __builtin_amdgcn_fmed3f(vfb, -2.0f, fnv);
-                        uint32_t w[4];
-                        __builtin_memcpy(w, gp + (uint64_t)(uint32_t)((int)vca + 2) * qs.row, sizeof(w));
-                        term16(cvw, fu2, fu, k, va, vfa, w[0], w[2]);
-                        uint32_t b0, b1;
-                        pair_rows(vcb - vca, w[0], w[1], w[2], w[3], b0, b1);
-                        term16(cvw, fu2, fu, k + 1, vb, vfb, b0, b1);
-                    }
-                } else {
-#pragma unroll
-                    for (int k = 0; k < ZB; ++k) {
-                        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                        const float vf = floorf(v);
-                        const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                        uint32_t w[3];
-                        __builtin_memcpy(w, gp + (uint64_t)(uint32_t)(vc + 2) * qs.row, sizeof(w));
-                        term16(cvw, fu2, fu, k, v, vf, w[0], w[2]);
-                    }
-                }
+                const auto col4 = [&](float vc) {
+                    uint32_t w[4];
+                    __builtin_memcpy(w, gp + (uint64_t)(uint32_t)((int)vc + 2) * qs.row, sizeof(w));
+                    return w4{w[0], w[1], w[2], w[3]};
+                };
+                const auto col3 = [&](float vc) {
+                    uint32_t w[3];
+                    __builtin_memcpy(w, gp + (uint64_t)(uint32_t)((int)vc + 2) * qs.row, sizeof(w));
+                    return w4{w[0], w[1], w[2], 0u};
+                };
+                vi_column16<GROUP>(col4, col3, term16, cvw, fu2, fu, zp0, cv, fnv);
             } else {
 #pragma unroll
                 for (int k = 0; k < ZB; ++k) {
@@ -326,10 +340,10 @@ __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in
         vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-template <typename TS, typename TV, bool FDK, bool PAIRS>
+template <typename TS, typename TV, bool FDK, int GROUP>
 __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
                                                   TV* __restrict__ vol, float vol_scale) {
-    vi_tile<TS, TV, FDK, PAIRS>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
+    vi_tile<TS, TV, FDK, GROUP>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // Shared launches (FP32 sinograms and volumes; kItemBF16: BF16 tap columns,
@@ -361,10 +375,10 @@ __global__ __launch_bounds__(TX* TY, 7) void vi_batch_kernel(const BatchTable* _
         if (!(it.flags & kItemBF16))
             vi_tile<float, float, false>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx, by, bz);
         else if (it.flags & kItemPairs)
-            vi_tile<__hip_bfloat16, float, false, true>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out, 1.0f,
+            vi_tile<__hip_bfloat16, float, false, 2>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out, 1.0f,
                                                         it.a, (int)bx, by, bz);
         else
-            vi_tile<__hip_bfloat16, float, false, false>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out,
+            vi_tile<__hip_bfloat16, float, false, 1>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out,
                                                          1.0f, it.a, (int)bx, by, bz);
     }
 }
@@ -416,32 +430,40 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 
 }  // namespace
 
-// True when every column-view's row step A = delta FDD/dv / w is in
-// [0, 1.98] (1.99 in the kernel, less a margin far above A's FP32 rounding).
-// w = delta (cos x' + sin y') + FOD is linear in (x', y'), so its minimum over
-// the voxel centres is at a corner of their rectangle.
-bool vi_rows_pairable(const tomo_geom* g) {
+// Voxels per 16-bit tap-column load the geometry allows: 3 when every
+// column-view's row step A = delta FDD/dv / w is in [0, 0.99], 2 when it is
+// in [0, 1.98], else 1 (the kernel's bounds 0.995 and 1.99 less a margin far
+// above A's FP32 rounding).  w = delta (cos x' + sin y') + FOD is linear in
+// (x', y'), so its minimum over the voxel centres is at a corner of their
+// rectangle.
+int vi_row_group(const tomo_geom* g) {
     const DevGeom& d = g->dev;
     const double xs[2] = {-(double)d.cx, (double)(d.nx - 1) - (double)d.cx};
     const double ys[2] = {-(double)d.cy, (double)(d.ny - 1) - (double)d.cy};
+    double amax = 0.0;
     for (int p = 0; p < d.nphi; ++p) {
         const double* c = &g->view[(size_t)p * kViewStride];
         double wmin = 1.0e300;
         for (double x : xs)
             for (double y : ys) wmin = std::min(wmin, std::fma(c[2], x, std::fma(c[3], y, d.fod)));
-        if (!(wmin > 0.0) || !(c[6] >= 0.0) || !(c[6] / wmin <= 1.98)) return false;
+        if (!(wmin > 0.0) || !(c[6] >= 0.0)) return 1;
+        amax = std::max(amax, c[6] / wmin);
     }
-    return true;
+    return amax <= 0.99 ? 3 : (amax <= 1.98 ? 2 : 1);
 }
 
 template <typename TS, typename TV, bool FDK>
-static void launch_vi_kernel(const DevGeom& d, dim3 grid, bool pairs, const void* quads, float inv, void* vol,
+static void launch_vi_kernel(const DevGeom& d, dim3 grid, int group, const void* quads, float inv, void* vol,
                              float vol_scale, hipStream_t s) {
-    if (sizeof(TS) == 2 && pairs)
-        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, sizeof(TS) == 2>), grid, dim3(TX * TY), 0, s, d,
-                           (const TS*)quads, inv, (TV*)vol, vol_scale);
+    constexpr bool k16 = sizeof(TS) == 2;
+    if (k16 && group == 3)
+        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, k16 ? 3 : 1>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads,
+                           inv, (TV*)vol, vol_scale);
+    else if (k16 && group == 2)
+        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, k16 ? 2 : 1>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads,
+                           inv, (TV*)vol, vol_scale);
     else
-        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, false>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
+        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, 1>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
                            (TV*)vol, vol_scale);
 }
 
@@ -455,7 +477,7 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
     const dim3 qgrid(ceil_div(d.nu + 4, 64), ceil_div(d.nv + 3, 4), d.nphi);
     const dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
     const float inv = 1.0f / sino.scale;
-    const bool pairs = sino.dtype != TOMO_F32 && vi_rows_pairable(g);
+    const int group = sino.dtype != TOMO_F32 ? vi_row_group(g) : 1;
     hipError_t le = hipSuccess;
     TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
         hipLaunchKernelGGL((vi_quads_kernel<TS>), qgrid, dim3(256), 0, s, (const TS*)sino.p, (TS*)quads, d.nu, d.nv);
@@ -463,9 +485,9 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
         if (le == hipSuccess) {
             TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
                 if (fdk)
-                    launch_vi_kernel<TS, TV, true>(d, grid, pairs, quads, inv, vol.p, vol.scale, s);
+                    launch_vi_kernel<TS, TV, true>(d, grid, group, quads, inv, vol.p, vol.scale, s);
                 else
-                    launch_vi_kernel<TS, TV, false>(d, grid, pairs, quads, inv, vol.p, vol.scale, s);
+                    launch_vi_kernel<TS, TV, false>(d, grid, group, quads, inv, vol.p, vol.scale, s);
             });
             le = hipGetLastError();
         }

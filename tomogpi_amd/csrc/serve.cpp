@@ -170,7 +170,7 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo, size_t 
 // request is then marked failed by the caller).
 int admit(tomo_server* srv, Request& q) {
     q.bf16 = q.req.storage == TOMO_BF16;
-    q.vi_pairs = q.bf16 && vi_rows_pairable(&q.geom);
+    q.vi_pairs = q.bf16 && vi_row_group(&q.geom) >= 2;
     const Layout L = layout_of(q.geom, q.req.gd, q.req.algorithm, q.bf16 ? 2 : 4);
     TOMO_HIP(hipMallocFromPoolAsync(&q.block, L.total, srv->pool, srv->stream));
     char* b = (char*)q.block;

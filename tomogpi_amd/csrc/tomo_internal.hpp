@@ -74,9 +74,9 @@ namespace tomo {
 __host__ __device__ inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
 __host__ __device__ inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
 
-// True when VI on 16-bit tap columns may read one load per voxel pair for
-// every column and view of g (row steps in [0, 1.98]).
-bool vi_rows_pairable(const tomo_geom* g);
+// Voxels per 16-bit tap-column load VI may read for every column and view
+// of g: 3 (row steps in [0, 0.99]), 2 (in [0, 1.98]) or 1.
+int vi_row_group(const tomo_geom* g);
 
 // Elements of the VI tap-quad copy of a sinogram: nphi views of (nv + 3)
 // rows of (nu + 4) pairs; its fill kernel's grid is (ceil((nu+4)/64),
