@@ -234,6 +234,17 @@ int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* para
                 const float* sinogram, float* volume, tomo_gd_iter_stat* stats,
                 void* stream);
 
+/* Weighted gradient (Iter3D::doWeightedGradient, C/Iter3D_Algo_Gradient.cu:
+ * 227-371; the reference's dispatcher, C/Iter3D.cu:78-90, never selects it):
+ * the loop of tomo_gd_run with the residual multiplied by the weight
+ * sinogram (same layout as the sinogram; host memory with TOMO_FLAG_HOST_PTR)
+ * before the backprojection.  residual_sq and the step's |P dJ|^2 stay
+ * unweighted, as in the reference, whose alpha_min starts at -999999999999
+ * here (pass it in alpha_min_init).  Synchronous.                          */
+int tomo_wgd_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_gd_params* params,
+                 const float* sinogram, const float* weight, float* volume,
+                 tomo_gd_iter_stat* stats, void* stream);
+
 /* Conjugate gradient on the same criterion and step rule (the configIteration
  * key "gradient Conjugate"; reference: Iter3D::doConjugateGradient,
  * C/Iter3D_Algo_Gradient.cu:375-559, whose direction starts at 0 and whose
@@ -280,7 +291,8 @@ int tomo_sirt_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_sirt_params* 
  * between rounds.  A request's arithmetic
  * does not depend on what shares its launches, so its result is the same
  * alone and in any mix.  A request is a simple-gradient reconstruction or a
- * SIRT reconstruction (tomo_sirt_run).  Storage is FP32.                     */
+ * SIRT reconstruction (tomo_sirt_run).  Its state is FP32; the copies its
+ * operators read are FP32 or BF16 (tomo_serve_request.storage).              */
 typedef struct tomo_server tomo_server;
 
 #define TOMO_SCHEDULE_DIRECT     0 /* the operations of tomo_gd_run          */

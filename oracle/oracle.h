@@ -119,6 +119,13 @@ typedef struct og_gd_stat {
 int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino,
               float* vol, og_gd_stat* stats);
 
+/* Weighted gradient (C/Iter3D_Algo_Gradient.cu:227-371): the simple loop with
+ * the residual multiplied by the weight sinogram before the backprojection;
+ * J_MC and the step's |P dJ|^2 stay unweighted, as the reference writes
+ * them.  The reference starts alpha_min at -999999999999 here.              */
+int og_wgd_run(const og_geom* g, const og_gd_params* p, const float* sino, const float* weight,
+               float* vol, og_gd_stat* stats);
+
 /* Conjugate gradient on the criterion of og_gd_run (defined by this build:
  * the reference's doConjugateGradient, C/Iter3D_Algo_Gradient.cu:375-559,
  * starts its direction at 0 and divides by its zero norm at iteration 2):

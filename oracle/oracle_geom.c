@@ -258,7 +258,12 @@ static void backproject(const og_geom* g, const og_gd_params* p, const float* si
  *   else alpha = alpha_min;
  *   x += (float)alpha * dJ, clamped at 0 if positivity (C/Projector.cu:109-124,
  *   C/Image3D_CPU.cu:151-205).                                             */
-int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino, float* vol, og_gd_stat* stats) {
+/* The simple gradient, and with a weight sinogram the weighted one
+ * (C/Iter3D_Algo_Gradient.cu:227-371): r = y - P x, J_MC = sum r^2, then
+ * r = r * W before the backprojection; the step keeps the unweighted
+ * |P dJ|^2 of the simple loop, as the reference does.                      */
+static int gd_loop(const og_geom* g, const og_gd_params* p, const float* sino, const float* weight, float* vol,
+                   og_gd_stat* stats) {
     const size_t nv = (size_t)g->nx * g->ny * g->nz, ns = (size_t)g->nu * g->nv * g->nphi;
     float* est = (float*)malloc(sizeof(float) * ns);
     float* dj = (float*)malloc(sizeof(float) * nv);
@@ -271,6 +276,8 @@ int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
         project(g, p, vol, est);
         for (size_t i = 0; i < ns; ++i) est[i] = sino[i] - est[i];
         st.residual_sq = sq_norm(est, ns);
+        if (weight)
+            for (size_t i = 0; i < ns; ++i) est[i] = est[i] * weight[i];
         backproject(g, p, est, dj);
         for (size_t i = 0; i < nv; ++i) dj[i] = dj[i] * -2.0f;
         double norm_reg = 0.0;
@@ -305,6 +312,15 @@ int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
     }
     free(est); free(dj); free(t1); free(t2);
     return 0;
+}
+
+int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino, float* vol, og_gd_stat* stats) {
+    return gd_loop(g, p, sino, NULL, vol, stats);
+}
+
+int og_wgd_run(const og_geom* g, const og_gd_params* p, const float* sino, const float* weight, float* vol,
+               og_gd_stat* stats) {
+    return gd_loop(g, p, sino, weight, vol, stats);
 }
 
 static double dot_f32(const float* a, const float* b, size_t n) {

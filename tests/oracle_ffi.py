@@ -157,6 +157,8 @@ class Oracle:
             lib.og_ellipsoid_table.argtypes = [C.c_int] * 4 + [C.c_uint64, C.c_double, C.c_double, dp, dp]
             lib.og_gd_run.argtypes = [G, C.POINTER(OgGdParams), fp, fp, C.POINTER(OgGdStat)]
             lib.og_cg_run.argtypes = [G, C.POINTER(OgGdParams), fp, fp, C.POINTER(OgGdStat)]
+            lib.og_wgd_run.argtypes = [G, C.POINTER(OgGdParams), fp, fp, fp, C.POINTER(OgGdStat)]
+            lib.og_wgd_run.restype = C.c_int
             lib.og_cg_run.restype = C.c_int
             lib.og_sirt_weights.argtypes = [G, C.POINTER(OgSirtParams), fp, fp]
             lib.og_sirt_weights.restype = C.c_int
@@ -284,13 +286,17 @@ class Oracle:
         return out
 
     def gd(self, sino, vol0=None, pair=0, cube=False, compat=False, iters=3, kopt=10, lam=0.0,
-           positivity=0, alpha_min_init=-0.0347711, matched=False):
+           positivity=0, alpha_min_init=-0.0347711, matched=False, weight=None):
         s = np.ascontiguousarray(sino, dtype=np.float32)
         v = self._vol(np.float32) if vol0 is None else np.array(vol0, dtype=np.float32, order="C")
         p = OgGdParams(pair=pair, cube=int(cube), compat=int(compat), iters=iters, kopt=kopt, lam=lam,
                        positivity=positivity, alpha_min_init=alpha_min_init, matched=int(matched))
         stats = (OgGdStat * max(iters, 1))()
-        rc = self.lib().og_gd_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(v), stats)
+        if weight is None:
+            rc = self.lib().og_gd_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(v), stats)
+        else:
+            w = np.ascontiguousarray(weight, dtype=np.float32)
+            rc = self.lib().og_wgd_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(w), _f32p(v), stats)
         assert rc == 0
         return v, [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj,
                         alpha=x.alpha) for x in stats[:iters]]

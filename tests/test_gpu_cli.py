@@ -69,6 +69,26 @@ def test_cli_proj_back(tmp_path, storage, cube):
     assert np.array_equal(back, o.vi(ref))
 
 
+def test_cli_double_precision_files(tmp_path):
+    """--precision=double reads and writes FP64 .v/.s files (the reference's
+    double containers write sizeof(double) * count) and runs the FP32
+    operators: the FP64 outputs hold the FP32 results exactly."""
+    vol = shepp_logan(N).astype(np.float64) + 1e-9  # the extra digits round away in FP32
+    vpath = tmp_path / "phantom64.v"
+    vol.tofile(vpath)
+    write_workdir(tmp_path, real=str(vpath))
+    run(tmp_path, "--op=proj", "--storage=mem_GPU", "--precision=double")
+    o = Oracle(geom())
+    ref = o.rs(vol.astype(np.float32))
+    assert os.path.getsize(tmp_path / "proj.s") == ref.size * 8
+    got = np.fromfile(tmp_path / "proj.s", dtype=np.float64).reshape(ref.shape)
+    assert np.array_equal(got, ref.astype(np.float64))
+    ref.astype(np.float64).tofile(tmp_path / f"Projections_{N}" / "P_ER_GPU.s")
+    run(tmp_path, "--op=back", "--storage=mem_GPU", "--precision=double")
+    back = np.fromfile(tmp_path / "back.v", dtype=np.float64).reshape(vol.shape)
+    assert np.array_equal(back, o.vi(ref).astype(np.float64))
+
+
 def test_cli_back_fdk(tmp_path):
     o = Oracle(geom())
     sino = o.rs(shepp_logan(N))
@@ -115,6 +135,23 @@ def test_cli_conjugate_gradient(tmp_path):
     final = np.fromfile(res[0] / "final_Volume.v", dtype=np.float32).reshape(o.geom.vol_shape)
     ref, _ = o.cg(sino, pair=0, cube=True, iters=3, lam=0.01)
     assert np.linalg.norm(final - ref) / np.linalg.norm(ref) <= 1e-6
+
+
+def test_cli_weighted_gradient(tmp_path):
+    """configIteration `gradient Weighted` reads <workdir>/weightedSinogram.s
+    and runs tomo_wgd_run (alpha_min from -999999999999, as the reference's
+    doWeightedGradient); the result matches the oracle's weighted loop."""
+    wd = write_workdir(tmp_path, iters=3, kopt=10, lam=0.01, gradient="Weighted")
+    o = Oracle(geom())
+    y = o.rs(ellipsoids(N, count=4, seed=9, amin=3.0, amax=8.0), cube=True)
+    w = (0.5 + np.random.default_rng(3).random(y.shape)).astype(np.float32)
+    y.tofile(wd / "P_ER_GPU.s")
+    w.tofile(wd / "weightedSinogram.s")
+    run(tmp_path)
+    res = sorted((wd / "TomoIter" / "Results").iterdir())[-1]
+    got = np.fromfile(res / "final_Volume.v", dtype=np.float32).reshape(o.geom.vol_shape)
+    ref, _ = o.gd(y, pair=0, cube=True, iters=3, kopt=10, lam=0.01, weight=w, alpha_min_init=-999999999999.0)
+    assert np.linalg.norm(got - ref) / np.linalg.norm(ref) <= 1e-6
 
 
 def test_cli_missing_sinogram_is_an_error(tmp_path):

@@ -9,8 +9,11 @@ at rel-L2 <= 1e-6, the step sizes at rel 1e-9.
 Mixed forms: FP16 (scaled to put max|x| at 2^14) and BF16 storage of inputs
 and outputs, FP32 arithmetic, FP64 geometry.  Error is stated per result as
 relative L2 against the FP64 oracle on the FP32 input; the bound asserted is
-the north-star 1e-3 for FP16 and 4e-3 for BF16 (BF16 keeps 8 significant
+the mixed form's 1e-3 for FP16 and 4e-3 for BF16 (BF16 keeps 8 significant
 bits: rounding a stored result alone costs about 1e-3 relative).
+
+The weighted gradient (tomo_wgd_run) is compared with the oracle's weighted
+loop under the same loop tolerance.
 """
 import numpy as np
 import pytest
@@ -50,6 +53,27 @@ def test_gd_loop_matches_oracle(small, pair, flags, lam, kopt, positivity):
             assert a[k] == pytest.approx(b[k], rel=1e-9, abs=1e-12), (k, a, b)
     assert rel_l2(vol.cpu().numpy(), ref_vol) <= 1e-6
     assert stats[-1]["residual_sq"] < stats[0]["residual_sq"]
+
+
+@pytest.mark.parametrize("pair", [lib.PAIR_RSVI, lib.PAIR_SIDDONVI])
+def test_weighted_gd_loop_matches_oracle(small, pair):
+    """tomo_wgd_run (the reference's doWeightedGradient) against the oracle's
+    weighted loop: the same operators, FP64 norms summed in another order."""
+    g, o, G = small
+    rng = np.random.default_rng(21)
+    sino = o.rs(ellipsoids(g.nx, count=6, seed=5, amin=2.0, amax=8.0))
+    w = (0.25 + rng.random(g.sino_shape)).astype(np.float32)
+    kw = dict(iters=3, kopt=10, lam=0.01, alpha_min_init=-999999999999.0)
+    vol = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+    st = G.gd_run(torch.from_numpy(sino).cuda(), vol, pair=pair, weight=torch.from_numpy(w).cuda(), **kw)
+    ref_vol, ref_st = o.gd(sino, pair=0 if pair == lib.PAIR_RSVI else 1, weight=w, **kw)
+    assert rel_l2(vol.cpu().numpy(), ref_vol) <= 1e-6
+    for a, b in zip(st, ref_st):
+        assert a["alpha"] == pytest.approx(b["alpha"], rel=1e-6)
+    # host pointers give the same volume
+    vh = np.zeros(g.vol_shape, dtype=np.float32)
+    G.gd_run(sino, vh, pair=pair, weight=w, **kw)
+    assert np.array_equal(vh, vol.cpu().numpy())
 
 
 def test_gd_loop_host_pointers(small):

@@ -34,6 +34,12 @@ them retypes the oracle's own formula:
 * SIRT    - first iteration composed from the pinned operators (weights,
             relaxation, sign); exact homogeneity; convergence on consistent
             data.
+* FP16 VI - the binary16 rounding against IEEE (numpy); constant-sinogram
+            closed form (exact where every half operation is exact) and an
+            FP64 error bound.
+* weighted gradient - W = 1 is the simple gradient; W = 2 doubles the first
+            iterate; a binary view mask acts on the residual before the
+            backprojection.
 """
 import numpy as np
 import pytest
@@ -439,6 +445,34 @@ def test_loop_is_exact_line_search_with_matched_pair():
         predicted = st["norm_dj"] ** 2 / (2.0 * st["norm_dj_proj"])
         assert Js[k] - Js[k + 1] == pytest.approx(predicted, rel=2e-3)
     assert all(a > b for a, b in zip(Js, Js[1:]))
+
+
+def test_weighted_gradient_pins():
+    """og_wgd_run (doWeightedGradient, C/Iter3D_Algo_Gradient.cu:227-371):
+    W = 1 reproduces the simple gradient bit for bit (a product by 1 is
+    exact); W = 2 from x0 = 0 doubles the first iterate exactly (every
+    operator, norm and the step scale by powers of two, alpha is unchanged);
+    a binary view mask W acts on the residual before the backprojection,
+    so y and W*y give the same first iterate, unlike the unweighted loop."""
+    g = scaled_geometry(16, 8)
+    o = Oracle(g)
+    y = o.rs(ellipsoids(16, count=4, seed=2, amin=2.0, amax=6.0))
+    kw = dict(pair=0, iters=3, kopt=10, lam=0.02, alpha_min_init=-999999999999.0)
+    x_ref, st_ref = o.gd(y, **kw)
+    x_w1, st_w1 = o.gd(y, weight=np.ones_like(y), **kw)
+    assert np.array_equal(x_w1, x_ref)
+    for a, b in zip(st_w1, st_ref):  # FP64 norms: OpenMP combines partial sums in any order
+        assert all(a[k] == pytest.approx(b[k], rel=1e-12) for k in a)
+    one = dict(kw, iters=1)
+    x1, st1 = o.gd(y, **one)
+    x2, st2 = o.gd(y, weight=np.full_like(y, 2.0), **one)
+    assert np.array_equal(x2, 2.0 * x1) and st2[0]["alpha"] == pytest.approx(st1[0]["alpha"], rel=1e-12)
+    mask = np.zeros_like(y)
+    mask[::2] = 1.0  # even views only
+    xa, _ = o.gd(y, weight=mask, **one)
+    xb, _ = o.gd(y * mask, weight=mask, **one)
+    assert np.array_equal(xa, xb)
+    assert not np.array_equal(xa, x1)
 
 
 def test_cg_exact_line_search_and_beats_gradient_with_matched_pair():

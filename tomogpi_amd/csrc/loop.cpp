@@ -35,8 +35,8 @@ int project(const tomo_geom* g, const tomo_gd_params& pr, const float* vol, floa
 
 }  // namespace
 
-int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const float* sino, float* vol,
-           tomo_gd_iter_stat* stats, hipStream_t s) {
+int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const float* sino, const float* weight,
+           float* vol, tomo_gd_iter_stat* stats, hipStream_t s) {
     const size_t nv = g->nvox(), ns = g->nsino();
     DevBuf est, dj, t1, t2;
     TOMO_HIP(hipMalloc(&est.p, ns * sizeof(float)));
@@ -54,6 +54,7 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         if ((rc = project(g, pr, vol, e, s))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
+        if (weight && (rc = launch_mul(e, weight, ns, s))) return rc;  // weighted form: r = W (y - P x)
         if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{d, TOMO_F32, 1.0f}, s))) return rc;
         if ((rc = launch_scale(d, -2.0f, nv, s))) return rc;
         double norm_reg = 0.0;
@@ -256,5 +257,25 @@ extern "C" int tomo_cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_para
 
 extern "C" int tomo_gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram,
                            float* volume, tomo_gd_iter_stat* stats, void* stream) {
-    return run_loop(ctx, g, params, sinogram, volume, stats, stream, tomo::gd_run);
+    return run_loop(ctx, g, params, sinogram, volume, stats, stream,
+                    [](tomo_ctx* c, const tomo_geom* gg, const tomo_gd_params& pr, const float* y, float* x,
+                       tomo_gd_iter_stat* st, hipStream_t s) { return tomo::gd_run(c, gg, pr, y, nullptr, x, st, s); });
+}
+
+extern "C" int tomo_wgd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, const float* sinogram,
+                            const float* weight, float* volume, tomo_gd_iter_stat* stats, void* stream) {
+    using namespace tomo;
+    if (!weight) return fail(TOMO_ERR_INVALID_ARG, "null weight sinogram");
+    DevBuf dw;
+    const float* w = weight;
+    if (ctx && g && params && (params->flags & TOMO_FLAG_HOST_PTR)) {
+        TOMO_HIP(hipSetDevice(ctx->device));
+        const size_t ns = g->nsino();
+        TOMO_HIP(hipMalloc(&dw.p, ns * sizeof(float)));
+        TOMO_HIP(hipMemcpy(dw.p, weight, ns * sizeof(float), hipMemcpyHostToDevice));
+        w = (const float*)dw.p;
+    }
+    return run_loop(ctx, g, params, sinogram, volume, stats, stream,
+                    [w](tomo_ctx* c, const tomo_geom* gg, const tomo_gd_params& pr, const float* y, float* x,
+                        tomo_gd_iter_stat* st, hipStream_t s) { return gd_run(c, gg, pr, y, w, x, st, s); });
 }

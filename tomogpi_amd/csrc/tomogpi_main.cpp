@@ -13,12 +13,20 @@
 //   mem_CPU, SiddonVI, CPU/C   reference C-path Siddon (compat form, 256^3) + VI
 //   mem_CPU, SiddonVI, other   clean Siddon + VI
 // --precision=half stores volumes and sinograms in FP16 (scaled per array)
-// with FP32 accumulation; double runs the FP32 operators.
+// with FP32 accumulation; its files stay FP32, as the reference's half
+// containers convert on load and save (C/Image3D_CPU.cu:659-735,
+// C/Sinogram3D_CPU.cu:556-606).  --precision=double runs the FP32 operators
+// and reads and writes FP64 files, the layout of the reference's double
+// containers (sizeof(T) * count, C/Image3D.cu:140-233, C/Sinogram3D.cu:
+// 138-184).
 //   --op=proj   loads configCT:realVolumeName, writes proj.s
 //   --op=back   loads <workdir>/<configTiff:tiffName>.s, writes back.v
 //   --op=back_fdk  same input, FDK-weighted backprojection, writes back_fdk.v
-//   (no --op)   simple-gradient (configIteration gradient Simple) or conjugate
-//               gradient (gradient Conjugate) reconstruction from the same sinogram, starting
+//   (no --op)   simple-gradient (configIteration gradient Simple), conjugate
+//               gradient (gradient Conjugate) or weighted-gradient (gradient
+//               Weighted, weights in <workdir>/weightedSinogram.s; a spelling
+//               the reference's dispatcher, C/Iter3D.cu:78-90, does not
+//               offer) reconstruction from the same sinogram, starting
 //               from configCT:initVolumeName (or zero), writes
 //               <workdir>/TomoIter/Results/Result_<date>/final_Volume.v
 // Every failure prints a message and returns a nonzero status.
@@ -199,23 +207,39 @@ int main(int argc, char** argv) {
         return rc;
     };
 
+    // Volume / sinogram files: FP32, or FP64 with --precision=double.
+    const int fdt = o.precision == "double" ? TOMO_F64 : TOMO_F32;
+    auto read_file = [&](const std::string& path, std::vector<float>& dst, size_t n) -> int {
+        if (fdt == TOMO_F32) return tomo_file_read(path.c_str(), dst.data(), TOMO_F32, n);
+        std::vector<double> tmp(n);
+        const int rc = tomo_file_read(path.c_str(), tmp.data(), TOMO_F64, n);
+        if (rc == TOMO_OK)
+            for (size_t i = 0; i < n; ++i) dst[i] = (float)tmp[i];
+        return rc;
+    };
+    auto write_file = [&](const std::string& path, const std::vector<float>& src, size_t n) -> int {
+        if (fdt == TOMO_F32) return tomo_file_write(path.c_str(), src.data(), TOMO_F32, n);
+        const std::vector<double> tmp(src.begin(), src.begin() + (ptrdiff_t)n);
+        return tomo_file_write(path.c_str(), tmp.data(), TOMO_F64, n);
+    };
+
     if (o.have_op && o.op == "proj") {
         const std::string real = cfg(work, "configCT", "realVolumeName", "none");
-        if (real != "none" && o.load) CHECK(tomo_file_read(real.c_str(), vol.data(), TOMO_F32, nvox), "realVolumeName");
+        if (real != "none" && o.load) CHECK(read_file(real, vol, nvox), "realVolumeName");
         t0 = clk::now();
         CHECK(run_op(true, vol, sino), "projection");
         t1 = clk::now();
-        if (o.save) CHECK(tomo_file_write("proj.s", sino.data(), TOMO_F32, nsino), "proj.s");
+        if (o.save) CHECK(write_file("proj.s", sino, nsino), "proj.s");
     } else if (o.have_op && (o.op == "back" || o.op == "back_fdk")) {
         // back_fdk: FDK-weighted VI (C/Iter3D_main.cu:296-316), written to back_fdk.v
         const bool fdk = o.op == "back_fdk";
         bflags = fdk ? TOMO_FLAG_FDK : 0;
-        if (o.load) CHECK(tomo_file_read(sino_path.c_str(), sino.data(), TOMO_F32, nsino), sino_path);
+        if (o.load) CHECK(read_file(sino_path, sino, nsino), sino_path);
         t0 = clk::now();
         CHECK(run_op(false, sino, vol), "backprojection");
         t1 = clk::now();
         const char* name = fdk ? "back_fdk.v" : "back.v";
-        if (o.save) CHECK(tomo_file_write(name, vol.data(), TOMO_F32, nvox), name);
+        if (o.save) CHECK(write_file(name, vol, nvox), name);
     } else if (o.have_op) {
         std::fprintf(stderr, "tomogpi: --op=%s is not supported (proj, back, back_fdk or none)\n", o.op.c_str());
         return TOMO_ERR_UNSUPPORTED;
@@ -230,17 +254,17 @@ int main(int argc, char** argv) {
         const std::string grad = cfg(work, "configIteration", "gradient", "Simple");
         const std::string pen = cfg(work, "configIteration", "penaltyFunction", "GGaussian");
         tomo::config_double(ci, "beta", &beta);
-        if (grad != "Simple" && grad != "Conjugate") {
-            std::fprintf(stderr, "tomogpi: gradient %s is not supported (Simple, Conjugate)\n", grad.c_str());
+        if (grad != "Simple" && grad != "Conjugate" && grad != "Weighted") {
+            std::fprintf(stderr, "tomogpi: gradient %s is not supported (Simple, Conjugate, Weighted)\n", grad.c_str());
             return TOMO_ERR_UNSUPPORTED;
         }
         if (lambda != 0.0 && (pen != "GGaussian" || beta != 2.0)) {
             std::fprintf(stderr, "tomogpi: penalty %s beta %g is not supported (GGaussian, beta 2)\n", pen.c_str(), beta);
             return TOMO_ERR_UNSUPPORTED;
         }
-        if (o.load) CHECK(tomo_file_read(sino_path.c_str(), sino.data(), TOMO_F32, nsino), sino_path);
+        if (o.load) CHECK(read_file(sino_path, sino, nsino), sino_path);
         const std::string init = cfg(work, "configCT", "initVolumeName", "none");
-        if (init != "none" && o.load) CHECK(tomo_file_read(init.c_str(), vol.data(), TOMO_F32, nvox), "initVolumeName");
+        if (init != "none" && o.load) CHECK(read_file(init, vol, nvox), "initVolumeName");
         // results directory <workdir>/TomoIter/Results/Result_<date>/ with the config files copied
         char stamp[64];
         const time_t now = time(nullptr);
@@ -262,10 +286,21 @@ int main(int argc, char** argv) {
         gp.lambda = lambda;
         gp.positivity = pos;
         gp.alpha_min_init = -0.0347711;
+        std::vector<float> weight;
+        if (grad == "Weighted") {
+            // doWeightedGradient: weights from <workdir>/weightedSinogram.s,
+            // alpha_min from -999999999999 (C/Iter3D_Algo_Gradient.cu:237-250)
+            weight.assign(nsino, 0.0f);
+            CHECK(read_file(work + "weightedSinogram.s", weight, nsino), "weightedSinogram.s");
+            gp.alpha_min_init = -999999999999.0;
+        }
         std::vector<tomo_gd_iter_stat> st(iters > 0 ? iters : 1);
         t0 = clk::now();
         if (grad == "Conjugate")
             CHECK(tomo_cg_run(ctx, geom, &gp, sino.data(), vol.data(), st.data(), nullptr), "conjugate gradient");
+        else if (grad == "Weighted")
+            CHECK(tomo_wgd_run(ctx, geom, &gp, sino.data(), weight.data(), vol.data(), st.data(), nullptr),
+                  "weighted gradient");
         else
             CHECK(tomo_gd_run(ctx, geom, &gp, sino.data(), vol.data(), st.data(), nullptr), "gradient descent");
         t1 = clk::now();
@@ -279,7 +314,7 @@ int main(int argc, char** argv) {
                              st[k].norm_dj_proj, st[k].alpha);
         }
         if (log) std::fclose(log);
-        if (o.save) CHECK(tomo_file_write((res + "final_Volume.v").c_str(), vol.data(), TOMO_F32, nvox), "final_Volume.v");
+        if (o.save) CHECK(write_file(res + "final_Volume.v", vol, nvox), "final_Volume.v");
         std::printf("results in %s\n", res.c_str());
     }
     const long ms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();

@@ -35,7 +35,7 @@ EXPORTS = [
     "tomo_synchronize", "tomo_geom_create", "tomo_geom_desc_from_config", "tomo_geom_destroy",
     "tomo_geom_get_info", "tomo_geom_get_phi", "tomo_project", "tomo_backproject", "tomo_project_adjoint",
     "tomo_backproject_adjoint", "tomo_project_work", "tomo_convert", "tomo_diff", "tomo_scale", "tomo_axpy",
-    "tomo_sq_norm", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_cg_run", "tomo_sirt_run", "tomo_phantom_shepp_logan",
+    "tomo_sq_norm", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_wgd_run", "tomo_cg_run", "tomo_sirt_run", "tomo_phantom_shepp_logan",
     "tomo_phantom_ellipsoids", "tomo_file_read", "tomo_file_write", "tomo_serve_create", "tomo_serve_submit",
     "tomo_serve_poll", "tomo_serve_wait", "tomo_serve_iter_stats", "tomo_serve_get_stats", "tomo_serve_destroy",
 ]
@@ -148,6 +148,7 @@ def load() -> C.CDLL:
         "tomo_laplacian": ([vp, vp, vp, vp, vp], i),
         "tomo_gd_run": ([vp, vp, C.POINTER(GdParams), vp, vp, C.POINTER(GdIterStat), vp], i),
         "tomo_cg_run": ([vp, vp, C.POINTER(GdParams), vp, vp, C.POINTER(GdIterStat), vp], i),
+        "tomo_wgd_run": ([vp, vp, C.POINTER(GdParams), vp, vp, vp, C.POINTER(GdIterStat), vp], i),
         "tomo_sirt_run": ([vp, vp, C.POINTER(SirtParams), vp, vp, C.POINTER(d), vp], i),
         "tomo_phantom_shepp_logan": ([vp, vp, vp, vp], i),
         "tomo_phantom_ellipsoids": ([vp, vp, i, C.c_uint64, d, d, vp, vp], i),
@@ -364,18 +365,28 @@ class Geometry:
         return vol
 
     def gd_run(self, sino, vol, pair=PAIR_RSVI, flags=0, iters=10, kopt=10, lam=0.0, positivity=0,
-               alpha_min_init=-0.0347711, stream=None, conjugate=False):
-        """Simple-gradient loop (tomo_gd_run), or conjugate gradient (tomo_cg_run)."""
+               alpha_min_init=-0.0347711, stream=None, conjugate=False, weight=None):
+        """Simple-gradient loop (tomo_gd_run), conjugate gradient (tomo_cg_run),
+        or with a weight sinogram the weighted gradient (tomo_wgd_run)."""
         p = GdParams(pair=pair, flags=flags, gradient_iteration_nb=iters, optimal_step_iteration_nb=kopt, lam=lam,
                      positivity=positivity, alpha_min_init=alpha_min_init)
-        fn = self.lib.tomo_cg_run if conjugate else self.lib.tomo_gd_run
         stats = (GdIterStat * max(iters, 1))()
         ps, hs = _ptr(sino)
         pv, hv = _ptr(vol)
         if hs:
             p.flags |= FLAG_HOST_PTR
-        check(fn(self.ctx.h, self.h, C.byref(p), C.c_void_p(ps), C.c_void_p(pv), stats,
-                 None if hs else C.c_void_p(_stream(stream))))
+        st = None if hs else C.c_void_p(_stream(stream))
+        if weight is not None:
+            if conjugate:
+                raise ValueError("the weighted form is a simple gradient")
+            pw, hw = _ptr(weight)
+            if hw != hs:
+                raise ValueError("weight and sinogram must both be host or both be device arrays")
+            check(self.lib.tomo_wgd_run(self.ctx.h, self.h, C.byref(p), C.c_void_p(ps), C.c_void_p(pw),
+                                        C.c_void_p(pv), stats, st))
+        else:
+            fn = self.lib.tomo_cg_run if conjugate else self.lib.tomo_gd_run
+            check(fn(self.ctx.h, self.h, C.byref(p), C.c_void_p(ps), C.c_void_p(pv), stats, st))
         return [dict(residual_sq=s.residual_sq, norm_dj=s.norm_dj, norm_dj_proj=s.norm_dj_proj, alpha=s.alpha)
                 for s in stats[:iters]]
 
