@@ -26,6 +26,9 @@ namespace tomo {
 constexpr int kBatchMax = 64;
 constexpr int kBatchGrid = 2048;
 constexpr int kVecTile = 4096;  // elements per tile of the element-wise ops
+// VI workgroup: kViTileX x kViTileY voxel columns of kViTileZ voxels (a wave
+// is 64 consecutive x columns, whose taps lie along detector rows).
+constexpr int kViTileX = 64, kViTileY = 4, kViTileZ = 16;
 
 struct BatchItem {
     DevGeom g;

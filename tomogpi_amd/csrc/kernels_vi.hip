@@ -22,7 +22,7 @@
 namespace tomo {
 namespace {
 
-constexpr int TX = 16, TY = 16, ZB = 16;
+constexpr int TX = kViTileX, TY = kViTileY, ZB = kViTileZ;
 
 struct ColumnView {
     float u, A;

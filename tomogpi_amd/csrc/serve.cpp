@@ -313,7 +313,7 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
             vq.in = q.r;
             vq.out = q.tq;
             vq.flags = cflag;
-            BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, 16), cdiv(d.ny, 16), cdiv(d.nz, 16));
+            BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, kViTileX), cdiv(d.ny, kViTileY), cdiv(d.nz, kViTileZ));
             vi.in = q.tq;
             vi.out = q.dj;
             vi.a = 1.0f;
@@ -388,7 +388,7 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
         vq.in = q.r;
         vq.out = q.tq;
         vq.flags = cflag;
-        BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, 16), cdiv(d.ny, 16), cdiv(d.nz, 16));
+        BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, kViTileX), cdiv(d.ny, kViTileY), cdiv(d.nz, kViTileZ));
         vi.in = q.tq;
         vi.out = q.dj;
         vi.a = -2.0f;
