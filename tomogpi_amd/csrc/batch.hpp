@@ -29,6 +29,11 @@ constexpr int kVecTile = 4096;  // elements per tile of the element-wise ops
 // VI workgroup: kViTileX x kViTileY voxel columns of kViTileZ voxels (a wave
 // is 64 consecutive x columns, whose taps lie along detector rows).
 constexpr int kViTileX = 64, kViTileY = 4, kViTileZ = 16;
+// Projector workgroup (RS, Siddon): kProjTileU detector columns x kProjTileV
+// rows of one view; a wave is 32 columns x 2 rows, rays close enough to
+// enter and leave the FOV within a few slabs of each other (the waves walk
+// slabs or samples in lockstep).
+constexpr int kProjTileU = 32, kProjTileV = 8;
 
 struct BatchItem {
     DevGeom g;

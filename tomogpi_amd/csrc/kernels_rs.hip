@@ -21,7 +21,7 @@
 namespace tomo {
 namespace {
 
-constexpr int BX = 64, BY = 4;
+constexpr int BX = kProjTileU, BY = kProjTileV;
 
 struct RsRay {
     float ex, ey, ez, dx, dy, dz;
@@ -278,7 +278,7 @@ __host__ __device__ __forceinline__ CopyShape rs_copy_shape(int nx, int ny, int 
 // iterations after the wave's first entry along A, k0 = its entry's distance
 // behind that one in samples (|d_A| per sample): at iteration k every lane
 // samples near the same A plane, one sample per lane per iteration.  Its 64
-// rays (adjacent detector columns) then read neighbouring quads of the
+// rays (a 32 x 2 block of detector cells) then read neighbouring quads of the
 // z-pair copy of A, two loads per sample (16 bytes each for FP32 storage,
 // 8 for 16-bit).  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
 // (every view of a row band before the next band).
@@ -409,7 +409,7 @@ __global__ __launch_bounds__(BX* BY) void rs_count_kernel(DevGeom g, int cube, u
     if (m < g.nu && n < g.nv) c = (unsigned long long)rs_setup(g, cube, p, m, n).lr;
     // wave-level sum, one atomic per wave
     for (int off = 32; off > 0; off >>= 1) c += __shfl_down(c, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(total, c);
+    if (((threadIdx.y * BX + threadIdx.x) & 63) == 0) atomicAdd(total, c);  // first lane of the wave
 }
 
 }  // namespace

@@ -22,7 +22,7 @@
 namespace tomo {
 namespace {
 
-constexpr int BX = 64, BY = 4;
+constexpr int BX = kProjTileU, BY = kProjTileV;
 
 struct SidRay {
     int idx[3], step[3];
@@ -234,7 +234,7 @@ __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD&
 // with e1 <= e2 the interior crossings (tA when absent), every end clipped to
 // tmax.  These are the segments of the general step in the same order; an
 // absent one has zero length and adds exactly 0.  ZNEG: the wave's rays go
-// down in z (one detector row, one sign).
+// down in z (rows on one side of the detector centre, one sign).
 // Quad reads of a z-pair copy: a buffer resource with a 32-bit byte offset
 // (copies below 4 GiB; the walk carries the offset in bytes) or a plain
 // global load by element index.
@@ -333,7 +333,7 @@ __device__ __forceinline__ void sid_walk_quads_b(const SidRay& r, const QR& rd, 
 // it.  What changes is the schedule and the memory layout: when every ray of
 // the wave has the same dominant in-plane axis A (|d_A| >= the other two
 // components) and the same direction along it, the wave walks the slabs of A
-// together.  The 64 rays of a wave are adjacent detector columns, spread
+// together.  The 64 rays of a wave are a 32-column x 2-row detector block, spread
 // across the other in-plane axis, so at a given slab they read voxels that
 // are contiguous when that other axis varies fastest: x-fastest (the caller's
 // layout) for y-slabs, a y-fastest copy for x-slabs.  Waves that mix axes
@@ -378,7 +378,7 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
         for (int c = 0; c < 4; ++c)
             if (__ballot(ok && code == c) == okmask) mode = c;
     }
-    // one detector row per wave: one z direction (or none)
+    // one z direction per wave (its rows on one side of the detector centre), or none
     const bool zneg_all = __ballot(ok && r.step[2] > 0) == 0ull;
     const bool zpos_all = __ballot(ok && r.step[2] < 0) == 0ull;
     if (!zneg_all && !zpos_all) mode = -1;
@@ -488,7 +488,7 @@ __global__ __launch_bounds__(BX* BY) void siddon_count_kernel(DevGeom g, unsigne
     }
     unsigned long long cc = c;
     for (int off = 32; off > 0; off >>= 1) cc += __shfl_down(cc, off, 64);
-    if ((threadIdx.x & 63) == 0) atomicAdd(total, cc);
+    if (((threadIdx.y * BX + threadIdx.x) & 63) == 0) atomicAdd(total, cc);  // first lane of the wave
 }
 
 // Reference C path, statement for statement in its arithmetic types.

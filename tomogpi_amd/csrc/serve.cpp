@@ -276,7 +276,7 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
         const tomo_gd_params& gd = q.req.gd;
         const size_t nv = q.geom.nvox(), ns = q.geom.nsino();
         const int k = q.k;  // 1-based iteration of this set
-        const uint32_t pgx = cdiv(d.nu, 64), pgz = cdiv(d.nv, 4);
+        const uint32_t pgx = cdiv(d.nu, kProjTileU), pgz = cdiv(d.nv, kProjTileV);
         const bool sid = gd.pair == TOMO_PAIR_SIDDONVI;
         const int cube = (gd.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0;
         const int32_t cflag = q.bf16 ? kItemBF16 : 0;  // operator copies in BF16
