@@ -85,31 +85,66 @@ def timed(fn, reps, stream):
 
 
 def cpu_baseline(vol_host, n, nphi, views, threads):
-    """FP32 oracle (test-infrastructure C restatement) on the host cores:
-    VI backprojection and clean Siddon projection over the first `views`
-    views of the same acquisition."""
+    """FP32 oracle (test-infrastructure C restatement) on the host cores.
+    value: VI backprojection and clean Siddon projection over the first
+    `views` views of the same acquisition on `threads` threads.  Beside it:
+    RS on the same views, Siddon single-threaded on one view (the reference's
+    C path is single-threaded), the 256^3 Shepp-Logan workload in full
+    (256 views; Siddon, VI and RS on `threads` threads) and the host."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import numpy as np
-    from oracle_ffi import Oracle, scaled_geometry, set_threads
+    from oracle_ffi import Oracle, scaled_geometry, set_threads, shepp_logan
+
+    def sub(nn, k, total):
+        g = scaled_geometry(nn, k)
+        g.num_stop, g.den_stop = 2.0 * k, float(total)   # same angles as views 0..k-1
+        return Oracle(g)
+
+    def clock(fn):
+        t0 = time.perf_counter()
+        out = fn()
+        return out, time.perf_counter() - t0
+
     set_threads(threads)
-    g = scaled_geometry(n, views)
-    g.num_stop, g.den_stop = 2.0 * views, float(nphi)   # same angles as views 0..views-1
-    o = Oracle(g)
-    t0 = time.perf_counter()
-    sino = o.siddon(vol_host)
-    t1 = time.perf_counter()
-    o.vi(sino)
-    t2 = time.perf_counter()
+    o = sub(n, views, nphi)
+    sino, t_sid = clock(lambda: o.siddon(vol_host))
+    _, t_vi = clock(lambda: o.vi(sino))
+    _, t_rs = clock(lambda: o.rs(vol_host))
     upd = float(n) ** 3 * views
+    set_threads(1)
+    o1 = sub(n, 1, nphi)
+    _, t_sid1 = clock(lambda: o1.siddon(vol_host))
+    set_threads(threads)
+    # the 256^3 workload in full
+    o0 = Oracle(scaled_geometry(256, 256))
+    v0 = shepp_logan(256)
+    s0, t0_sid = clock(lambda: o0.siddon(v0))
+    _, t0_vi = clock(lambda: o0.vi(s0))
+    _, t0_rs = clock(lambda: o0.rs(v0))
+    upd0 = 256.0 ** 3 * 256
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
     return {
-        "value": round(2 * upd / (t2 - t0) / 1e9, 4),
+        "value": round(2 * upd / (t_sid + t_vi) / 1e9, 4),
         "unit": "GUPS",
         "cores": threads,
         "kind": "port",
         "sample": f"clean Siddon projection + VI backprojection of the {n}^3 phantom over views 0..{views - 1} "
-                  f"of {nphi} (FP32 oracle, OpenMP): siddon {t1 - t0:.2f} s, vi {t2 - t1:.2f} s",
-        "siddon_gups": round(upd / (t1 - t0) / 1e9, 4),
-        "vi_gups": round(upd / (t2 - t1) / 1e9, 4),
+                  f"of {nphi} (FP32 oracle, OpenMP, {threads} threads): siddon {t_sid:.2f} s, vi {t_vi:.2f} s",
+        "siddon_gups": round(upd / t_sid / 1e9, 4),
+        "vi_gups": round(upd / t_vi / 1e9, 4),
+        "rs_gups": round(upd / t_rs / 1e9, 4),
+        "siddon_1thread_gups": round(float(n) ** 3 / t_sid1 / 1e9, 4),
+        "c0_256_full": {"siddon_s": round(t0_sid, 3), "vi_s": round(t0_vi, 3), "rs_s": round(t0_rs, 3),
+                        "siddon_gups": round(upd0 / t0_sid / 1e9, 4), "vi_gups": round(upd0 / t0_vi / 1e9, 4),
+                        "rs_gups": round(upd0 / t0_rs / 1e9, 4),
+                        "workload": "256^3 Shepp-Logan, 256 views, 256x256 detector, FP32 oracle"},
+        "host": {"cpu_model": model, "nproc": os.cpu_count(), "threads": threads},
     }
 
 
