@@ -170,6 +170,33 @@ def test_16bit_vi_bit_exact_fine_rows(ctx, dt):
         G.close()
 
 
+# Short source distance: the row step A runs from 1.3 to 4.6 over the volume,
+# so within one launch some workgroup-views stage their taps in LDS and the
+# others read them directly; on FINE_ROWS every view reads them directly.
+NEAR_SOURCE = Geometry(fod=80.0, fdd=160.0, nphi=16, su=160.0, sv=160.0, nu=160, nv=160, sx=64.0, sy=64.0,
+                       sz=64.0, nx=64, ny=64, nz=64)
+
+
+@pytest.mark.parametrize("name", ["near_source", "fine_rows"])
+@pytest.mark.parametrize("fdk", [False, True])
+def test_vi_bit_exact_unstaged_views(ctx, name, fdk):
+    """FP32 VI where the taps of some or all views do not fit the staging buffer."""
+    g = NEAR_SOURCE if name == "near_source" else FINE_ROWS
+    print(f"[case] geometry={name} fdk={fdk}", flush=True)
+    G = lib.Geometry(ctx, to_desc(g))
+    try:
+        o = Oracle(g)
+        rng = np.random.default_rng(13)
+        sino = rng.random(g.sino_shape, dtype=np.float32)
+        ref = o.vi_fdk(sino) if fdk else o.vi(sino)
+        out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.backproject(dev(sino), out, flags=lib.FLAG_FDK if fdk else 0)
+        got = out.cpu().numpy()
+        assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+    finally:
+        G.close()
+
+
 @pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "vi"])
 def test_adjoint_dot(setup, ctx, op):
     """<A x, y> = <x, A^T y> with the GPU operator and its GPU transpose (rel 1e-5)."""

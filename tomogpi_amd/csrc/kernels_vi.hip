@@ -13,6 +13,7 @@
 // terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
 // and the per-voxel v, bilinear weights and sum in FP32.
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <type_traits>
 
@@ -331,6 +332,272 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged VI for FP32 sinograms.
+//
+// A workgroup owns SX x SY voxel columns of ZS voxels.  For each view it
+// copies the detector taps its voxels read into LDS with coalesced loads
+// (taps off the detector as 0), and each voxel-view update reads its four
+// taps from there with two ds_read2_b32 instead of one 16-byte gather from
+// the tap-quad copy; the arithmetic is vi_tile's.  The taps are the box of
+// rows [v0, v1 + 1] and columns [u0, u1 + 1] with (u0, u1) the extreme
+// clamped cell corners floor(u) of the tile's columns and (v0, v1) the
+// extreme floor(v) of its voxels.  A table holds, per (column tile, view),
+// u0, u1 and the extreme A of the tile's columns (vi_box_kernel, the same
+// FP64 column terms); since v = fmaf(z', A, cv) is monotone in z' and in A,
+// the row extremes are the floors at the four (z', A) corners, so the box is
+// exact.  The rows are not clamped as in vi_tile: a row off the detector
+// holds zeros in the box, as the clamped cell it stands for.  View p is
+// staged in buffer p & 1, so one barrier per view separates the copy from
+// the reads.  A view whose box exceeds the buffer reads the taps directly.
+constexpr int SX = 32, SY = 8;   // voxel columns per workgroup; a wave is 32 x 2 columns
+constexpr int kStagePitch = 65;  // floats per staged row (odd: rows of one column fall in different banks)
+
+struct ViBox {
+    int32_t u0, u1;    // extreme clamped cell corners of the tile's columns
+    float amin, amax;  // extreme A of the tile's columns
+};
+
+template <int ZS>
+struct StageShape {
+    static constexpr int rows = ZS == 32 ? 64 : 44;  // staged rows per view
+    static constexpr int taps = rows * kStagePitch;  // floats per buffer
+    static constexpr int slots = ZS == 32 ? 12 : 8;  // box taps per thread at most (box taps <= slots * 256)
+};
+
+__device__ __forceinline__ int wave_min(int v) { return __reduce_min_sync(~0ull, v); }
+__device__ __forceinline__ int wave_max(int v) { return __reduce_max_sync(~0ull, v); }
+
+// Order-preserving map of a finite float to int: x < y  <=>  key(x) < key(y).
+__device__ __forceinline__ int fkey(float v) {
+    const int b = __float_as_int(v);
+    return b >= 0 ? b : (int)(0x80000000u - (uint32_t)b);
+}
+__device__ __forceinline__ float fkey_inv(int k) {
+    return __int_as_float(k >= 0 ? k : (int)(0x80000000u - (uint32_t)k));
+}
+
+// One workgroup per (column tile, view): the tile's ViBox for that view.
+__global__ __launch_bounds__(SX* SY) void vi_box_kernel(DevGeom g, ViBox* __restrict__ box, int ntx) {
+    const uint32_t p = blockIdx.x % (uint32_t)g.nphi, tile = blockIdx.x / (uint32_t)g.nphi;
+    const int i = (int)(tile % ntx) * SX + (int)(threadIdx.x % SX);
+    const int j = (int)(tile / ntx) * SY + (int)(threadIdx.x / SX);
+    int k[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
+    if (i < g.nx && j < g.ny) {
+        const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
+        const ColumnView cvw = column_view<false>(g, g.view + (size_t)p * kViewStride, xp, yp);
+        const int uc = (int)__builtin_amdgcn_fmed3f(floorf(cvw.u), -2.0f, (float)g.nu);
+        k[0] = k[1] = uc;
+        k[2] = k[3] = fkey(cvw.A);
+    }
+    __shared__ int part[SX * SY / 64][4];
+    const int w = threadIdx.x >> 6;
+    const int r0 = wave_min(k[0]), r1 = wave_max(k[1]), r2 = wave_min(k[2]), r3 = wave_max(k[3]);
+    if ((threadIdx.x & 63) == 0) {
+        part[w][0] = r0;
+        part[w][1] = r1;
+        part[w][2] = r2;
+        part[w][3] = r3;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int m[4] = {part[0][0], part[0][1], part[0][2], part[0][3]};
+        for (int q = 1; q < SX * SY / 64; ++q) {
+            m[0] = min(m[0], part[q][0]);
+            m[1] = max(m[1], part[q][1]);
+            m[2] = min(m[2], part[q][2]);
+            m[3] = max(m[3], part[q][3]);
+        }
+        ViBox b;
+        b.u0 = m[0];
+        b.u1 = m[1];
+        b.amin = fkey_inv(m[2]);
+        b.amax = fkey_inv(m[3]);
+        box[(size_t)tile * g.nphi + p] = b;
+    }
+}
+
+// Per-view staging plan of a workgroup: the box and the taps this thread
+// copies (slot s holds tap idx = tid + s * SX * SY of the box, row-major).
+struct StagePlan {
+    int u0, v0, cols, taps;
+    float rc;   // 1 / cols (v_rcp_f32)
+    bool fits;
+};
+
+template <int ZS>
+__device__ __forceinline__ StagePlan stage_plan(const ViBox& bb, float zp0, float cv) {
+    const float zp1 = zp0 + (float)(ZS - 1);
+    const float va = fmaf(zp0, bb.amin, cv), vb = fmaf(zp0, bb.amax, cv);
+    const float vc = fmaf(zp1, bb.amin, cv), vd = fmaf(zp1, bb.amax, cv);
+    const float vlo = floorf(fminf(fminf(va, vb), fminf(vc, vd)));
+    const float vhi = floorf(fmaxf(fmaxf(va, vb), fmaxf(vc, vd)));
+    StagePlan s;
+    s.u0 = bb.u0;
+    s.cols = bb.u1 - bb.u0 + 2;
+    s.fits = vlo >= -32768.0f && vhi <= 32768.0f && vhi - vlo + 2.0f <= (float)StageShape<ZS>::rows &&
+             s.cols <= kStagePitch;
+    const int taps = ((int)(vhi - vlo) + 2) * s.cols;
+    s.fits = s.fits && taps <= StageShape<ZS>::slots * SX * SY;
+    // wave-uniform: keep the plan in scalar registers
+    s.fits = __builtin_amdgcn_readfirstlane((int)s.fits) != 0;
+    s.v0 = __builtin_amdgcn_readfirstlane(s.fits ? (int)vlo : 0);
+    s.taps = __builtin_amdgcn_readfirstlane(s.fits ? taps : 0);
+    s.rc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(__builtin_amdgcn_rcpf((float)s.cols))));
+    return s;
+}
+
+// Row and column of box tap idx: (idx + 0.5) / cols lies at least 0.5 / cols
+// from an integer, far above the rounding of the product.
+__device__ __forceinline__ void stage_rc(const StagePlan& s, int idx, int& r, int& c) {
+    r = (int)(((float)idx + 0.5f) * s.rc);
+    c = idx - r * s.cols;
+}
+
+template <typename TV, bool FDK, int ZS>
+__device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __restrict__ sino,
+                                               const ViBox* __restrict__ boxes, TV* __restrict__ vol,
+                                               float vol_scale, float post, int bx, int by, int bz,
+                                               float* __restrict__ stage) {
+    using SS = StageShape<ZS>;
+    constexpr int NT = SX * SY;
+    constexpr int kSlots = SS::slots;
+    const int tid = threadIdx.x;
+    const int i = bx * SX + (tid % SX);
+    const int j = by * SY + (tid / SX);
+    const int k0 = bz * ZS;
+    const bool active = i < g.nx && j < g.ny;
+    const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
+    const float zp0 = (float)((double)k0 - (double)g.cz);
+    const float cv = g.cv;
+    const float fnu = (float)g.nu;
+    const int nu = g.nu, nv = g.nv;
+    const size_t plane = (size_t)nu * nv;
+    const ViBox* bt = boxes + (size_t)(by * ((g.nx + SX - 1) / SX) + bx) * g.nphi;
+    float acc[ZS];
+#pragma unroll
+    for (int k = 0; k < ZS; ++k) acc[k] = 0.0f;
+    auto add = [&](const ColumnView& cvw, int k, float r) {
+        if (FDK) {
+            const float vp = (zp0 + (float)k) * cvw.A;
+            const float wt = cvw.num / sqrtf(fmaf(vp, vp, cvw.base));
+            acc[k] += wt * r;
+        } else {
+            acc[k] += r;
+        }
+    };
+    auto term = [&](const ColumnView& cvw, float fu, int k, float v, float vf, float t00, float t01, float t10,
+                    float t11) {
+        // t00 = (u, v), t01 = (u, v+1), t10 = (u+1, v), t11 = (u+1, v+1)
+        const float fv = v - vf;
+        const float a = fmaf(fu, t10 - t00, t00);
+        const float b = fmaf(fu, t11 - t01, t01);
+        add(cvw, k, fmaf(fv, b - a, a));
+    };
+    // the taps of view p's box this thread copies, loaded one view ahead; a
+    // tap off the detector is read at an offset past the view's end (0)
+    float val[kSlots];
+    const uint32_t view_bytes = (uint32_t)(plane * sizeof(float));
+    auto load = [&](const StagePlan& s, int p) {
+        const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(sino + (size_t)p * plane, view_bytes);
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            if (q * NT < s.taps) {
+                int t = tid;
+                asm volatile("" : "+v"(t));  // keep the slot's index arithmetic here, not hoisted
+                const int idx = t + q * NT;
+                int r, c;
+                stage_rc(s, idx, r, c);
+                const int u = s.u0 + c, v = s.v0 + r;
+                const bool in = idx < s.taps && (unsigned)u < (unsigned)nu && (unsigned)v < (unsigned)nv;
+                const uint32_t off = in ? (uint32_t)(v * nu + u) * 4u : 0xfffffff0u;
+                val[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off, 0, 0));
+            }
+        }
+    };
+    StagePlan cur = stage_plan<ZS>(bt[0], zp0, cv);
+    load(cur, 0);
+    for (int p = 0; p < g.nphi; ++p) {
+        float* sb = stage + (p & 1) * SS::taps;
+#pragma unroll
+        for (int q = 0; q < kSlots; ++q) {
+            if (q * NT < cur.taps) {
+                int t = tid;
+                asm volatile("" : "+v"(t));
+                const int idx = t + q * NT;
+                if (idx < cur.taps) {
+                    int r, c;
+                    stage_rc(cur, idx, r, c);
+                    sb[r * kStagePitch + c] = val[q];
+                }
+            }
+        }
+        __syncthreads();
+        const StagePlan now = cur;
+        if (p + 1 < g.nphi) {
+            cur = stage_plan<ZS>(bt[p + 1], zp0, cv);
+            load(cur, p + 1);
+        }
+        if (active) {
+            const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
+            const float uf = floorf(cvw.u);
+            const float fu = cvw.u - uf;
+            const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
+            if (now.fits) {
+                // byte offset in the buffer of tap (uc, vf): ((vf - v0) pitch + uc - u0) 4,
+                // one exact fma of small integers
+                const float colb =
+                    (float)(((int)uc - now.u0 - now.v0 * kStagePitch + (p & 1) * SS::taps) * 4);
+#pragma unroll
+                for (int k = 0; k < ZS; ++k) {
+                    const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                    const float vf = floorf(v);
+                    const uint32_t off = (uint32_t)fmaf(vf, (float)(kStagePitch * 4), colb);
+                    const float* q = (const float*)((const char*)stage + off);
+                    term(cvw, fu, k, v, vf, q[0], q[kStagePitch], q[1], q[kStagePitch + 1]);
+                }
+            } else {
+                // box larger than the buffer: the four taps straight from the view, one voxel at a time
+                const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(sino + (size_t)p * plane, view_bytes);
+                const int iu = (int)uc;
+                const bool u0ok = (unsigned)iu < (unsigned)nu, u1ok = (unsigned)(iu + 1) < (unsigned)nu;
+                ColumnView cw = cvw;
+                asm volatile("" : "+v"(cw.A));  // this path's rows are its own, not hoisted above the branch
+                auto rd = [&](bool ok, int v, int u) {
+                    const uint32_t off = ok ? (uint32_t)(v * nu + u) * 4u : 0xfffffff0u;
+                    return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off, 0, 0));
+                };
+#pragma unroll
+                for (int k = 0; k < ZS; ++k) {
+                    const float v = fmaf(zp0 + (float)k, cw.A, cv);
+                    const float vf = floorf(v);
+                    const int iv = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, (float)nv);
+                    const bool v0ok = (unsigned)iv < (unsigned)nv, v1ok = (unsigned)(iv + 1) < (unsigned)nv;
+                    term(cw, fu, k, v, vf, rd(u0ok && v0ok, iv, iu), rd(u0ok && v1ok, iv + 1, iu),
+                         rd(u1ok && v0ok, iv, iu + 1), rd(u1ok && v1ok, iv + 1, iu + 1));
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+            }
+        }
+    }
+    if (!active) return;
+    const size_t sxy = (size_t)g.nx * g.ny;
+    const size_t base = (size_t)i + (size_t)g.nx * j;
+#pragma unroll
+    for (int k = 0; k < ZS; ++k)
+        if (k0 + k < g.nz)
+            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? acc[k] : (acc[k] * g.delta) * post,
+                               vol_scale);
+}
+
+template <typename TV, bool FDK, int ZS>
+__global__ __launch_bounds__(SX* SY, 6) void vi_staged_kernel(DevGeom g, const float* __restrict__ sino,
+                                                         const ViBox* __restrict__ boxes, TV* __restrict__ vol,
+                                                         float vol_scale) {
+    __shared__ float stage[2 * StageShape<ZS>::taps];
+    vi_staged_tile<TV, FDK, ZS>(g, sino, boxes, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z, stage);
+}
+
 template <typename TS>
 __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
                                                      int nv) {
@@ -467,17 +734,51 @@ static void launch_vi_kernel(const DevGeom& d, dim3 grid, int group, const void*
                            (TV*)vol, vol_scale);
 }
 
+// Voxels per thread column of the staged kernel.
+constexpr int kStagedZ = 16;
+
+template <typename TV>
+static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino, void* vol, float vol_scale,
+                                   hipStream_t s) {
+    const int ntx = ceil_div(d.nx, SX), nty = ceil_div(d.ny, SY);
+    void* boxes = nullptr;
+    hipError_t e = hipMallocAsync(&boxes, (size_t)ntx * nty * d.nphi * sizeof(ViBox), s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(vi_box_kernel, dim3((unsigned)((size_t)ntx * nty * d.nphi)), dim3(SX * SY), 0, s, d,
+                       (ViBox*)boxes, ntx);
+    e = hipGetLastError();
+    if (e == hipSuccess) {
+        const dim3 grid(ntx, nty, ceil_div(d.nz, kStagedZ));
+        if (fdk)
+            hipLaunchKernelGGL((vi_staged_kernel<TV, true, kStagedZ>), grid, dim3(SX * SY), 0, s, d, sino,
+                               (const ViBox*)boxes, (TV*)vol, vol_scale);
+        else
+            hipLaunchKernelGGL((vi_staged_kernel<TV, false, kStagedZ>), grid, dim3(SX * SY), 0, s, d, sino,
+                               (const ViBox*)boxes, (TV*)vol, vol_scale);
+        e = hipGetLastError();
+    }
+    const hipError_t f = hipFreeAsync(boxes, s);
+    return e != hipSuccess ? e : f;
+}
+
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) {
     const DevGeom& d = g->dev;
-    // Tap-quad (FP32) or tap-column (16-bit) copy of the sinogram from the
-    // stream-ordered pool, released after the backprojection on the same
-    // stream; both take 4 bytes per tap-quad element.
+    if (sino.dtype == TOMO_F32) {
+        // FP32 sinograms: LDS-staged taps read from the sinogram itself.
+        hipError_t e = hipSuccess;
+        TOMO_DISPATCH_STORAGE(vol.dtype, TV, { e = launch_vi_staged<TV>(d, fdk, (const float*)sino.p, vol.p,
+                                                                        vol.scale, s); });
+        TOMO_HIP(e);
+        return TOMO_OK;
+    }
+    // 16-bit sinograms: tap-column copy from the stream-ordered pool,
+    // released after the backprojection on the same stream.
     void* quads = nullptr;
     TOMO_HIP(hipMallocAsync(&quads, vi_quad_elems(d.nu, d.nv, d.nphi) * 4, s));
     const dim3 qgrid(ceil_div(d.nu + 4, 64), ceil_div(d.nv + 3, 4), d.nphi);
     const dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
     const float inv = 1.0f / sino.scale;
-    const int group = sino.dtype != TOMO_F32 ? vi_row_group(g) : 1;
+    const int group = vi_row_group(g);
     hipError_t le = hipSuccess;
     TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
         hipLaunchKernelGGL((vi_quads_kernel<TS>), qgrid, dim3(256), 0, s, (const TS*)sino.p, (TS*)quads, d.nu, d.nv);
