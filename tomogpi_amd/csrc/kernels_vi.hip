@@ -362,7 +362,6 @@ template <int ZS>
 struct StageShape {
     static constexpr int rows = ZS == 32 ? 64 : 44;  // staged rows per view
     static constexpr int taps = rows * kStagePitch;  // floats per buffer
-    static constexpr int slots = ZS == 32 ? 12 : 8;  // box taps per thread at most (box taps <= slots * 256)
 };
 
 __device__ __forceinline__ int wave_min(int v) { return __reduce_min_sync(~0ull, v); }
@@ -417,11 +416,11 @@ __global__ __launch_bounds__(SX* SY) void vi_box_kernel(DevGeom g, ViBox* __rest
     }
 }
 
-// Per-view staging plan of a workgroup: the box and the taps this thread
-// copies (slot s holds tap idx = tid + s * SX * SY of the box, row-major).
+// Per-view staging plan of a workgroup: the box of taps (rows from v0,
+// columns from u0) and whether it fits the buffer.  Wave w copies box rows
+// w, w + 4, ..., one column per lane, so a box is at most 64 columns wide.
 struct StagePlan {
-    int u0, v0, cols, taps;
-    float rc;   // 1 / cols (v_rcp_f32)
+    int u0, v0, cols, rows;
     bool fits;
 };
 
@@ -435,23 +434,13 @@ __device__ __forceinline__ StagePlan stage_plan(const ViBox& bb, float zp0, floa
     StagePlan s;
     s.u0 = bb.u0;
     s.cols = bb.u1 - bb.u0 + 2;
-    s.fits = vlo >= -32768.0f && vhi <= 32768.0f && vhi - vlo + 2.0f <= (float)StageShape<ZS>::rows &&
-             s.cols <= kStagePitch;
-    const int taps = ((int)(vhi - vlo) + 2) * s.cols;
-    s.fits = s.fits && taps <= StageShape<ZS>::slots * SX * SY;
+    const bool fits = vlo >= -32768.0f && vhi <= 32768.0f && vhi - vlo + 2.0f <= (float)StageShape<ZS>::rows &&
+                      s.cols <= 64;
     // wave-uniform: keep the plan in scalar registers
-    s.fits = __builtin_amdgcn_readfirstlane((int)s.fits) != 0;
+    s.fits = __builtin_amdgcn_readfirstlane((int)fits) != 0;
     s.v0 = __builtin_amdgcn_readfirstlane(s.fits ? (int)vlo : 0);
-    s.taps = __builtin_amdgcn_readfirstlane(s.fits ? taps : 0);
-    s.rc = __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(__builtin_amdgcn_rcpf((float)s.cols))));
+    s.rows = __builtin_amdgcn_readfirstlane(s.fits ? (int)(vhi - vlo) + 2 : 0);
     return s;
-}
-
-// Row and column of box tap idx: (idx + 0.5) / cols lies at least 0.5 / cols
-// from an integer, far above the rounding of the product.
-__device__ __forceinline__ void stage_rc(const StagePlan& s, int idx, int& r, int& c) {
-    r = (int)(((float)idx + 0.5f) * s.rc);
-    c = idx - r * s.cols;
 }
 
 template <typename TV, bool FDK, int ZS>
@@ -461,8 +450,10 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
                                                float* __restrict__ stage) {
     using SS = StageShape<ZS>;
     constexpr int NT = SX * SY;
-    constexpr int kSlots = SS::slots;
+    constexpr int NW = NT / 64;                             // waves per workgroup
+    constexpr int kSlots = (SS::rows + NW - 1) / NW;        // box rows per wave at most
     const int tid = threadIdx.x;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int i = bx * SX + (tid % SX);
     const int j = by * SY + (tid / SX);
     const int k0 = bz * ZS;
@@ -494,49 +485,39 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
         const float b = fmaf(fu, t11 - t01, t01);
         add(cvw, k, fmaf(fv, b - a, a));
     };
-    // the taps of view p's box this thread copies, loaded one view ahead; a
-    // tap off the detector is read at an offset past the view's end (0)
-    float val[kSlots];
+    // Copy of view p's box into buffer b: wave wv copies box rows wv + q NW,
+    // lane l column l, straight from memory into LDS (buffer_load ... lds).
+    // A tap off the detector or beyond the box is read at an offset past the
+    // view's end, which writes 0.
     const uint32_t view_bytes = (uint32_t)(plane * sizeof(float));
-    auto load = [&](const StagePlan& s, int p) {
+    auto copy_box = [&](const StagePlan& s, int p, int b) {
+        if (!s.fits) return;
         const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(sino + (size_t)p * plane, view_bytes);
+        const int u = s.u0 + lane;
+        const bool uin = lane < s.cols && (unsigned)u < (unsigned)nu;
+        const int vw = s.v0 + wv;
+        const uint32_t base = ((uint32_t)vw * (uint32_t)nu + (uint32_t)u) * 4u;  // mod 2^32, exact when in range
+        float* dst = stage + b * SS::taps + wv * kStagePitch;
 #pragma unroll
         for (int q = 0; q < kSlots; ++q) {
-            if (q * NT < s.taps) {
-                int t = tid;
-                asm volatile("" : "+v"(t));  // keep the slot's index arithmetic here, not hoisted
-                const int idx = t + q * NT;
-                int r, c;
-                stage_rc(s, idx, r, c);
-                const int u = s.u0 + c, v = s.v0 + r;
-                const bool in = idx < s.taps && (unsigned)u < (unsigned)nu && (unsigned)v < (unsigned)nv;
-                const uint32_t off = in ? (uint32_t)(v * nu + u) * 4u : 0xfffffff0u;
-                val[q] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off, 0, 0));
+            if (wv + q * NW < s.rows) {
+                const bool in = uin && (unsigned)(vw + q * NW) < (unsigned)nv;
+                const uint32_t off = in ? base + (uint32_t)(q * NW) * (uint32_t)nu * 4u : 0xfffffff0u;
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(
+                    rsrc, (__attribute__((address_space(3))) void*)(dst + q * NW * kStagePitch), 4, off, 0, 0, 0);
             }
         }
     };
-    StagePlan cur = stage_plan<ZS>(bt[0], zp0, cv);
-    load(cur, 0);
+    StagePlan nxt = stage_plan<ZS>(bt[0], zp0, cv);
+    copy_box(nxt, 0, 0);
     for (int p = 0; p < g.nphi; ++p) {
-        float* sb = stage + (p & 1) * SS::taps;
-#pragma unroll
-        for (int q = 0; q < kSlots; ++q) {
-            if (q * NT < cur.taps) {
-                int t = tid;
-                asm volatile("" : "+v"(t));
-                const int idx = t + q * NT;
-                if (idx < cur.taps) {
-                    int r, c;
-                    stage_rc(cur, idx, r, c);
-                    sb[r * kStagePitch + c] = val[q];
-                }
-            }
-        }
+        // this wave's copies of view p have landed; every wave is done with view p - 1
+        __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
-        const StagePlan now = cur;
+        const StagePlan now = nxt;
         if (p + 1 < g.nphi) {
-            cur = stage_plan<ZS>(bt[p + 1], zp0, cv);
-            load(cur, p + 1);
+            nxt = stage_plan<ZS>(bt[p + 1], zp0, cv);
+            copy_box(nxt, p + 1, (p + 1) & 1);
         }
         if (active) {
             const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
