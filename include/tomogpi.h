@@ -330,11 +330,11 @@ typedef struct tomo_serve_request {
     float*       volume;        /* result, written when the request is done */
     int32_t      algorithm;     /* TOMO_ALGO_*                              */
     int32_t      storage;       /* element type of the operator copies (the
-                                   z-pair copies the projectors read, the
-                                   tap copy VI reads): TOMO_F32 (0) or
-                                   TOMO_BF16, rounded from the FP32 state
-                                   each time they are made; the state, the
-                                   sums and the result stay FP32            */
+                                   z-pair copies the projectors read):
+                                   TOMO_F32 (0) or TOMO_BF16, rounded from
+                                   the FP32 state each time they are made;
+                                   VI reads the FP32 residual, and the
+                                   state, the sums and the result stay FP32 */
     double       relaxation;    /* SIRT relaxation (1.0 for plain SIRT)     */
 } tomo_serve_request;
 

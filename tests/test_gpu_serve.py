@@ -209,12 +209,12 @@ def test_sirt_served_in_a_mix_equals_plain_sirt(ctx, problems):
 
 def test_bf16_copies_alone_equals_mix_and_close_to_fp32(ctx):
     """BF16 operator copies (storage=BF16): the projectors read BF16 z-pair
-    copies and VI BF16 tap columns, rounded from the FP32 state each time
-    they are made; sums, state and result stay FP32.  Each result is the
-    same alone and in the mix (bit-identical), and within rel-L2 2e-2 of
-    the same request with FP32 copies (6 iterations).  The mix has both
-    pairs, GD and SIRT, and a geometry whose row steps are near 4 (VI reads
-    one tap-column load per voxel there, one per voxel pair elsewhere)."""
+    copies rounded from the FP32 state each time they are made; VI, sums,
+    state and result stay FP32.  Each result is the same alone and in the
+    mix (bit-identical), and within rel-L2 2e-2 of the same request with
+    FP32 copies (6 iterations).  The mix has both pairs, GD and SIRT, and a
+    geometry whose row steps are near 4 (VI's detector boxes exceed the
+    staging buffer there and it reads the taps directly)."""
     from test_gpu_parity import FINE_ROWS
     specs = [  # (geometry, pair, algorithm, lam, kopt)
         (scaled_geometry(32, 24), lib.PAIR_SIDDONVI, lib.ALGO_GD, 0.01, 10),

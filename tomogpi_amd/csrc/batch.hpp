@@ -26,9 +26,22 @@ namespace tomo {
 constexpr int kBatchMax = 64;
 constexpr int kBatchGrid = 2048;
 constexpr int kVecTile = 4096;  // elements per tile of the element-wise ops
-// VI workgroup: kViTileX x kViTileY voxel columns of kViTileZ voxels (a wave
-// is 64 consecutive x columns, whose taps lie along detector rows).
+// VI workgroup on 16-bit tap columns: kViTileX x kViTileY voxel columns of
+// kViTileZ voxels (a wave is 64 consecutive x columns, whose taps lie along
+// detector rows).  The LDS-staged VI of FP32 sinograms (single problems and
+// shared launches) takes kViStageX x kViStageY columns of kViTileZ voxels.
 constexpr int kViTileX = 64, kViTileY = 4, kViTileZ = 16;
+constexpr int kViStageX = 32, kViStageY = 8;
+
+// Detector box of one (staged VI column tile, view): the extreme clamped
+// cell corners floor(u) and the extreme row step A of the tile's columns.
+struct ViBox {
+    int32_t u0, u1;
+    float amin, amax;
+};
+inline size_t vi_box_count(const DevGeom& d) {
+    return (size_t)((d.nx + kViStageX - 1) / kViStageX) * ((d.ny + kViStageY - 1) / kViStageY) * d.nphi;
+}
 // Projector workgroup (RS, Siddon): kProjTileU detector columns x kProjTileV
 // rows of one view; a wave is 32 columns x 2 rows, rays close enough to
 // enter and leave the FOV within a few slabs of each other (the waves walk
@@ -61,13 +74,10 @@ struct BatchTable {
     alignas(128) uint32_t next;
 };
 
-// Item flag bits of the operator launches (copies, projectors, VI): the
-// operator copies (z-pairs, tap columns) are BF16, converted from the FP32
-// state by the copy launches; VI on BF16 tap columns may read one load per
-// voxel pair (row steps below 2, shown on the host).  RS keeps bit 0 for the
-// cube FOV.
+// Item flag bit of the projector launches: the operator copies (z-pairs) are
+// BF16, converted from the FP32 state by the copy launches.  RS keeps bit 0
+// for the cube FOV.
 constexpr int32_t kItemBF16 = 0x10;
-constexpr int32_t kItemPairs = 0x20;
 
 // Per-problem scalars of the gradient loop, FP64, in `slot`.
 enum GdSlot {
@@ -116,8 +126,8 @@ hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s);   // in vol
 hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s);   // in volume -> out qy (FP32/BF16)
 hipError_t launch_batch_siddon(BatchTable* t, hipStream_t s);          // in qx, in2 qy -> out sinogram
 hipError_t launch_batch_rs(BatchTable* t, hipStream_t s);              // in qx, in2 qy -> out; flags bit0 cube
-hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s);  // in sinogram -> tap quads / BF16 columns
-hipError_t launch_batch_vi(BatchTable* t, hipStream_t s);              // in tap quads -> out volume * a
+hipError_t launch_batch_vi(BatchTable* t, hipStream_t s);              // in sinogram, aux boxes -> out volume * a
+int launch_vi_boxes(const DevGeom& d, void* boxes, hipStream_t s);     // vi_box_count(d) ViBox of the geometry
 hipError_t launch_batch_diff_sq(const BatchTable* t, hipStream_t s);   // out = in - in2; aux partials of out^2
 hipError_t launch_batch_sq(const BatchTable* t, hipStream_t s);        // aux partials of in^2
 hipError_t launch_batch_laplacian(const BatchTable* t, hipStream_t s); // out = L in; aux partials of out^2 if set

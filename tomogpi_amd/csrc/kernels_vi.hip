@@ -104,26 +104,18 @@ __device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __r
     out[o + 1] = b;
 }
 
-// Tap-column copy of a 16-bit sinogram (same grid as the tap quads), or
-// BF16 tap columns of an FP32 sinogram (TI = float, nearest even).
-template <typename TI>
-__device__ __forceinline__ uint32_t column_bits(TI v) {
-    if constexpr (std::is_same<TI, float>::value) return __bfloat16_as_ushort(__float2bfloat16(v));
-    else return v;
-}
-template <typename TI>
-__device__ __forceinline__ void vi_columns_tile(const TI* __restrict__ in, uint16_t* __restrict__ out, int nu,
+// Tap-column copy of a 16-bit sinogram (same grid as the tap quads).
+__device__ __forceinline__ void vi_columns_tile(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int nu,
                                                 int nv, int bx, int by, int p) {
     const int uu = bx * 64 + (threadIdx.x & 63);
     const int vv = by * 4 + (threadIdx.x >> 6);
     if (uu > nu + 3 || vv > nv + 2) return;
     const int u = uu - 2, v = vv - 2;
     const bool uok = u >= 0 && u < nu;
-    const TI* gp = in + (size_t)p * nu * nv;
+    const uint16_t* gp = in + (size_t)p * nu * nv;
     uint32_t t[4];
 #pragma unroll
-    for (int r = 0; r < 4; ++r)
-        t[r] = (uok && v + r >= 0 && v + r < nv) ? column_bits(gp[(size_t)(v + r) * nu + u]) : 0u;
+    for (int r = 0; r < 4; ++r) t[r] = (uok && v + r >= 0 && v + r < nv) ? gp[(size_t)(v + r) * nu + u] : 0u;
     const QuadShape qs = quad_shape(nu, nv, 4);
     const uint2 w = make_uint2(t[0] | (t[1] << 16), t[2] | (t[3] << 16));
     __builtin_memcpy(out + (size_t)p * qs.view + (size_t)vv * qs.row + 4u * (uint32_t)uu, &w, sizeof(w));
@@ -350,13 +342,8 @@ __device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__
 // holds zeros in the box, as the clamped cell it stands for.  View p is
 // staged in buffer p & 1, so one barrier per view separates the copy from
 // the reads.  A view whose box exceeds the buffer reads the taps directly.
-constexpr int SX = 32, SY = 8;   // voxel columns per workgroup; a wave is 32 x 2 columns
+constexpr int SX = kViStageX, SY = kViStageY;  // voxel columns per workgroup; a wave is 32 x 2 columns
 constexpr int kStagePitch = 65;  // floats per staged row (odd: rows of one column fall in different banks)
-
-struct ViBox {
-    int32_t u0, u1;    // extreme clamped cell corners of the tile's columns
-    float amin, amax;  // extreme A of the tile's columns
-};
 
 template <int ZS>
 struct StageShape {
@@ -376,9 +363,10 @@ __device__ __forceinline__ float fkey_inv(int k) {
     return __int_as_float(k >= 0 ? k : (int)(0x80000000u - (uint32_t)k));
 }
 
-// One workgroup per (column tile, view): the tile's ViBox for that view.
-__global__ __launch_bounds__(SX* SY) void vi_box_kernel(DevGeom g, ViBox* __restrict__ box, int ntx) {
-    const uint32_t p = blockIdx.x % (uint32_t)g.nphi, tile = blockIdx.x / (uint32_t)g.nphi;
+// The ViBox of column tile `tile` (row-major over ntx tiles per row) in view
+// p, one workgroup of SX x SY threads (one per column).
+__device__ __forceinline__ void vi_box_tile(const DevGeom& g, ViBox* __restrict__ box, int ntx, uint32_t tile,
+                                            uint32_t p) {
     const int i = (int)(tile % ntx) * SX + (int)(threadIdx.x % SX);
     const int j = (int)(tile / ntx) * SY + (int)(threadIdx.x / SX);
     int k[4] = {INT_MAX, INT_MIN, INT_MAX, INT_MIN};
@@ -392,6 +380,7 @@ __global__ __launch_bounds__(SX* SY) void vi_box_kernel(DevGeom g, ViBox* __rest
     __shared__ int part[SX * SY / 64][4];
     const int w = threadIdx.x >> 6;
     const int r0 = wave_min(k[0]), r1 = wave_max(k[1]), r2 = wave_min(k[2]), r3 = wave_max(k[3]);
+    __syncthreads();  // the previous tile's reads of part are done
     if ((threadIdx.x & 63) == 0) {
         part[w][0] = r0;
         part[w][1] = r1;
@@ -414,6 +403,11 @@ __global__ __launch_bounds__(SX* SY) void vi_box_kernel(DevGeom g, ViBox* __rest
         b.amax = fkey_inv(m[3]);
         box[(size_t)tile * g.nphi + p] = b;
     }
+}
+
+// One workgroup per (column tile, view).
+__global__ __launch_bounds__(SX* SY) void vi_box_kernel(DevGeom g, ViBox* __restrict__ box, int ntx) {
+    vi_box_tile(g, box, ntx, blockIdx.x / (uint32_t)g.nphi, blockIdx.x % (uint32_t)g.nphi);
 }
 
 // Per-view staging plan of a workgroup: the box of taps (rows from v0,
@@ -583,7 +577,7 @@ template <typename TS>
 __global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
                                                      int nv) {
     if constexpr (sizeof(TS) == 2)
-        vi_columns_tile<uint16_t>((const uint16_t*)in, (uint16_t*)out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
+        vi_columns_tile((const uint16_t*)in, (uint16_t*)out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
     else
         vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
 }
@@ -594,40 +588,18 @@ __global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restr
     vi_tile<TS, TV, FDK, GROUP>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// Shared launches (FP32 sinograms and volumes; kItemBF16: BF16 tap columns,
-// kItemPairs: their pair reads).  Tap quads / columns: tile grid
-// (ceil((nu+4)/64), ceil((nv+3)/4), nphi).  VI: tile grid (ceil(nx/16),
-// ceil(ny/16), ceil(nz/16)); the stored value is (delta * sum) * a.
-__global__ __launch_bounds__(256) void vi_quads_batch_kernel(const BatchTable* __restrict__ t) {
-    TOMO_BATCH_STRIDED(t) {
-        const BatchItem& it = t->item[batch_find(t, tile)];
-        const uint32_t local = tile - it.begin;
-        const uint32_t bx = local % it.gx, rest = local / it.gx;
-        if (it.flags & kItemBF16)
-            vi_columns_tile<float>((const float*)it.in, (uint16_t*)it.out, it.g.nu, it.g.nv, (int)bx,
-                                   (int)(rest % it.gy), (int)(rest / it.gy));
-        else
-            vi_quads_tile<float>((const float*)it.in, (float*)it.out, it.g.nu, it.g.nv, (int)bx, (int)(rest % it.gy),
-                                 (int)(rest / it.gy));
-    }
-}
-
-// Register budget for 7 waves per SIMD (72 VGPRs, no spills); unbounded, the
-// tile loop around both address paths takes 129 VGPRs and 3 waves.
-__global__ __launch_bounds__(TX* TY, 7) void vi_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+// Shared launch (FP32 sinograms and volumes): tile grid (ceil(nx/SX),
+// ceil(ny/SY), ceil(nz/16)) on the sinogram `in` with the problem's box table
+// `aux` (launch_vi_boxes); the stored value is (delta * sum) * a.
+__global__ __launch_bounds__(SX* SY, 6) void vi_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+    __shared__ float stage[2 * StageShape<kViTileZ>::taps];
     TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
         const int by = (int)(rest % it.gy), bz = (int)(rest / it.gy);
-        if (!(it.flags & kItemBF16))
-            vi_tile<float, float, false>(it.g, (const float*)it.in, 1.0f, (float*)it.out, 1.0f, it.a, (int)bx, by, bz);
-        else if (it.flags & kItemPairs)
-            vi_tile<__hip_bfloat16, float, false, 2>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out, 1.0f,
-                                                        it.a, (int)bx, by, bz);
-        else
-            vi_tile<__hip_bfloat16, float, false, 1>(it.g, (const __hip_bfloat16*)it.in, 1.0f, (float*)it.out,
-                                                         1.0f, it.a, (int)bx, by, bz);
+        vi_staged_tile<float, false, kViTileZ>(it.g, (const float*)it.in, (const ViBox*)it.aux, (float*)it.out, 1.0f,
+                                               it.a, (int)bx, by, bz, stage);
     }
 }
 
@@ -716,17 +688,16 @@ static void launch_vi_kernel(const DevGeom& d, dim3 grid, int group, const void*
 }
 
 // Voxels per thread column of the staged kernel.
-constexpr int kStagedZ = 16;
+constexpr int kStagedZ = kViTileZ;
 
 template <typename TV>
 static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino, void* vol, float vol_scale,
                                    hipStream_t s) {
     const int ntx = ceil_div(d.nx, SX), nty = ceil_div(d.ny, SY);
     void* boxes = nullptr;
-    hipError_t e = hipMallocAsync(&boxes, (size_t)ntx * nty * d.nphi * sizeof(ViBox), s);
+    hipError_t e = hipMallocAsync(&boxes, vi_box_count(d) * sizeof(ViBox), s);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(vi_box_kernel, dim3((unsigned)((size_t)ntx * nty * d.nphi)), dim3(SX * SY), 0, s, d,
-                       (ViBox*)boxes, ntx);
+    hipLaunchKernelGGL(vi_box_kernel, dim3((unsigned)vi_box_count(d)), dim3(SX * SY), 0, s, d, (ViBox*)boxes, ntx);
     e = hipGetLastError();
     if (e == hipSuccess) {
         const dim3 grid(ntx, nty, ceil_div(d.nz, kStagedZ));
@@ -779,13 +750,15 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
     return TOMO_OK;
 }
 
-hipError_t launch_batch_vi_quads(const BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(vi_quads_batch_kernel, dim3(kBatchGrid), dim3(256), 0, s, t);
-    return hipGetLastError();
+int launch_vi_boxes(const DevGeom& d, void* boxes, hipStream_t s) {
+    const int ntx = ceil_div(d.nx, SX);
+    hipLaunchKernelGGL(vi_box_kernel, dim3((unsigned)vi_box_count(d)), dim3(SX * SY), 0, s, d, (ViBox*)boxes, ntx);
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
 }
 
 hipError_t launch_batch_vi(BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(TX * TY), 0, s, t, &t->next);
+    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(SX * SY), 0, s, t, &t->next);
     return hipGetLastError();
 }
 

@@ -50,7 +50,7 @@ enum Op {
     OP_AXPY_PX, OP_AXPY_LLX,                // recurrent form: px += alpha pd, t2 += alpha t3
     OP_DIFF,                                // r = y - px, partials of r^2
     OP_FIN_RES,
-    OP_VQ, OP_VI,                           // dJ = -2 B r
+    OP_VI,                                  // dJ = -2 B r
     OP_AXPY_REG,                            // dJ += 2 lambda t2
     OP_LAP_DJ, OP_FIN_REG,                  // t1 = L dJ, sum t1^2
     OP_LAP_LDJ,                             // recurrent form: t3 = L t1
@@ -77,7 +77,6 @@ struct Request {
     double *part_res = nullptr, *part_vol = nullptr, *part_reg = nullptr, *part_pd = nullptr;
     double *slot = nullptr, *hist = nullptr;
     bool bf16 = false;      // BF16 operator copies
-    bool vi_pairs = false;  // BF16 VI reads one tap-column load per voxel pair
     int k = 0;              // iterations done
     bool prev_opt = false;  // the previous iteration computed P dJ (and L(L dJ))
     std::vector<double> hist_host;
@@ -149,7 +148,7 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo, size_t 
     sizes[i++] = ns * 4;                                                 // 10 px
     sizes[i++] = ns * 4;                                                 // 11 r
     sizes[i++] = ns * 4;                                                 // 12 pd (SIRT: sinogram weight R)
-    sizes[i++] = vi_quad_elems(d.nu, d.nv, d.nphi) * 4;                  // 13 tq (FP32 quads or BF16 columns)
+    sizes[i++] = vi_box_count(d) * sizeof(ViBox);                        // 13 tq (VI box table, set at admission)
     sizes[i++] = (size_t)vec_tiles(ns) * 8;                              // 14 part_res
     sizes[i++] = (size_t)vec_tiles(nv) * 8;                              // 15 part_vol
     sizes[i++] = lam ? (size_t)lap_tiles(d) * 8 : 0;                     // 16 part_reg
@@ -170,7 +169,6 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo, size_t 
 // request is then marked failed by the caller).
 int admit(tomo_server* srv, Request& q) {
     q.bf16 = q.req.storage == TOMO_BF16;
-    q.vi_pairs = q.bf16 && vi_row_group(&q.geom) >= 2;
     const Layout L = layout_of(q.geom, q.req.gd, q.req.algorithm, q.bf16 ? 2 : 4);
     TOMO_HIP(hipMallocFromPoolAsync(&q.block, L.total, srv->pool, srv->stream));
     char* b = (char*)q.block;
@@ -207,6 +205,8 @@ int admit(tomo_server* srv, Request& q) {
                             hipMemcpyHostToDevice, s));
     TOMO_HIP(hipMemcpyAsync(q.y, q.req.sinogram, ns * 4, q.host ? hipMemcpyHostToDevice : hipMemcpyDeviceToDevice, s));
     int rc;
+    // the VI box table of the geometry, read by every round's backprojection
+    if ((rc = launch_vi_boxes(q.geom.dev, q.tq, s))) return rc;
     if (sirt) {
         // R = 1/(P 1) into pd, C = relaxation/(B 1) into t1, with the
         // single-problem operators (the same arithmetic as tomo_sirt_run).
@@ -295,7 +295,6 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
             p.out = out;
             p.flags = cube | cflag;
         };
-        const int32_t vflag = cflag | (q.vi_pairs ? kItemPairs : 0);
         if (q.req.algorithm == TOMO_ALGO_SIRT) {
             add_projection(q.x, q.px, OP_ZPX_X, OP_ZPY_X, OP_SID_X, OP_RS_X);
             BatchItem& df = tb[OP_DIFF].add_vec(d, ns);
@@ -309,15 +308,11 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
             fr.n = vec_tiles(ns);
             fr.slot = q.slot;
             fr.flags = kSlotResidual;
-            BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 4, 64), cdiv(d.nv + 3, 4), d.nphi);
-            vq.in = q.r;
-            vq.out = q.tq;
-            vq.flags = cflag;
-            BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, kViTileX), cdiv(d.ny, kViTileY), cdiv(d.nz, kViTileZ));
-            vi.in = q.tq;
+            BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, kViStageX), cdiv(d.ny, kViStageY), cdiv(d.nz, kViTileZ));
+            vi.in = q.r;
+            vi.aux = q.tq;
             vi.out = q.dj;
             vi.a = 1.0f;
-            vi.flags = vflag;
             BatchItem& st = tb[OP_STEP].add_one(d);
             st.slot = q.slot;
             st.flags = 2;
@@ -384,15 +379,11 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
         fr.n = vec_tiles(ns);
         fr.slot = q.slot;
         fr.flags = kSlotResidual;
-        BatchItem& vq = tb[OP_VQ].add(d, cdiv(d.nu + 4, 64), cdiv(d.nv + 3, 4), d.nphi);
-        vq.in = q.r;
-        vq.out = q.tq;
-        vq.flags = cflag;
-        BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, kViTileX), cdiv(d.ny, kViTileY), cdiv(d.nz, kViTileZ));
-        vi.in = q.tq;
+        BatchItem& vi = tb[OP_VI].add(d, cdiv(d.nx, kViStageX), cdiv(d.ny, kViStageY), cdiv(d.nz, kViTileZ));
+        vi.in = q.r;
+        vi.aux = q.tq;
         vi.out = q.dj;
         vi.a = -2.0f;
-        vi.flags = vflag;
         if (lam) {
             BatchItem& ar = tb[OP_AXPY_REG].add_vec(d, nv);
             ar.in = q.t2;
@@ -466,7 +457,6 @@ int launch_round(tomo_server* srv, int set) {
     TOMO_STEP(launch_batch_axpy(T + OP_AXPY_LLX, s));
     TOMO_STEP(launch_batch_diff_sq(T + OP_DIFF, s));
     TOMO_STEP(launch_batch_finish(T + OP_FIN_RES, s));
-    TOMO_STEP(launch_batch_vi_quads(T + OP_VQ, s));
     TOMO_STEP(launch_batch_vi(T + OP_VI, s));
     TOMO_STEP(launch_batch_axpy(T + OP_AXPY_REG, s));
     TOMO_STEP(launch_batch_laplacian(T + OP_LAP_DJ, s));
