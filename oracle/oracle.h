@@ -62,11 +62,6 @@ void og_vi_T_f64(const og_geom* g, const double* vol, double* sino);
 /* FDK-weighted VI (back_fdk, K/BackProjector/FDK3D_kernel_16reg.cuh). */
 void og_vi_fdk_f32(const og_geom* g, const float* sino, float* vol);
 
-/* FP16-arithmetic VI (the mixed-precision form on FP16-stored sinograms;
- * oracle_mixed.c states the arithmetic).  sino holds binary16 values as
- * float; og_round_half rounds to binary16, ties to even.                    */
-double og_round_half(double x);
-void og_vi_h16(const og_geom* g, const float* sino, float* vol);
 void og_vi_fdk_f64(const og_geom* g, const double* sino, double* vol);
 
 /* RS projector; cube = 0: inscribed cylinder (K/Projector/

@@ -114,11 +114,6 @@ def _f64p(a):
     return a.ctypes.data_as(C.POINTER(C.c_double))
 
 
-def round_half(x: float) -> float:
-    """The oracle's binary16 rounding (ties to even) of one double."""
-    return Oracle.lib().og_round_half(float(x))
-
-
 class Oracle:
     _lib = None
 
@@ -134,7 +129,6 @@ class Oracle:
             fp, dp = C.POINTER(C.c_float), C.POINTER(C.c_double)
             for name, a in [("og_vi_f32", [G, fp, fp]), ("og_vi_f64", [G, dp, dp]),
                             ("og_vi_fdk_f32", [G, fp, fp]), ("og_vi_fdk_f64", [G, dp, dp]),
-                            ("og_vi_h16", [G, fp, fp]),
                             ("og_vi_T_f64", [G, dp, dp]),
                             ("og_rs_f32", [G, C.c_int, fp, fp]), ("og_rs_f64", [G, C.c_int, dp, dp]),
                             ("og_rs_T_f64", [G, C.c_int, dp, dp]),
@@ -142,8 +136,6 @@ class Oracle:
                             ("og_siddon_T_f64", [G, dp, dp])]:
                 getattr(lib, name).argtypes = a
                 getattr(lib, name).restype = None
-            lib.og_round_half.argtypes = [C.c_double]
-            lib.og_round_half.restype = C.c_double
             lib.og_rs_count.argtypes = [G, C.c_int]
             lib.og_rs_count.restype = C.c_uint64
             lib.og_siddon_count.argtypes = [G]
@@ -211,13 +203,6 @@ class Oracle:
             s = np.ascontiguousarray(sino, dtype=np.float32)
             out = self._vol(np.float32)
             self.lib().og_vi_f32(C.byref(self._g), _f32p(s), _f32p(out))
-        return out
-
-    def vi_h16(self, sino):
-        """FP16-arithmetic VI on a sinogram of binary16 values (as float32)."""
-        s = np.ascontiguousarray(sino, dtype=np.float32)
-        out = self._vol(np.float32)
-        self.lib().og_vi_h16(C.byref(self._g), _f32p(s), _f32p(out))
         return out
 
     def vi_fdk(self, sino, f64=False):

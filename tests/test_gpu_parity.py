@@ -134,35 +134,35 @@ def test_16bit_volume_bit_exact(setup, dt, op):
 
 
 # Detector rows of a quarter voxel at magnification 2: the row step A of a
-# column is about 4, so 16-bit VI reads one tap-column load per voxel.
+# column is about 4, so VI's detector boxes exceed the LDS staging buffer and
+# every view reads its taps directly.
 FINE_ROWS = Geometry(fod=200.0, fdd=400.0, nphi=12, su=40 * 0.5, sv=40 * 0.5, nu=40, nv=40, sx=24.0, sy=24.0,
                      sz=24.0, nx=24, ny=24, nz=24, num_start=0.0, den_start=1.0, num_stop=2.0, den_stop=1.0)
 
 
-def _vi16_case(g, o, G, dt):
-    """16-bit sinogram at storage scale 1 into an FP32 volume: FP16 against
-    the oracle's FP16-arithmetic VI, BF16 (FP32 arithmetic on the widened
-    taps) against the FP32 oracle, both on the rounded values, bit for bit."""
+def _vi16_case(g, o, G, dt, scale=1.0):
+    """16-bit sinogram stored as value * scale (a power of two) into an FP32
+    volume: VI widens the taps to FP32 (stored / scale, exact) and equals the
+    FP32 oracle on the widened values, bit for bit."""
     x = o.rs(ellipsoids(g.nx, g.ny, g.nz, count=12, seed=6, amin=2.0, amax=12.0))
-    s16 = dev(x).to(torch.float16 if dt == "f16" else torch.bfloat16)
-    rounded = s16.float().cpu().numpy()
-    ref = o.vi_h16(rounded) if dt == "f16" else o.vi(rounded)
+    s16 = (dev(x) * scale).to(torch.float16 if dt == "f16" else torch.bfloat16)
+    widened = (s16.double() / scale).float().cpu().numpy()
+    ref = o.vi(widened)
     out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
-    G.backproject(s16, out)
+    G.backproject(s16, out, sino_scale=scale)
     got = out.cpu().numpy()
     assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
 
 
-@pytest.mark.parametrize("dt", ["f16", "bf16"])
-def test_16bit_vi_bit_exact(setup, dt):
-    """Row steps below 2 (every geometry here): one load per voxel pair."""
+@pytest.mark.parametrize("dt,scale", [("f16", 1.0), ("bf16", 1.0), ("f16", 2.0 ** -6)])
+def test_16bit_vi_bit_exact(setup, dt, scale):
     g, o, G = setup
-    _vi16_case(g, o, G, dt)
+    _vi16_case(g, o, G, dt, scale)
 
 
 @pytest.mark.parametrize("dt", ["f16", "bf16"])
 def test_16bit_vi_bit_exact_fine_rows(ctx, dt):
-    """Row steps near 4: one load per voxel."""
+    """Every view read directly (boxes over the staging buffer)."""
     G = lib.Geometry(ctx, to_desc(FINE_ROWS))
     try:
         _vi16_case(FINE_ROWS, Oracle(FINE_ROWS), G, dt)

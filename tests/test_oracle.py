@@ -34,9 +34,6 @@ them retypes the oracle's own formula:
 * SIRT    - first iteration composed from the pinned operators (weights,
             relaxation, sign); exact homogeneity; convergence on consistent
             data.
-* FP16 VI - the binary16 rounding against IEEE (numpy); constant-sinogram
-            closed form (exact where every half operation is exact) and an
-            FP64 error bound.
 * weighted gradient - W = 1 is the simple gradient; W = 2 doubles the first
             iterate; a binary view mask acts on the residual before the
             backprojection.
@@ -45,7 +42,7 @@ import numpy as np
 import pytest
 
 import geom_ref as gr
-from oracle_ffi import (Geometry, Oracle, ellipsoid_table, ellipsoids, laplacian, round_half, scaled_geometry,
+from oracle_ffi import (Geometry, Oracle, ellipsoid_table, ellipsoids, laplacian, scaled_geometry,
                         shepp_logan)
 
 
@@ -102,45 +99,6 @@ def test_vi_constant_sinogram_closed_form(f64):
     exp *= gr.derived(g)["delta"]
     assert np.abs(f - exp).max() <= (1e-9 if f64 else 1e-5) * np.abs(exp).max()
     assert exp.max() > 0 and (exp == exp.max()).sum() > 10  # interior voxels see every view
-
-
-def test_round_half_is_ieee_binary16():
-    """og_round_half against numpy's binary16 conversion (round to nearest,
-    ties to even): random magnitudes over the whole half range and beyond,
-    subnormals, exact ties and the overflow threshold 65520."""
-    rng = np.random.default_rng(0)
-    xs = rng.standard_normal(20000) * 10.0 ** rng.integers(-9, 6, 20000)
-    edges = [65519.0, 65520.0, -65520.0, 65504.0, 2.0 ** -24, 2.0 ** -25, 3 * 2.0 ** -26, 2.0 ** -14,
-             1 + 2.0 ** -11, 1 + 3 * 2.0 ** -11, 2049.0, 2051.0, 0.0, -0.0, 1e-30]
-    with np.errstate(over="ignore"):
-        for x in np.concatenate([xs, edges]):
-            want = float(np.float16(x))
-            got = round_half(x)
-            assert got == want or (np.isinf(got) and np.isinf(want) and np.sign(got) == np.sign(want)), x
-
-
-def test_vi_h16_constant_sinogram_and_fp64_bound():
-    """FP16-arithmetic VI: on g = 1 every voxel with full bilinear support in
-    every view gets exactly delta * nphi (all half operations exact there) and
-    every other voxel the closed form within the half rounding of its
-    weights; on a smooth sinogram of half values it stays within rel-L2 1e-3
-    of the FP64 VI."""
-    g = small_geom(20, 12, nu=26, nv=14)
-    o = Oracle(g)
-    f = o.vi_h16(np.ones(g.sino_shape, dtype=np.float32))
-    exp = np.zeros(g.vol_shape)
-    for u, v in _projections(g):
-        exp += _support_weight(u, g.nu) * _support_weight(v, g.nv)
-    delta = gr.derived(g)["delta"]
-    full = exp == g.nphi
-    assert full.sum() > 10
-    assert np.array_equal(f[full], np.full(full.sum(), np.float32(np.float32(g.nphi) * np.float32(delta))))
-    assert np.abs(f - exp * delta).max() <= 2.0 ** -10 * g.nphi * delta
-    x = o.rs(ellipsoids(g.nx, g.ny, g.nz, count=6, seed=8, amin=2.0, amax=8.0))
-    xh = x.astype(np.float16).astype(np.float32)
-    ref = o.vi(xh.astype(np.float64), f64=True)
-    err = np.linalg.norm(o.vi_h16(xh) - ref) / np.linalg.norm(ref)
-    assert 0 < err <= 1e-3
 
 
 @pytest.mark.parametrize("f64", [True, False])

@@ -26,10 +26,9 @@ namespace tomo {
 constexpr int kBatchMax = 64;
 constexpr int kBatchGrid = 2048;
 constexpr int kVecTile = 4096;  // elements per tile of the element-wise ops
-// VI workgroup on 16-bit tap columns: kViTileX x kViTileY voxel columns of
-// kViTileZ voxels (a wave is 64 consecutive x columns, whose taps lie along
-// detector rows).  The LDS-staged VI of FP32 sinograms (single problems and
-// shared launches) takes kViStageX x kViStageY columns of kViTileZ voxels.
+// VI workgroup (single problems and shared launches): kViStageX x kViStageY
+// voxel columns of kViTileZ voxels, the taps staged in LDS.  Its transpose
+// takes kViTileX x kViTileY columns (a wave is 64 consecutive x columns).
 constexpr int kViTileX = 64, kViTileY = 4, kViTileZ = 16;
 constexpr int kViStageX = 32, kViStageY = 8;
 

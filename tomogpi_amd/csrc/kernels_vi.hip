@@ -9,9 +9,10 @@
 //   v = A z' + cv,   A = (FDD/dv) delta / w
 //   f = delta * sum_p bilinear(g_p, u, v)      (taps outside the detector = 0)
 // The output is overwritten.  As in the reference's "16reg" blocking, a
-// thread owns a column of ZB voxels that share u and A; the per-(column,view)
+// thread owns a column of 16 voxels that share u and A; the per-(column,view)
 // terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
-// and the per-voxel v, bilinear weights and sum in FP32.
+// and the per-voxel v, bilinear weights and sum in FP32.  A 16-bit sinogram is
+// widened to FP32 (value = stored / scale) before the same kernel.
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -23,7 +24,7 @@
 namespace tomo {
 namespace {
 
-constexpr int TX = kViTileX, TY = kViTileY, ZB = kViTileZ;
+constexpr int TX = kViTileX, TY = kViTileY, ZB = kViTileZ;  // VI^T workgroup
 
 struct ColumnView {
     float u, A;
@@ -59,269 +60,6 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
         r.A = 0.0f;
     }
     return r;
-}
-
-// Tap-quad copy of an FP32 sinogram: element (p, vv, uu), vv in [0, nv+2],
-// uu in [0, nu+3], holds the detector values (u, v) and (u, v+1) with
-// u = uu-2, v = vv-2 (0 off the detector).  Two neighbouring elements are the
-// four bilinear taps of the cell with lower corner (u, v), so one 16-byte
-// load serves one voxel-view update.  A 16-bit sinogram gets the tap-column
-// copy instead: element (p, vv, uu) holds (u, v..v+3), and two neighbouring
-// elements (16 bytes) are the taps of columns u, u+1 on rows v..v+3, enough
-// for two voxels of a column whose rows differ by at most 2.  Cells with a
-// corner in [-2, nu] x [-2, nv] are in the copies; a cell whose corner is
-// outside [-1, n-1] on either axis has no tap on the detector, and clamping
-// its corner into [-2, n] lands on another such cell, all zeros, which
-// replaces the per-tap range tests.  Values are copied as stored.
-struct QuadShape {
-    uint32_t row;        // elements per row of the copy: per (nu + 4)
-    uint32_t view;       // elements per view: row (nv + 3)
-};
-__host__ __device__ __forceinline__ QuadShape quad_shape(int nu, int nv, uint32_t per) {
-    QuadShape q;
-    q.row = per * (uint32_t)(nu + 4);
-    q.view = q.row * (uint32_t)(nv + 3);
-    return q;
-}
-
-template <typename TS>
-__device__ __forceinline__ void vi_quads_tile(const TS* __restrict__ in, TS* __restrict__ out, int nu, int nv, int bx,
-                                              int by, int p) {
-    const int uu = bx * 64 + (threadIdx.x & 63);
-    const int vv = by * 4 + (threadIdx.x >> 6);
-    if (uu > nu + 3 || vv > nv + 2) return;
-    const int u = uu - 2, v = vv - 2;
-    const bool uok = u >= 0 && u < nu;
-    const TS* gp = in + (size_t)p * nu * nv;
-    TS a, b;
-    __builtin_memset(&a, 0, sizeof(a));
-    __builtin_memset(&b, 0, sizeof(b));
-    if (uok && v >= 0 && v < nv) a = gp[(size_t)v * nu + u];
-    if (uok && v + 1 >= 0 && v + 1 < nv) b = gp[(size_t)(v + 1) * nu + u];
-    const QuadShape qs = quad_shape(nu, nv, 2);
-    const size_t o = (size_t)p * qs.view + (size_t)vv * qs.row + 2u * (uint32_t)uu;
-    out[o] = a;
-    out[o + 1] = b;
-}
-
-// Tap-column copy of a 16-bit sinogram (same grid as the tap quads).
-__device__ __forceinline__ void vi_columns_tile(const uint16_t* __restrict__ in, uint16_t* __restrict__ out, int nu,
-                                                int nv, int bx, int by, int p) {
-    const int uu = bx * 64 + (threadIdx.x & 63);
-    const int vv = by * 4 + (threadIdx.x >> 6);
-    if (uu > nu + 3 || vv > nv + 2) return;
-    const int u = uu - 2, v = vv - 2;
-    const bool uok = u >= 0 && u < nu;
-    const uint16_t* gp = in + (size_t)p * nu * nv;
-    uint32_t t[4];
-#pragma unroll
-    for (int r = 0; r < 4; ++r) t[r] = (uok && v + r >= 0 && v + r < nv) ? gp[(size_t)(v + r) * nu + u] : 0u;
-    const QuadShape qs = quad_shape(nu, nv, 4);
-    const uint2 w = make_uint2(t[0] | (t[1] << 16), t[2] | (t[3] << 16));
-    __builtin_memcpy(out + (size_t)p * qs.view + (size_t)vv * qs.row + 4u * (uint32_t)uu, &w, sizeof(w));
-}
-
-// Raw taps of a later voxel of a group from the group's tap-column load:
-// rows d, d+1 of columns u (words c0 = rows 0,1 and c1 = rows 2,3) and u+1
-// (e0, e1), d in {0, 1, 2} its clamped row less the first voxel's.  When the
-// voxels' v differ by less than 2 (A below 1.99 for neighbours, below 0.995
-// two apart), their floors (and the clamped rows) differ by 0, 1 or 2.
-__device__ __forceinline__ void pair_rows(float d, uint32_t c0, uint32_t c1, uint32_t e0, uint32_t e1, uint32_t& b0,
-                                          uint32_t& b1) {
-    const uint32_t cm = __builtin_amdgcn_alignbyte(c1, c0, 2), em = __builtin_amdgcn_alignbyte(e1, e0, 2);
-    b0 = d == 0.0f ? c0 : (d == 1.0f ? cm : c1);
-    b1 = d == 0.0f ? e0 : (d == 1.0f ? em : e1);
-}
-
-// Four words of a tap-column load (the non-buffer path's stand-in for the
-// buffer load's vector).
-struct w4 {
-    uint32_t v[4];
-    __device__ __forceinline__ uint32_t operator[](int i) const { return v[i]; }
-};
-
-// The 16 voxels of a column for one view on 16-bit tap columns, GROUP
-// voxels per load: the group's first voxel reads rows 0,1 of the load, the
-// others rows d, d+1 with d their clamped row less the first's (0..2).
-// col4 / col3 load 4 / 3 words at a clamped row; term16 adds one term.
-template <int GROUP, typename C4, typename C3, typename T16, typename CV>
-__device__ __forceinline__ void vi_column16(const C4& col4, const C3& col3, const T16& term16, const CV& cvw,
-                                            __half2 fu2, float fu, float zp0, float cv, float fnv) {
-    constexpr int kFull = GROUP == 1 ? 0 : ZB / GROUP * GROUP;
-#pragma unroll
-    for (int k = 0; k < kFull; k += GROUP) {
-        const float va = fmaf(zp0 + (float)k, cvw.A, cv);
-        const float vfa = floorf(va);
-        const float vca = __builtin_amdgcn_fmed3f(vfa, -2.0f, fnv);
-        const auto w = col4(vca);
-        term16(cvw, fu2, fu, k, va, vfa, w[0], w[2]);
-#pragma unroll
-        for (int j = 1; j < GROUP; ++j) {
-            const float vb = fmaf(zp0 + (float)(k + j), cvw.A, cv);
-            const float vfb = floorf(vb);
-            const float vcb = __builtin_amdgcn_fmed3f(vfb, -2.0f, fnv);
-            uint32_t b0, b1;
-            pair_rows(vcb - vca, w[0], w[1], w[2], w[3], b0, b1);
-            term16(cvw, fu2, fu, k + j, vb, vfb, b0, b1);
-        }
-    }
-#pragma unroll
-    for (int k = kFull; k < ZB; ++k) {
-        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-        const float vf = floorf(v);
-        const auto w = col3(__builtin_amdgcn_fmed3f(vf, -2.0f, fnv));
-        term16(cvw, fu2, fu, k, v, vf, w[0], w[2]);
-    }
-}
-
-// Forward VI on the tap-quad copy (FP32) or the tap-column copy (16-bit).  Bilinear interpolation at (u, v): the
-// taps (u,v), (u,v+1), (u+1,v), (u+1,v+1) of the cell with lower corner
-// (floor u, floor v), 0 off the detector, interpolated along u, then v, with
-// explicit fmaf.  The cell corner is clamped into the copy (a cell off the
-// detector reads four zeros and adds +0).  FAST (copy below 4 GiB, one view
-// below 16 MiB): buffer loads at a byte offset computed as one exact FP32
-// fma of small integers; otherwise element indices in integers.
-// VI stores delta * sum; FDK weights each term by W = num / sqrt(fma(v', v',
-// base)), v' = z' A, and stores the sum itself.  16-bit storage reads the
-// taps of GROUP consecutive voxels of the column with one tap-column load:
-// GROUP 2 when the host has shown A in [0, 1.99] for every column and view
-// (rows of voxels k, k+1 differ by at most 2), GROUP 3 when A is in
-// [0, 0.995] (rows of k, k+2 differ by at most 2), else 1; the voxels left
-// after the last full group load alone.  The taps and the arithmetic of each
-// term are the same for every GROUP.
-template <typename TS, typename TV, bool FDK, int GROUP = 1>
-__device__ __forceinline__ void vi_tile(const DevGeom& g, const TS* __restrict__ quads, float sino_inv,
-                                        TV* __restrict__ vol, float vol_scale, float post, int bx, int by, int bz) {
-    const int i = bx * TX + (threadIdx.x % TX);
-    const int j = by * TY + (threadIdx.x / TX);
-    const int k0 = bz * ZB;
-    if (i >= g.nx || j >= g.ny) return;
-    const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
-    const float zp0 = (float)((double)k0 - (double)g.cz);
-    const float cv = g.cv;
-    float acc[ZB];
-#pragma unroll
-    for (int k = 0; k < ZB; ++k) acc[k] = 0.0f;
-    constexpr bool kF16 = std::is_same<TS, __half>::value;
-    constexpr bool kBF16 = std::is_same<TS, __hip_bfloat16>::value;
-    constexpr uint32_t kPer = (kF16 || kBF16) ? 4u : 2u;  // values per element: tap columns / tap quads
-    const QuadShape qs = quad_shape(g.nu, g.nv, kPer);
-    const float fnu = (float)g.nu, fnv = (float)g.nv;
-    constexpr uint32_t es = sizeof(TS);
-    const uint64_t view_bytes = (uint64_t)qs.view * es;
-    const bool fast = view_bytes < (1ull << 24) && view_bytes * (uint64_t)g.nphi < (1ull << 32);
-    // one voxel-view term from its interpolated value r
-    auto add = [&](const ColumnView& cvw, int k, float r) {
-        if (FDK) {
-            const float vp = (zp0 + (float)k) * cvw.A;
-            const float wt = cvw.num / sqrtf(fmaf(vp, vp, cvw.base));
-            acc[k] += wt * r;
-        } else {
-            acc[k] += r;
-        }
-    };
-    // FP32 taps (FP32 storage: the oracle's arithmetic)
-    auto term = [&](const ColumnView& cvw, float fu, int k, float v, float vf, const float4& t) {
-        const float fv = v - vf;
-        const float a = fmaf(fu, t.z - t.x, t.x);
-        const float b = fmaf(fu, t.w - t.y, t.y);
-        add(cvw, k, fmaf(fv, b - a, a));
-    };
-    // 16-bit storage, raw taps (g(u,v), g(u,v+1)) in w0 and (g(u+1,v),
-    // g(u+1,v+1)) in w1; the storage scale is applied once at the store.
-    // FP16: both rows interpolated along u in one packed FP16 fma, the v
-    // step in FP32 (error well inside the mixed-form bound; accumulation
-    // stays FP32).  BF16: FP32 interpolation of the widened taps.
-    auto term16 = [&](const ColumnView& cvw, __half2 fu2, float fu, int k, float v, float vf, uint32_t w0,
-                      uint32_t w1) {
-        const float fv = v - vf;
-        if constexpr (kF16) {
-            const __half2 p0 = __builtin_bit_cast(__half2, w0), p1 = __builtin_bit_cast(__half2, w1);
-            const __half2 ab = __hfma2(fu2, __hsub2(p1, p0), p0);
-            const __half a = __low2half(ab);
-            add(cvw, k, fmaf(fv, __half2float(__hsub(__high2half(ab), a)), __half2float(a)));
-        } else {
-            const float t00 = __uint_as_float(w0 << 16), t10 = __uint_as_float(w0 & 0xffff0000u);
-            const float t01 = __uint_as_float(w1 << 16), t11 = __uint_as_float(w1 & 0xffff0000u);
-            const float a = fmaf(fu, t01 - t00, t00);
-            const float b = fmaf(fu, t11 - t10, t10);
-            add(cvw, k, fmaf(fv, b - a, a));
-        }
-    };
-    if (fast) {
-        const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(quads, (uint32_t)(view_bytes * (uint64_t)g.nphi));
-        const float row_b = (float)(qs.row * es);
-        for (int p = 0; p < g.nphi; ++p) {
-            const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
-            const float uf = floorf(cvw.u);
-            const float fu = cvw.u - uf;
-            const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
-            // byte offset of cell (uc, vc): ((vc + 2) row + per (uc + 2)) es
-            const float base = fmaf(uc + 2.0f, (float)(kPer * es), 2.0f * row_b);
-            const uint32_t soff = (uint32_t)p * (uint32_t)view_bytes;
-            if constexpr (kF16 || kBF16) {
-                const __half2 fu2 = __float2half2_rn(fu);
-                const auto col4 = [&](float vc) {
-                    return __builtin_amdgcn_raw_buffer_load_b128(rsrc, (int)(uint32_t)fmaf(vc, row_b, base),
-                                                                 (int)soff, 0);
-                };
-                const auto col3 = [&](float vc) {
-                    return __builtin_amdgcn_raw_buffer_load_b96(rsrc, (int)(uint32_t)fmaf(vc, row_b, base),
-                                                                (int)soff, 0);
-                };
-                vi_column16<GROUP>(col4, col3, term16, cvw, fu2, fu, zp0, cv, fnv);
-            } else {
-#pragma unroll
-                for (int k = 0; k < ZB; ++k) {
-                    const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                    const float vf = floorf(v);
-                    const float vc = __builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                    const uint32_t off = (uint32_t)fmaf(vc, row_b, base);
-                    term(cvw, fu, k, v, vf, BufLoad4<TS>::load(rsrc, off, soff, sino_inv));
-                }
-            }
-        }
-    } else {
-        for (int p = 0; p < g.nphi; ++p) {
-            const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
-            const float uf = floorf(cvw.u);
-            const float fu = cvw.u - uf;
-            const int uc = (int)__builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
-            const TS* gp = quads + (size_t)p * qs.view + kPer * (uint32_t)(uc + 2);
-            if constexpr (kF16 || kBF16) {
-                const __half2 fu2 = __float2half2_rn(fu);
-                const auto col4 = [&](float vc) {
-                    uint32_t w[4];
-                    __builtin_memcpy(w, gp + (uint64_t)(uint32_t)((int)vc + 2) * qs.row, sizeof(w));
-                    return w4{w[0], w[1], w[2], w[3]};
-                };
-                const auto col3 = [&](float vc) {
-                    uint32_t w[3];
-                    __builtin_memcpy(w, gp + (uint64_t)(uint32_t)((int)vc + 2) * qs.row, sizeof(w));
-                    return w4{w[0], w[1], w[2], 0u};
-                };
-                vi_column16<GROUP>(col4, col3, term16, cvw, fu2, fu, zp0, cv, fnv);
-            } else {
-#pragma unroll
-                for (int k = 0; k < ZB; ++k) {
-                    const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                    const float vf = floorf(v);
-                    const int vc = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, fnv);
-                    const uint64_t e = (uint64_t)(uint32_t)(vc + 2) * qs.row;
-                    term(cvw, fu, k, v, vf, Storage<TS>::load4(gp, e, sino_inv));
-                }
-            }
-        }
-    }
-    const size_t sxy = (size_t)g.nx * g.ny;
-    const size_t base = (size_t)i + (size_t)g.nx * j;
-    const float in_scale = (kF16 || kBF16) ? sino_inv : 1.0f;  // 16-bit taps are read unscaled
-#pragma unroll
-    for (int k = 0; k < ZB; ++k) {
-        const float a = (kF16 || kBF16) ? acc[k] * in_scale : acc[k];
-        if (k0 + k < g.nz)
-            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? a : (a * g.delta) * post, vol_scale);
-    }
 }
 
 // ---------------------------------------------------------------------------
@@ -573,21 +311,6 @@ __global__ __launch_bounds__(SX* SY, 6) void vi_staged_kernel(DevGeom g, const f
     vi_staged_tile<TV, FDK, ZS>(g, sino, boxes, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z, stage);
 }
 
-template <typename TS>
-__global__ __launch_bounds__(256) void vi_quads_kernel(const TS* __restrict__ in, TS* __restrict__ out, int nu,
-                                                     int nv) {
-    if constexpr (sizeof(TS) == 2)
-        vi_columns_tile((const uint16_t*)in, (uint16_t*)out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
-    else
-        vi_quads_tile<TS>(in, out, nu, nv, blockIdx.x, blockIdx.y, blockIdx.z);
-}
-
-template <typename TS, typename TV, bool FDK, int GROUP>
-__global__ __launch_bounds__(TX* TY) void vi_kernel(DevGeom g, const TS* __restrict__ quads, float sino_inv,
-                                                  TV* __restrict__ vol, float vol_scale) {
-    vi_tile<TS, TV, FDK, GROUP>(g, quads, sino_inv, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z);
-}
-
 // Shared launch (FP32 sinograms and volumes): tile grid (ceil(nx/SX),
 // ceil(ny/SY), ceil(nz/16)) on the sinogram `in` with the problem's box table
 // `aux` (launch_vi_boxes); the stored value is (delta * sum) * a.
@@ -650,43 +373,6 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 
 }  // namespace
 
-// Voxels per 16-bit tap-column load the geometry allows: 3 when every
-// column-view's row step A = delta FDD/dv / w is in [0, 0.99], 2 when it is
-// in [0, 1.98], else 1 (the kernel's bounds 0.995 and 1.99 less a margin far
-// above A's FP32 rounding).  w = delta (cos x' + sin y') + FOD is linear in
-// (x', y'), so its minimum over the voxel centres is at a corner of their
-// rectangle.
-int vi_row_group(const tomo_geom* g) {
-    const DevGeom& d = g->dev;
-    const double xs[2] = {-(double)d.cx, (double)(d.nx - 1) - (double)d.cx};
-    const double ys[2] = {-(double)d.cy, (double)(d.ny - 1) - (double)d.cy};
-    double amax = 0.0;
-    for (int p = 0; p < d.nphi; ++p) {
-        const double* c = &g->view[(size_t)p * kViewStride];
-        double wmin = 1.0e300;
-        for (double x : xs)
-            for (double y : ys) wmin = std::min(wmin, std::fma(c[2], x, std::fma(c[3], y, d.fod)));
-        if (!(wmin > 0.0) || !(c[6] >= 0.0)) return 1;
-        amax = std::max(amax, c[6] / wmin);
-    }
-    return amax <= 0.99 ? 3 : (amax <= 1.98 ? 2 : 1);
-}
-
-template <typename TS, typename TV, bool FDK>
-static void launch_vi_kernel(const DevGeom& d, dim3 grid, int group, const void* quads, float inv, void* vol,
-                             float vol_scale, hipStream_t s) {
-    constexpr bool k16 = sizeof(TS) == 2;
-    if (k16 && group == 3)
-        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, k16 ? 3 : 1>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads,
-                           inv, (TV*)vol, vol_scale);
-    else if (k16 && group == 2)
-        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, k16 ? 2 : 1>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads,
-                           inv, (TV*)vol, vol_scale);
-    else
-        hipLaunchKernelGGL((vi_kernel<TS, TV, FDK, 1>), grid, dim3(TX * TY), 0, s, d, (const TS*)quads, inv,
-                           (TV*)vol, vol_scale);
-}
-
 // Voxels per thread column of the staged kernel.
 constexpr int kStagedZ = kViTileZ;
 
@@ -715,38 +401,34 @@ static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino,
 
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) {
     const DevGeom& d = g->dev;
-    if (sino.dtype == TOMO_F32) {
-        // FP32 sinograms: LDS-staged taps read from the sinogram itself.
-        hipError_t e = hipSuccess;
-        TOMO_DISPATCH_STORAGE(vol.dtype, TV, { e = launch_vi_staged<TV>(d, fdk, (const float*)sino.p, vol.p,
-                                                                        vol.scale, s); });
-        TOMO_HIP(e);
-        return TOMO_OK;
-    }
-    // 16-bit sinograms: tap-column copy from the stream-ordered pool,
-    // released after the backprojection on the same stream.
-    void* quads = nullptr;
-    TOMO_HIP(hipMallocAsync(&quads, vi_quad_elems(d.nu, d.nv, d.nphi) * 4, s));
-    const dim3 qgrid(ceil_div(d.nu + 4, 64), ceil_div(d.nv + 3, 4), d.nphi);
-    const dim3 grid(ceil_div(d.nx, TX), ceil_div(d.ny, TY), ceil_div(d.nz, ZB));
-    const float inv = 1.0f / sino.scale;
-    const int group = vi_row_group(g);
-    hipError_t le = hipSuccess;
-    TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-        hipLaunchKernelGGL((vi_quads_kernel<TS>), qgrid, dim3(256), 0, s, (const TS*)sino.p, (TS*)quads, d.nu, d.nv);
-        le = hipGetLastError();
-        if (le == hipSuccess) {
-            TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
-                if (fdk)
-                    launch_vi_kernel<TS, TV, true>(d, grid, group, quads, inv, vol.p, vol.scale, s);
-                else
-                    launch_vi_kernel<TS, TV, false>(d, grid, group, quads, inv, vol.p, vol.scale, s);
-            });
-            le = hipGetLastError();
+    if (sino.dtype != TOMO_F32 && sino.dtype != TOMO_F16 && sino.dtype != TOMO_BF16)
+        return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");
+    // A 16-bit sinogram is widened (value = stored / scale) into an FP32
+    // copy from the stream-ordered pool, released after the backprojection
+    // on the same stream: the staged kernel is bound by VALU issue, so 16-bit
+    // taps would not make it faster.
+    const float* taps = (const float*)sino.p;
+    void* wide = nullptr;
+    if (sino.dtype != TOMO_F32) {
+        TOMO_HIP(hipMallocAsync(&wide, g->nsino() * sizeof(float), s));
+        const int rc = launch_convert(sino, MutBuf{wide, TOMO_F32, 1.0f}, g->nsino(), s);
+        if (rc) {
+            (void)hipFreeAsync(wide, s);
+            return rc;
         }
-    });
-    TOMO_HIP(hipFreeAsync(quads, s));
-    TOMO_HIP(le);
+        taps = (const float*)wide;
+    }
+    hipError_t e = hipSuccess;
+    switch (vol.dtype) {
+        case TOMO_F32: e = launch_vi_staged<float>(d, fdk, taps, vol.p, vol.scale, s); break;
+        case TOMO_F16: e = launch_vi_staged<__half>(d, fdk, taps, vol.p, vol.scale, s); break;
+        case TOMO_BF16: e = launch_vi_staged<__hip_bfloat16>(d, fdk, taps, vol.p, vol.scale, s); break;
+        default:
+            if (wide) (void)hipFreeAsync(wide, s);
+            return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");
+    }
+    if (wide) TOMO_HIP(hipFreeAsync(wide, s));
+    TOMO_HIP(e);
     return TOMO_OK;
 }
 

@@ -74,15 +74,6 @@ namespace tomo {
 __host__ __device__ inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
 __host__ __device__ inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
 
-// Voxels per 16-bit tap-column load VI may read for every column and view
-// of g: 3 (row steps in [0, 0.99]), 2 (in [0, 1.98]) or 1.
-int vi_row_group(const tomo_geom* g);
-
-// Elements of the VI tap-quad copy of a sinogram: nphi views of (nv + 3)
-// rows of (nu + 4) pairs; its fill kernel's grid is (ceil((nu+4)/64),
-// ceil((nv+3)/4), nphi).
-inline size_t vi_quad_elems(int nu, int nv, int nphi) { return (size_t)nphi * (nv + 3) * (nu + 4) * 2; }
-
 // Derives the geometry exactly as the reference constructors do.
 int derive_geometry(const tomo_geom_desc& d, tomo_geom* g);
 
