@@ -294,21 +294,31 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
             const float Vb = bneg ? (ZNEG ? v.y : v.x) : (ZNEG ? v.w : v.z);
             const float Vz = bneg ? (ZNEG ? v.z : v.w) : (ZNEG ? v.x : v.y);
             const float Vbz = bneg ? (ZNEG ? v.x : v.y) : (ZNEG ? v.z : v.w);
-            const bool hb = tB < tA, hz = tz < tA;
-            const float tb = hb ? tB : tA, tzz = hz ? tz : tA;
-            const float e1 = min_nn(tb, tzz), e2 = max_nn(tb, tzz);
-            const float V1 = tb < tzz ? Vb : (tzz < tb ? Vz : Vbz);
-            const float V2 = hb ? (hz ? Vbz : Vb) : (hz ? Vz : V0);
-            const float c1 = min_nn(e1, r.tmax), c2 = min_nn(e2, r.tmax), c3 = min_nn(tA, r.tmax);
+            // Segment ends: c3 = min(tA, tmax) closes the slab; the interior
+            // crossings are c1 = min(tB, tz, c3) and c2 = min(max(tB, tz), c3)
+            // (min(x, tA) of each crossing, in order, clipped to tmax).  The
+            // middle segment's voxel is Vb when B is crossed first and Vz
+            // when z is; the last one's is the voxel after the crossings
+            // taken (tB <= tA: B stepped).  Where these choices differ from
+            // the general step's (a tie, or a crossing exactly at tA) the
+            // segment has zero length and adds exactly 0.
+            const bool stepB = tB <= tA, stepZ = tz <= tA;
+            const float V1 = tB < tz ? Vb : Vz;
+            const float V2 = stepB ? (stepZ ? Vbz : Vb) : (stepZ ? Vz : V0);
+            const float c3 = min_nn(tA, r.tmax);
+            const float c1 = min_nn(min_nn(tB, tz), c3), c2 = min_nn(max_nn(tB, tz), c3);
             acc = fmaf(c1 - t, V0, acc);
             acc = fmaf(c2 - c1, V1, acc);
             acc = fmaf(c3 - c2, V2, acc);
             t = c3;
-            const bool stepB = hb || tB == tA, stepZ = hz || tz == tA;
-            if (stepB) { e += dB; cB += 1.0f; tB = fmaf(cB, r.inc[B], r.first[B]); }
-            if (stepZ) { e += dZ; cz += 1.0f; tz = fmaf(cz, r.inc[2], r.first[2]); }
+            // the next crossing times follow from the counts (an axis not
+            // stepped recomputes the same time)
+            e += (stepB ? dB : 0u) + (stepZ ? dZ : 0u) + dA;
+            cB += stepB ? 1.0f : 0.0f;
+            cz += stepZ ? 1.0f : 0.0f;
+            tB = fmaf(cB, r.inc[B], r.first[B]);
+            tz = fmaf(cz, r.inc[2], r.first[2]);
             iA += sA;
-            e += dA;
             cA += 1.0f;
             tA = fmaf(cA, r.inc[A], r.first[A]);
             live = t < r.tmax;
