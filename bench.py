@@ -491,10 +491,10 @@ def main():
     dom = "vi" if vi_ms >= sid_ms else "siddon"
     dom_bytes = b_vi if dom == "vi" else b_sid
     dom_ms = vi_ms if dom == "vi" else sid_ms
-    dom_kernel = "vi_kernel" if dom == "vi" else "siddon_kernel"
+    dom_kernel = "vi_staged_kernel" if dom == "vi" else "siddon_kernel"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
     measured = load_traffic(n, nphi)
-    for name, key in (("siddon", "siddon_kernel"), ("vi", "vi_kernel"), ("rs", "rs_kernel")):
+    for name, key in (("siddon", "siddon_kernel"), ("vi", "vi_staged_kernel"), ("rs", "rs_kernel")):
         if key in measured:
             ops[name]["hbm_traffic_bytes"] = measured[key]
     roofline = {"bound": "hbm", "kernel": dom_kernel,
