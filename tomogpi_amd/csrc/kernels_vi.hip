@@ -156,13 +156,25 @@ struct StagePlan {
     bool fits;
 };
 
+// The box rows from the four (z', A) corners; with A >= 0 (a wave-uniform
+// sign test on the table's bits) the lowest row is at z' = zp0 with the
+// largest A when zp0 < 0 (the smallest otherwise) and the highest at z' = zp1
+// with the largest A when zp1 > 0, the same values with two fmas.
 template <int ZS>
-__device__ __forceinline__ StagePlan stage_plan(const ViBox& bb, float zp0, float cv) {
+__device__ __forceinline__ StagePlan stage_plan(const ViBox& bb, float zp0, float cv, bool lo_amax, bool hi_amax) {
     const float zp1 = zp0 + (float)(ZS - 1);
-    const float va = fmaf(zp0, bb.amin, cv), vb = fmaf(zp0, bb.amax, cv);
-    const float vc = fmaf(zp1, bb.amin, cv), vd = fmaf(zp1, bb.amax, cv);
-    const float vlo = floorf(fminf(fminf(va, vb), fminf(vc, vd)));
-    const float vhi = floorf(fmaxf(fmaxf(va, vb), fmaxf(vc, vd)));
+    float vlo, vhi;
+    if (__float_as_int(bb.amin) >= 0) {
+        vlo = fmaf(zp0, lo_amax ? bb.amax : bb.amin, cv);
+        vhi = fmaf(zp1, hi_amax ? bb.amax : bb.amin, cv);
+    } else {
+        const float va = fmaf(zp0, bb.amin, cv), vb = fmaf(zp0, bb.amax, cv);
+        const float vc = fmaf(zp1, bb.amin, cv), vd = fmaf(zp1, bb.amax, cv);
+        vlo = fminf(fminf(va, vb), fminf(vc, vd));
+        vhi = fmaxf(fmaxf(va, vb), fmaxf(vc, vd));
+    }
+    vlo = floorf(vlo);
+    vhi = floorf(vhi);
     StagePlan s;
     s.u0 = bb.u0;
     s.cols = bb.u1 - bb.u0 + 2;
@@ -219,28 +231,30 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     };
     // Copy of view p's box into buffer b: wave wv copies box rows wv + q NW,
     // lane l column l, straight from memory into LDS (buffer_load ... lds).
-    // A tap off the detector or beyond the box is read at an offset past the
-    // view's end, which writes 0.
-    const uint32_t view_bytes = (uint32_t)(plane * sizeof(float));
+    // Each row is read through a buffer resource of that detector row (of no
+    // bytes for a row off the detector), so a tap off the detector reads past
+    // the resource's end, which writes 0; the lane offsets are the same for
+    // every row.
     auto copy_box = [&](const StagePlan& s, int p, int b) {
         if (!s.fits) return;
-        const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(sino + (size_t)p * plane, view_bytes);
-        const int u = s.u0 + lane;
-        const bool uin = lane < s.cols && (unsigned)u < (unsigned)nu;
-        const int vw = s.v0 + wv;
-        const uint32_t base = ((uint32_t)vw * (uint32_t)nu + (uint32_t)u) * 4u;  // mod 2^32, exact when in range
+        const float* view = sino + (size_t)p * plane;
+        const uint32_t loff = (uint32_t)(s.u0 + lane) * 4u;  // past the row's end unless 0 <= u < nu
         float* dst = stage + b * SS::taps + wv * kStagePitch;
 #pragma unroll
         for (int q = 0; q < kSlots; ++q) {
             if (wv + q * NW < s.rows) {
-                const bool in = uin && (unsigned)(vw + q * NW) < (unsigned)nv;
-                const uint32_t off = in ? base + (uint32_t)(q * NW) * (uint32_t)nu * 4u : 0xfffffff0u;
+                const int v = s.v0 + wv + q * NW;
+                const bool vin = (unsigned)v < (unsigned)nv;
+                const __amdgpu_buffer_rsrc_t rr =
+                    buffer_rsrc(view + (size_t)(vin ? v : 0) * nu, vin ? (uint32_t)nu * 4u : 0u);
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
-                    rsrc, (__attribute__((address_space(3))) void*)(dst + q * NW * kStagePitch), 4, off, 0, 0, 0);
+                    rr, (__attribute__((address_space(3))) void*)(dst + q * NW * kStagePitch), 4, loff, 0, 0, 0);
             }
         }
     };
-    StagePlan nxt = stage_plan<ZS>(bt[0], zp0, cv);
+    const bool lo_amax = zp0 < 0.0f, hi_amax = zp0 + (float)(ZS - 1) > 0.0f;
+    const uint32_t view_bytes = (uint32_t)(plane * sizeof(float));
+    StagePlan nxt = stage_plan<ZS>(bt[0], zp0, cv, lo_amax, hi_amax);
     copy_box(nxt, 0, 0);
     for (int p = 0; p < g.nphi; ++p) {
         // this wave's copies of view p have landed; every wave is done with view p - 1
@@ -248,7 +262,7 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
         __syncthreads();
         const StagePlan now = nxt;
         if (p + 1 < g.nphi) {
-            nxt = stage_plan<ZS>(bt[p + 1], zp0, cv);
+            nxt = stage_plan<ZS>(bt[p + 1], zp0, cv, lo_amax, hi_amax);
             copy_box(nxt, p + 1, (p + 1) & 1);
         }
         if (active) {
