@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "config.hpp"
+#include "batch.hpp"
 #include "tomo_internal.hpp"
 
 namespace tomo {
@@ -104,6 +105,11 @@ int upload_geometry(tomo_geom* g) {
     TOMO_HIP(hipMemcpy(g->d_view_f, g->view_f.data(), sizeof(float) * g->view_f.size(), hipMemcpyHostToDevice));
     g->dev.view = g->d_view;
     g->dev.view_f = g->d_view_f;
+    // the VI box table depends on the geometry only: made once, here
+    TOMO_HIP(hipMalloc(&g->d_vi_boxes, vi_box_count(g->dev) * sizeof(ViBox)));
+    const int rc = launch_vi_boxes(g->dev, g->d_vi_boxes, nullptr);
+    if (rc) return rc;
+    TOMO_HIP(hipStreamSynchronize(nullptr));
     return TOMO_OK;
 }
 
@@ -204,6 +210,7 @@ int tomo_geom_create(tomo_ctx* ctx, const tomo_geom_desc* desc, tomo_geom** out)
     if ((rc = upload_geometry(g.get()))) {
         if (g->d_view) (void)hipFree(g->d_view);
         if (g->d_view_f) (void)hipFree(g->d_view_f);
+        if (g->d_vi_boxes) (void)hipFree(g->d_vi_boxes);
         return rc;
     }
     *out = g.release();
@@ -215,6 +222,7 @@ int tomo_geom_destroy(tomo_geom* g) {
     if (g->ctx) (void)hipSetDevice(g->ctx->device);
     if (g->d_view) (void)hipFree(g->d_view);
     if (g->d_view_f) (void)hipFree(g->d_view_f);
+    if (g->d_vi_boxes) (void)hipFree(g->d_vi_boxes);
     delete g;
     return TOMO_OK;
 }

@@ -390,15 +390,21 @@ __global__ __launch_bounds__(TX* TY) void vi_adjoint_kernel(DevGeom g, const TV*
 // Voxels per thread column of the staged kernel.
 constexpr int kStagedZ = kViTileZ;
 
+// The staged kernel on an FP32 sinogram, with the geometry's box table
+// (`boxes`) or, when null, one made for this call from the stream-ordered
+// pool.
 template <typename TV>
-static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino, void* vol, float vol_scale,
-                                   hipStream_t s) {
+static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino, const void* boxes, void* vol,
+                                   float vol_scale, hipStream_t s) {
     const int ntx = ceil_div(d.nx, SX), nty = ceil_div(d.ny, SY);
-    void* boxes = nullptr;
-    hipError_t e = hipMallocAsync(&boxes, vi_box_count(d) * sizeof(ViBox), s);
-    if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(vi_box_kernel, dim3((unsigned)vi_box_count(d)), dim3(SX * SY), 0, s, d, (ViBox*)boxes, ntx);
-    e = hipGetLastError();
+    void* own = nullptr;
+    hipError_t e = hipSuccess;
+    if (!boxes) {
+        if ((e = hipMallocAsync(&own, vi_box_count(d) * sizeof(ViBox), s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(vi_box_kernel, dim3((unsigned)vi_box_count(d)), dim3(SX * SY), 0, s, d, (ViBox*)own, ntx);
+        e = hipGetLastError();
+        boxes = own;
+    }
     if (e == hipSuccess) {
         const dim3 grid(ntx, nty, ceil_div(d.nz, kStagedZ));
         if (fdk)
@@ -409,14 +415,19 @@ static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino,
                                (const ViBox*)boxes, (TV*)vol, vol_scale);
         e = hipGetLastError();
     }
-    const hipError_t f = hipFreeAsync(boxes, s);
-    return e != hipSuccess ? e : f;
+    if (own) {
+        const hipError_t f = hipFreeAsync(own, s);
+        if (e == hipSuccess) e = f;
+    }
+    return e;
 }
 
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) {
     const DevGeom& d = g->dev;
     if (sino.dtype != TOMO_F32 && sino.dtype != TOMO_F16 && sino.dtype != TOMO_BF16)
         return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");
+    // byte offsets within one view are 32-bit
+    if ((size_t)d.nu * d.nv >= (1ull << 30)) return fail(TOMO_ERR_UNSUPPORTED, "VI needs views of fewer than 2^30 cells");
     // A 16-bit sinogram is widened (value = stored / scale) into an FP32
     // copy from the stream-ordered pool, released after the backprojection
     // on the same stream: the staged kernel is bound by VALU issue, so 16-bit
@@ -434,9 +445,9 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
     }
     hipError_t e = hipSuccess;
     switch (vol.dtype) {
-        case TOMO_F32: e = launch_vi_staged<float>(d, fdk, taps, vol.p, vol.scale, s); break;
-        case TOMO_F16: e = launch_vi_staged<__half>(d, fdk, taps, vol.p, vol.scale, s); break;
-        case TOMO_BF16: e = launch_vi_staged<__hip_bfloat16>(d, fdk, taps, vol.p, vol.scale, s); break;
+        case TOMO_F32: e = launch_vi_staged<float>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, s); break;
+        case TOMO_F16: e = launch_vi_staged<__half>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, s); break;
+        case TOMO_BF16: e = launch_vi_staged<__hip_bfloat16>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, s); break;
         default:
             if (wide) (void)hipFreeAsync(wide, s);
             return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");

@@ -62,6 +62,7 @@ struct tomo_geom {
     std::vector<float> view_f;
     double* d_view = nullptr;
     float* d_view_f = nullptr;
+    void* d_vi_boxes = nullptr;  // VI box table (tomo_geom_create), or null: made per call
     tomo::DevGeom dev{};
     size_t nvox() const { return (size_t)info.nx * info.ny * info.nz; }
     size_t nsino() const { return (size_t)info.nu * info.nv * info.nphi; }
