@@ -50,6 +50,8 @@ def parse():
                     help="edges of the mixed-precision workload (720 views, 768x512 detector); empty skips it")
     ap.add_argument("--batch-problems", type=int, default=64,
                     help="uneven SIRT problems in the batched leg (0 skips it)")
+    ap.add_argument("--loop-iters", type=int, default=10,
+                    help="gradient iterations of the 512^3 reconstruction leg (0: skip)")
     ap.add_argument("--calibration-copy", action="store_true",
                     help="run one element-wise copy of the volume (known bytes) before the timed steps, "
                          "as a reference kernel for FETCH_SIZE / WRITE_SIZE counter scaling")
@@ -488,6 +490,10 @@ def main():
         "rs": {"ms": round(rs_ms, 3), "gups": round(upd / rs_ms / 1e6, 2), "alg_bytes": b_rs,
                "hbm_frac": round(b_rs / (rs_ms * 1e-3) / HBM_PEAK, 4), "n_samp": n_samp},
     }
+    # compulsory traffic: the volume and the sinogram once each
+    comp = float(n) ** 3 * 4 + nsino * 4
+    for name, ms in (("siddon", sid_ms), ("vi", vi_ms), ("rs", rs_ms)):
+        ops[name]["compulsory_frac"] = round(comp / (ms * 1e-3) / HBM_PEAK, 4)
     dom = "vi" if vi_ms >= sid_ms else "siddon"
     dom_bytes = b_vi if dom == "vi" else b_sid
     dom_ms = vi_ms if dom == "vi" else sid_ms
@@ -505,6 +511,29 @@ def main():
     if a.mixed_oracle_views > 0:
         mixed = mixed_leg(ctx, G, desc, vol, {"siddon": sid_ms, "vi": vi_ms, "rs": rs_ms}, stream,
                           a.mixed_oracle_views)
+
+    loop = None
+    if a.loop_iters > 0:
+        # The workload's reconstruction: SIDDONVI gradient descent with the
+        # quadratic regulariser (lambda 0.01) from zero on y = P(phantom),
+        # every step optimal (kopt = iterations + 1).
+        siddon()
+        y = sino.clone()
+        x = torch.zeros_like(vol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        hist = G.gd_run(y, x, pair=lib.PAIR_SIDDONVI, iters=a.loop_iters, kopt=a.loop_iters + 1, lam=0.01,
+                        stream=stream)
+        torch.cuda.synchronize()
+        t_loop = time.perf_counter() - t0
+        loop = {"pair": "SIDDONVI", "iterations": a.loop_iters, "lambda": 0.01, "kopt": a.loop_iters + 1,
+                "s": round(t_loop, 3), "ms_per_iteration": round(t_loop * 1e3 / a.loop_iters, 2),
+                "gups": round(3.0 * upd * a.loop_iters / t_loop / 1e9, 2),
+                "residual_sq_first": hist[0]["residual_sq"], "residual_sq_last": hist[-1]["residual_sq"],
+                "note": "per iteration two Siddon projections (P x, P dJ) and one VI backprojection (3 N^3 Nphi "
+                        "updates), plus the Laplacian terms, norms and step rule"}
+        print(f"[loop] {loop}", file=sys.stderr, flush=True)
+        del x, y
 
     serving = None
     del vol, sino, back
@@ -545,6 +574,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": base,
         "operators": ops,
+        "loop": loop,
         "mixed": mixed,
         "mixed_c2": mixed_c2,
         "serving": serving,
