@@ -83,9 +83,13 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
 constexpr int SX = kViStageX, SY = kViStageY;  // voxel columns per workgroup; a wave is 32 x 2 columns
 constexpr int kStagePitch = 65;  // floats per staged row (odd: rows of one column fall in different banks)
 
+// Staging buffer of a workgroup of 16-voxel columns: up to 44 rows of 64
+// taps (at 512^3 / 512 views the boxes need at most 41); two of them are 23 KB
+// of LDS, 6 workgroups per CU.
 template <int ZS>
 struct StageShape {
-    static constexpr int rows = ZS == 32 ? 64 : 44;  // staged rows per view
+    static_assert(ZS == 16, "staging buffer sized for 16-voxel columns");
+    static constexpr int rows = 44;                  // staged rows per view
     static constexpr int taps = rows * kStagePitch;  // floats per buffer
 };
 
