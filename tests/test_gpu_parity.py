@@ -197,6 +197,45 @@ def test_vi_bit_exact_unstaged_views(ctx, name, fdk):
         G.close()
 
 
+# Degenerate shapes: one voxel, one view and one detector cell; a one-slice
+# volume under a two-row detector; a three-column volume 40 voxels tall
+# (three z-tiles, the last partial) under a 4 x 50 detector.
+EDGE_GEOMS = {
+    "single": Geometry(fod=10.0, fdd=20.0, nphi=1, su=2.0, sv=2.0, nu=1, nv=1, sx=1.0, sy=1.0, sz=1.0, nx=1, ny=1,
+                       nz=1),
+    "thin": Geometry(fod=30.0, fdd=60.0, nphi=3, su=18.0, sv=4.0, nu=9, nv=2, sx=7.0, sy=5.0, sz=1.0, nx=7, ny=5,
+                     nz=1, num_start=1.0, den_start=4.0),
+    "tall": Geometry(fod=40.0, fdd=80.0, nphi=5, su=8.0, sv=100.0, nu=4, nv=50, sx=3.0, sy=3.0, sz=40.0, nx=3, ny=3,
+                     nz=40),
+}
+
+
+@pytest.mark.parametrize("name", list(EDGE_GEOMS))
+def test_degenerate_shapes_bit_exact(ctx, name):
+    g = EDGE_GEOMS[name]
+    print(f"[case] geometry={name}", flush=True)
+    G = lib.Geometry(ctx, to_desc(g))
+    try:
+        o = Oracle(g)
+        rng = np.random.default_rng(21)
+        sino = rng.random(g.sino_shape, dtype=np.float32)
+        vol = rng.random(g.vol_shape, dtype=np.float32)
+        outv = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        outs = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        cases = [("vi", o.vi(sino), lambda: G.backproject(dev(sino), outv), outv),
+                 ("vi_fdk", o.vi_fdk(sino), lambda: G.backproject(dev(sino), outv, flags=lib.FLAG_FDK), outv),
+                 ("rs", o.rs(vol), lambda: G.project(dev(vol), outs, lib.PROJ_RS), outs),
+                 ("rs_cube", o.rs(vol, cube=True), lambda: G.project(dev(vol), outs, lib.PROJ_RS,
+                                                                       lib.FLAG_RS_FOV_CUBE), outs),
+                 ("siddon", o.siddon(vol), lambda: G.project(dev(vol), outs, lib.PROJ_SIDDON), outs)]
+        for op, ref, run, out in cases:
+            run()
+            got = out.cpu().numpy()
+            assert np.array_equal(got, ref), (op, rel_l2(got, ref), rel_linf(got, ref))
+    finally:
+        G.close()
+
+
 @pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "vi"])
 def test_adjoint_dot(setup, ctx, op):
     """<A x, y> = <x, A^T y> with the GPU operator and its GPU transpose (rel 1e-5)."""
