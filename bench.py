@@ -42,7 +42,7 @@ def parse():
     ap.add_argument("--cpu-views", type=int, default=4, help="views in the CPU baseline sample")
     ap.add_argument("--serving-requests", type=int, default=24,
                     help="requests in the serving sample (0 skips the serving leg)")
-    ap.add_argument("--serving-rates", default="2,10,50", help="Poisson arrival rates (req/s)")
+    ap.add_argument("--serving-rates", default="2,5,10,20,35,50", help="Poisson arrival rates (req/s)")
     ap.add_argument("--serving-schedule", choices=["direct", "recurrent"], default="recurrent")
     ap.add_argument("--mixed-oracle-views", type=int, default=8,
                     help="views of the FP64 oracle check of the mixed forms (0 skips the mixed leg)")
