@@ -286,7 +286,19 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
     }
     const uint32_t dA = (uint32_t)sA * sa_e * u, dB = 2u * (uint32_t)sB * u, dZ = (uint32_t)r.step[2] * sz_e * u;
     int I = sA > 0 ? wave_min(live ? iA : 0x7fffffff) : wave_max(live ? iA : -0x7fffffff);
-    for (int outer = 0; outer <= nA + 1 && __any(live); ++outer) {
+    // The wave's iterations, counted once (a scalar loop instead of a vote
+    // per slab): a lane takes its slabs at iterations |iA - I| .. |iA - I| +
+    // K, K the first count with fma(K, inc, first) >= tmax (its last slab);
+    // the estimate below, a rounded quotient of two FP32 times, is off by less
+    // than one, and the count adds a margin of two.  Extra iterations find no
+    // lane at its slab and skip the body.
+    int span = 0;
+    if (live) {
+        const float kf = ceilf((r.tmax - r.first[A]) * __builtin_amdgcn_rcpf(r.inc[A]));
+        span = abs(iA - I) + (int)fminf(fmaxf(kf, 0.0f), 1.0e6f) + 3;
+    }
+    const int trips = __builtin_amdgcn_readfirstlane(min(wave_max(span), nA + 2));
+    for (int outer = 0; outer < trips; ++outer) {
         if (live && iA == I) {
             const float4 v = rd(e);
             // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
