@@ -285,21 +285,26 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
         e = ((uint32_t)iA * sa_e + (uint32_t)zz * sz_e + 2u * (uint32_t)bb) * u;
     }
     const uint32_t dA = (uint32_t)sA * sa_e * u, dB = 2u * (uint32_t)sB * u, dZ = (uint32_t)r.step[2] * sz_e * u;
-    int I = sA > 0 ? wave_min(live ? iA : 0x7fffffff) : wave_max(live ? iA : -0x7fffffff);
+    // Lockstep: the wave's slab starts at the first slab of its lanes (in
+    // each lane's direction along A) and advances one slab per iteration; a
+    // lane joins at iteration lag = |iA - I| and then takes one slab per
+    // iteration while it is live.
+    const int I = sA > 0 ? wave_min(live ? iA : 0x7fffffff) : wave_max(live ? iA : -0x7fffffff);
+    const int lag = abs(iA - I);
     // The wave's iterations, counted once (a scalar loop instead of a vote
-    // per slab): a lane takes its slabs at iterations |iA - I| .. |iA - I| +
-    // K, K the first count with fma(K, inc, first) >= tmax (its last slab);
-    // the estimate below, a rounded quotient of two FP32 times, is off by less
-    // than one, and the count adds a margin of two.  Extra iterations find no
-    // lane at its slab and skip the body.
+    // per slab): a lane's slabs are at iterations lag .. lag + K, K the first
+    // count with fma(K, inc, first) >= tmax (its last slab); the estimate
+    // below, a rounded quotient of two FP32 times, is off by less than one,
+    // and the count adds a margin of two.  Extra iterations find no live lane
+    // and skip the body.
     int span = 0;
     if (live) {
         const float kf = ceilf((r.tmax - r.first[A]) * __builtin_amdgcn_rcpf(r.inc[A]));
-        span = abs(iA - I) + (int)fminf(fmaxf(kf, 0.0f), 1.0e6f) + 3;
+        span = lag + (int)fminf(fmaxf(kf, 0.0f), 1.0e6f) + 3;
     }
     const int trips = __builtin_amdgcn_readfirstlane(min(wave_max(span), nA + 2));
     for (int outer = 0; outer < trips; ++outer) {
-        if (live && iA == I) {
+        if (live && outer >= lag) {
             const float4 v = rd(e);
             // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
             const float V0 = bneg ? (ZNEG ? v.w : v.z) : (ZNEG ? v.y : v.x);
@@ -330,12 +335,10 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
             cz += stepZ ? 1.0f : 0.0f;
             tB = fmaf(cB, r.inc[B], r.first[B]);
             tz = fmaf(cz, r.inc[2], r.first[2]);
-            iA += sA;
             cA += 1.0f;
             tA = fmaf(cA, r.inc[A], r.first[A]);
             live = t < r.tmax;
         }
-        I += sA;
     }
     acc_out = acc;
 }
