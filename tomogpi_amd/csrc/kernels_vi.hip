@@ -80,15 +80,23 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
 // holds zeros in the box, as the clamped cell it stands for.  View p is
 // staged in buffer p & 1, so one barrier per view separates the copy from
 // the reads.  A view whose box exceeds the buffer reads the taps directly.
-constexpr int SX = kViStageX, SY = kViStageY;  // voxel columns per workgroup; a wave is 32 x 2 columns
-constexpr int kStagePitch = 65;  // floats per staged row (odd: rows of one column fall in different banks)
+constexpr int SX = kViStageX, SY = kViStageY;  // voxel columns per workgroup; a wave is 8 x 8 columns
+// Floats per staged row.  A wave's 64 lanes read taps (u, r) at word
+// r * pitch + u; lanes spread over about 20 columns and a few rows, so two
+// lanes share a bank when their row and column offsets satisfy
+// dr * pitch + du = 0 mod 32.  With 8 x 8 waves a pitch of 75 (11 mod 32)
+// keeps such pairs rare at 512^3 / 512 views; with 32 x 2 waves (40 columns,
+// past the 32 banks) and a pitch of 65 (1 mod 32) a ds_read2_b32 took 2.6
+// extra cycles on bank conflicts.
+constexpr int kStagePitch = 75;
 
 // Staging buffer of a workgroup of 16-voxel columns: up to 44 rows of 64
-// taps (at 512^3 / 512 views the boxes need at most 41); two of them are 23 KB
-// of LDS, 6 workgroups per CU.
+// taps (at 512^3 / 512 views the boxes need at most 41); two of them are
+// 26.4 KB of LDS, 6 workgroups per CU (158.4 of 160 KB).
 template <int ZS>
 struct StageShape {
     static_assert(ZS == 16, "staging buffer sized for 16-voxel columns");
+    static_assert(SX == 32 && SY == 8, "four 8 x 8 waves per workgroup");
     static constexpr int rows = 44;                  // staged rows per view
     static constexpr int taps = rows * kStagePitch;  // floats per buffer
 };
@@ -202,8 +210,9 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     constexpr int kSlots = (SS::rows + NW - 1) / NW;        // box rows per wave at most
     const int tid = threadIdx.x;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
-    const int i = bx * SX + (tid % SX);
-    const int j = by * SY + (tid / SX);
+    // wave wv covers columns x in [8 wv, 8 wv + 8), y in [0, 8)
+    const int i = bx * SX + wv * 8 + (lane & 7);
+    const int j = by * SY + (lane >> 3);
     const int k0 = bz * ZS;
     const bool active = i < g.nx && j < g.ny;
     const double xp = (double)i - (double)g.cx, yp = (double)j - (double)g.cy;
