@@ -140,10 +140,20 @@ FINE_ROWS = Geometry(fod=200.0, fdd=400.0, nphi=12, su=40 * 0.5, sv=40 * 0.5, nu
                      sz=24.0, nx=24, ny=24, nz=24, num_start=0.0, den_start=1.0, num_stop=2.0, den_stop=1.0)
 
 
+# An FP16 sinogram is interpolated from u-pair words (two FP16 taps per
+# word) with FP16 u-weights rounded to steps of 1/2048, so each bilinear
+# sample differs from the FP32 form's by at most 2^-12 of the u-difference
+# of its taps; against the FP32 oracle on the same (widened) taps the
+# backprojection stays within these bounds.
+F16_VI_REL_L2 = 1e-4
+F16_VI_REL_LINF = 1e-3
+
+
 def _vi16_case(g, o, G, dt, scale=1.0):
     """16-bit sinogram stored as value * scale (a power of two) into an FP32
-    volume: VI widens the taps to FP32 (stored / scale, exact) and equals the
-    FP32 oracle on the widened values, bit for bit."""
+    volume.  BF16: VI widens the taps to FP32 (stored / scale, exact) and
+    equals the FP32 oracle on the widened values, bit for bit.  FP16: the
+    u-pair form, within F16_VI_REL_L2 / F16_VI_REL_LINF of the same."""
     x = o.rs(ellipsoids(g.nx, g.ny, g.nz, count=12, seed=6, amin=2.0, amax=12.0))
     s16 = (dev(x) * scale).to(torch.float16 if dt == "f16" else torch.bfloat16)
     widened = (s16.double() / scale).float().cpu().numpy()
@@ -151,21 +161,60 @@ def _vi16_case(g, o, G, dt, scale=1.0):
     out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
     G.backproject(s16, out, sino_scale=scale)
     got = out.cpu().numpy()
-    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+    if dt == "bf16":
+        assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+    else:
+        e2, ei = rel_l2(got, ref), rel_linf(got, ref)
+        print(f"[f16 vi] rel-L2 {e2:.2e} rel-Linf {ei:.2e} against the FP32 oracle on the same taps", flush=True)
+        assert e2 <= F16_VI_REL_L2 and ei <= F16_VI_REL_LINF, (e2, ei)
 
 
 @pytest.mark.parametrize("dt,scale", [("f16", 1.0), ("bf16", 1.0), ("f16", 2.0 ** -6)])
-def test_16bit_vi_bit_exact(setup, dt, scale):
+def test_16bit_vi(setup, dt, scale):
     g, o, G = setup
     _vi16_case(g, o, G, dt, scale)
 
 
 @pytest.mark.parametrize("dt", ["f16", "bf16"])
-def test_16bit_vi_bit_exact_fine_rows(ctx, dt):
+def test_16bit_vi_fine_rows(ctx, dt):
     """Every view read directly (boxes over the staging buffer)."""
     G = lib.Geometry(ctx, to_desc(FINE_ROWS))
     try:
         _vi16_case(FINE_ROWS, Oracle(FINE_ROWS), G, dt)
+    finally:
+        G.close()
+
+
+# Detectors wider than the volume's shadow (every u-tap of every voxel-view on
+# the detector), one with rows of about one voxel (views staged in LDS), one
+# with rows of about 2.5 voxels near the source (views staged and read directly).
+WIDE_DET = {
+    "staged": Geometry(fod=100.0, fdd=200.0, nphi=24, su=180.0, sv=100.0, nu=90, nv=50, sx=32.0, sy=32.0,
+                       sz=32.0, nx=32, ny=32, nz=32),
+    "direct": Geometry(fod=60.0, fdd=160.0, nphi=24, su=300.0, sv=300.0, nu=300, nv=300, sx=32.0, sy=32.0,
+                       sz=32.0, nx=32, ny=32, nz=32),
+}
+
+
+@pytest.mark.parametrize("name", list(WIDE_DET))
+@pytest.mark.parametrize("fdk", [False, True])
+@pytest.mark.parametrize("scale", [1.0, 2.0 ** -6])
+def test_f16_vi_constant_sinogram_exact(ctx, name, fdk, scale):
+    """FP16 u-pair VI of a constant sinogram equals the FP32 oracle bit for
+    bit: the FP16 u-weights sum to exactly 1, so each row interpolates to the
+    constant and the v-interpolation, FDK weight and sums are the FP32 form's."""
+    g = WIDE_DET[name]
+    print(f"[case] geometry={name} fdk={fdk} scale={scale}", flush=True)
+    G = lib.Geometry(ctx, to_desc(g))
+    try:
+        o = Oracle(g)
+        sino = np.full(g.sino_shape, 0.75, dtype=np.float32)
+        ref = o.vi_fdk(sino) if fdk else o.vi(sino)
+        s16 = (dev(sino) * scale).to(torch.float16)
+        out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.backproject(s16, out, flags=lib.FLAG_FDK if fdk else 0, sino_scale=scale)
+        got = out.cpu().numpy()
+        assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
     finally:
         G.close()
 

@@ -11,8 +11,10 @@
 // The output is overwritten.  As in the reference's "16reg" blocking, a
 // thread owns a column of 16 voxels that share u and A; the per-(column,view)
 // terms w, 1/w, u and A are evaluated in FP64 (the geometry of the FP32 form)
-// and the per-voxel v, bilinear weights and sum in FP32.  A 16-bit sinogram is
-// widened to FP32 (value = stored / scale) before the same kernel.
+// and the per-voxel v, bilinear weights and sum in FP32.  A BF16 sinogram is
+// widened to FP32 (value = stored / scale) before the same kernel; an FP16
+// sinogram is read as u-pairs of FP16 taps (vi_pair_kernel) and interpolated
+// with v_dot2_f32_f16 (FP16 u-weights in steps of 1/2048, FP32 sums).
 #include <algorithm>
 #include <climits>
 #include <cmath>
@@ -199,7 +201,51 @@ __device__ __forceinline__ StagePlan stage_plan(const ViBox& bb, float zp0, floa
     return s;
 }
 
-template <typename TV, bool FDK, int ZS>
+// FP16 u-weights (1 - w, w) of a column-view, w = fu rounded to a multiple of
+// 1/2048: both are exact FP16 values and sum to 1, so a constant row
+// interpolates to itself.
+__device__ __forceinline__ uint32_t pair_weights(float fu) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const float w1 = rintf(fu * 2048.0f) * (1.0f / 2048.0f);
+    return __builtin_bit_cast(uint32_t, h2{(_Float16)(1.0f - w1), (_Float16)w1});
+}
+
+// Bilinear sample from two u-pair words (rows v and v + 1): r0 = w . P0 and
+// the row difference w . P1 - r0, each one v_dot2_f32_f16 (products of FP16
+// values are exact in FP32), then r0 + fv (r1 - r0).  The builtins leave the
+// wait states a dot2 result needs before its use to the compiler.
+__device__ __forceinline__ float pair_sample(uint32_t p0, uint32_t p1, uint32_t w, float fv) {
+    typedef _Float16 h2 __attribute__((ext_vector_type(2)));
+    const h2 hw = __builtin_bit_cast(h2, w);
+    const float r0 = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, p0), hw, 0.0f, false);
+    const float dr = __builtin_amdgcn_fdot2(__builtin_bit_cast(h2, p1), hw, -r0, false);
+    return fmaf(fv, dr, r0);
+}
+
+// Four bilinear samples at once (the staged path's inner loop): the four
+// r0 = w . P0 first, then the four w . P1 - r0, so each dot2 result is read
+// three or more instructions after it is written (the wait a VALU read of a
+// dot2 result needs), and s_nop 2 before any later read of the last one.
+__device__ __forceinline__ void pair_sample4(const uint32_t (&p0)[4], const uint32_t (&p1)[4], uint32_t w,
+                                             float (&r0)[4], float (&dr)[4]) {
+    asm volatile(
+        "v_dot2_f32_f16 %0, %8, %16, 0\n\t"
+        "v_dot2_f32_f16 %1, %9, %16, 0\n\t"
+        "v_dot2_f32_f16 %2, %10, %16, 0\n\t"
+        "v_dot2_f32_f16 %3, %11, %16, 0\n\t"
+        "v_dot2_f32_f16 %4, %12, %16, %0 neg_lo:[0,0,1]\n\t"
+        "v_dot2_f32_f16 %5, %13, %16, %1 neg_lo:[0,0,1]\n\t"
+        "v_dot2_f32_f16 %6, %14, %16, %2 neg_lo:[0,0,1]\n\t"
+        "v_dot2_f32_f16 %7, %15, %16, %3 neg_lo:[0,0,1]\n\t"
+        "s_nop 2"
+        : "=&v"(r0[0]), "=&v"(r0[1]), "=&v"(r0[2]), "=&v"(r0[3]), "=&v"(dr[0]), "=&v"(dr[1]), "=&v"(dr[2]),
+          "=&v"(dr[3])
+        : "v"(p0[0]), "v"(p0[1]), "v"(p0[2]), "v"(p0[3]), "v"(p1[0]), "v"(p1[1]), "v"(p1[2]), "v"(p1[3]), "v"(w));
+}
+
+// PAIR: `sino` holds u-pair words (vi_pair_kernel; nu + 1 per row) instead of
+// FP32 taps; the box copy and the tap offsets are the same, one word per row.
+template <typename TV, bool FDK, int ZS, bool PAIR = false>
 __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __restrict__ sino,
                                                const ViBox* __restrict__ boxes, TV* __restrict__ vol,
                                                float vol_scale, float post, int bx, int by, int bz,
@@ -220,7 +266,8 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     const float cv = g.cv;
     const float fnu = (float)g.nu;
     const int nu = g.nu, nv = g.nv;
-    const size_t plane = (size_t)nu * nv;
+    const int rl = PAIR ? nu + 1 : nu;  // words per detector row
+    const size_t plane = (size_t)rl * nv;
     const ViBox* bt = boxes + (size_t)(by * ((g.nx + SX - 1) / SX) + bx) * g.nphi;
     float acc[ZS];
 #pragma unroll
@@ -251,7 +298,8 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     auto copy_box = [&](const StagePlan& s, int p, int b) {
         if (!s.fits) return;
         const float* view = sino + (size_t)p * plane;
-        const uint32_t loff = (uint32_t)(s.u0 + lane) * 4u;  // past the row's end unless 0 <= u < nu
+        // past the row's end unless 0 <= u < nu (FP32) or the pair index u + 1 <= nu
+        const uint32_t loff = (uint32_t)(s.u0 + (PAIR ? 1 : 0) + lane) * 4u;
         float* dst = stage + b * SS::taps + wv * kStagePitch;
 #pragma unroll
         for (int q = 0; q < kSlots; ++q) {
@@ -259,7 +307,7 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
                 const int v = s.v0 + wv + q * NW;
                 const bool vin = (unsigned)v < (unsigned)nv;
                 const __amdgpu_buffer_rsrc_t rr =
-                    buffer_rsrc(view + (size_t)(vin ? v : 0) * nu, vin ? (uint32_t)nu * 4u : 0u);
+                    buffer_rsrc(view + (size_t)(vin ? v : 0) * rl, vin ? (uint32_t)rl * 4u : 0u);
                 __builtin_amdgcn_raw_ptr_buffer_load_lds(
                     rr, (__attribute__((address_space(3))) void*)(dst + q * NW * kStagePitch), 4, loff, 0, 0, 0);
             }
@@ -283,28 +331,52 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
             const float uf = floorf(cvw.u);
             const float fu = cvw.u - uf;
             const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
+            const uint32_t pw = PAIR ? pair_weights(fu) : 0u;
             if (now.fits) {
                 // byte offset in the buffer of tap (uc, vf): ((vf - v0) pitch + uc - u0) 4,
                 // one exact fma of small integers
                 const float colb =
                     (float)(((int)uc - now.u0 - now.v0 * kStagePitch + (p & 1) * SS::taps) * 4);
+                if constexpr (PAIR) {
+                    static_assert(ZS % 4 == 0, "u-pair samples go four voxels at a time");
 #pragma unroll
-                for (int k = 0; k < ZS; ++k) {
-                    const float v = fmaf(zp0 + (float)k, cvw.A, cv);
-                    const float vf = floorf(v);
-                    const uint32_t off = (uint32_t)fmaf(vf, (float)(kStagePitch * 4), colb);
-                    const float* q = (const float*)((const char*)stage + off);
-                    term(cvw, fu, k, v, vf, q[0], q[kStagePitch], q[1], q[kStagePitch + 1]);
+                    for (int k4 = 0; k4 < ZS; k4 += 4) {
+                        uint32_t p0[4], p1[4];
+                        float fv[4], r0[4], dr[4];
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) {
+                            const float v = fmaf(zp0 + (float)(k4 + t), cvw.A, cv);
+                            const float vf = floorf(v);
+                            const uint32_t off = (uint32_t)fmaf(vf, (float)(kStagePitch * 4), colb);
+                            const uint32_t* qw = (const uint32_t*)((const char*)stage + off);
+                            p0[t] = qw[0];
+                            p1[t] = qw[kStagePitch];
+                            fv[t] = v - vf;
+                        }
+                        pair_sample4(p0, p1, pw, r0, dr);
+#pragma unroll
+                        for (int t = 0; t < 4; ++t) add(cvw, k4 + t, fmaf(fv[t], dr[t], r0[t]));
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < ZS; ++k) {
+                        const float v = fmaf(zp0 + (float)k, cvw.A, cv);
+                        const float vf = floorf(v);
+                        const uint32_t off = (uint32_t)fmaf(vf, (float)(kStagePitch * 4), colb);
+                        const float* q = (const float*)((const char*)stage + off);
+                        term(cvw, fu, k, v, vf, q[0], q[kStagePitch], q[1], q[kStagePitch + 1]);
+                    }
                 }
             } else {
                 // box larger than the buffer: the four taps straight from the view, one voxel at a time
                 const __amdgpu_buffer_rsrc_t rsrc = buffer_rsrc(sino + (size_t)p * plane, view_bytes);
                 const int iu = (int)uc;
                 const bool u0ok = (unsigned)iu < (unsigned)nu, u1ok = (unsigned)(iu + 1) < (unsigned)nu;
+                const bool pok = (unsigned)(iu + 1) <= (unsigned)nu;  // pair word iu + 1 exists
                 ColumnView cw = cvw;
                 asm volatile("" : "+v"(cw.A));  // this path's rows are its own, not hoisted above the branch
                 auto rd = [&](bool ok, int v, int u) {
-                    const uint32_t off = ok ? (uint32_t)(v * nu + u) * 4u : 0xfffffff0u;
+                    const uint32_t off = ok ? (uint32_t)(v * rl + u) * 4u : 0xfffffff0u;
                     return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rsrc, (int)off, 0, 0));
                 };
 #pragma unroll
@@ -313,8 +385,14 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
                     const float vf = floorf(v);
                     const int iv = (int)__builtin_amdgcn_fmed3f(vf, -2.0f, (float)nv);
                     const bool v0ok = (unsigned)iv < (unsigned)nv, v1ok = (unsigned)(iv + 1) < (unsigned)nv;
-                    term(cw, fu, k, v, vf, rd(u0ok && v0ok, iv, iu), rd(u0ok && v1ok, iv + 1, iu),
-                         rd(u1ok && v0ok, iv, iu + 1), rd(u1ok && v1ok, iv + 1, iu + 1));
+                    if constexpr (PAIR) {
+                        const uint32_t p0 = __builtin_bit_cast(uint32_t, rd(pok && v0ok, iv, iu + 1));
+                        const uint32_t p1 = __builtin_bit_cast(uint32_t, rd(pok && v1ok, iv + 1, iu + 1));
+                        add(cw, k, pair_sample(p0, p1, pw, v - vf));
+                    } else {
+                        term(cw, fu, k, v, vf, rd(u0ok && v0ok, iv, iu), rd(u0ok && v1ok, iv + 1, iu),
+                             rd(u1ok && v0ok, iv, iu + 1), rd(u1ok && v1ok, iv + 1, iu + 1));
+                    }
                     __builtin_amdgcn_sched_barrier(0);
                 }
             }
@@ -326,16 +404,33 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
 #pragma unroll
     for (int k = 0; k < ZS; ++k)
         if (k0 + k < g.nz)
-            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? acc[k] : (acc[k] * g.delta) * post,
+            Storage<TV>::store(vol, base + (size_t)(k0 + k) * sxy, FDK ? acc[k] * post : (acc[k] * g.delta) * post,
                                vol_scale);
 }
 
-template <typename TV, bool FDK, int ZS>
+// post: 1 for FP32 taps, 1 / storage scale for FP16 u-pairs.
+template <typename TV, bool FDK, int ZS, bool PAIR>
 __global__ __launch_bounds__(SX* SY, 6) void vi_staged_kernel(DevGeom g, const float* __restrict__ sino,
                                                          const ViBox* __restrict__ boxes, TV* __restrict__ vol,
-                                                         float vol_scale) {
+                                                         float vol_scale, float post) {
     __shared__ float stage[2 * StageShape<ZS>::taps];
-    vi_staged_tile<TV, FDK, ZS>(g, sino, boxes, vol, vol_scale, 1.0f, blockIdx.x, blockIdx.y, blockIdx.z, stage);
+    vi_staged_tile<TV, FDK, ZS, PAIR>(g, sino, boxes, vol, vol_scale, post, blockIdx.x, blockIdx.y, blockIdx.z,
+                                      stage);
+}
+
+// u-pair words of an FP16 sinogram, one workgroup per detector row: word j in
+// [0, nu] of the row holds taps (j - 1, j) as its low and high halves, a tap
+// off the row as 0.
+__global__ __launch_bounds__(256) void vi_pair_kernel(const uint16_t* __restrict__ sino,
+                                                      uint32_t* __restrict__ pairs, int nu) {
+    const size_t row = blockIdx.x;
+    const uint16_t* src = sino + row * (size_t)nu;
+    uint32_t* dst = pairs + row * (size_t)(nu + 1);
+    for (int j = threadIdx.x; j <= nu; j += blockDim.x) {
+        const uint32_t lo = j >= 1 ? src[j - 1] : 0u;
+        const uint32_t hi = j < nu ? src[j] : 0u;
+        dst[j] = lo | (hi << 16);
+    }
 }
 
 // Shared launch (FP32 sinograms and volumes): tile grid (ceil(nx/SX),
@@ -406,9 +501,9 @@ constexpr int kStagedZ = kViTileZ;
 // The staged kernel on an FP32 sinogram, with the geometry's box table
 // (`boxes`) or, when null, one made for this call from the stream-ordered
 // pool.
-template <typename TV>
+template <typename TV, bool PAIR>
 static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino, const void* boxes, void* vol,
-                                   float vol_scale, hipStream_t s) {
+                                   float vol_scale, float post, hipStream_t s) {
     const int ntx = ceil_div(d.nx, SX), nty = ceil_div(d.ny, SY);
     void* own = nullptr;
     hipError_t e = hipSuccess;
@@ -421,11 +516,11 @@ static hipError_t launch_vi_staged(const DevGeom& d, int fdk, const float* sino,
     if (e == hipSuccess) {
         const dim3 grid(ntx, nty, ceil_div(d.nz, kStagedZ));
         if (fdk)
-            hipLaunchKernelGGL((vi_staged_kernel<TV, true, kStagedZ>), grid, dim3(SX * SY), 0, s, d, sino,
-                               (const ViBox*)boxes, (TV*)vol, vol_scale);
+            hipLaunchKernelGGL((vi_staged_kernel<TV, true, kStagedZ, PAIR>), grid, dim3(SX * SY), 0, s, d, sino,
+                               (const ViBox*)boxes, (TV*)vol, vol_scale, post);
         else
-            hipLaunchKernelGGL((vi_staged_kernel<TV, false, kStagedZ>), grid, dim3(SX * SY), 0, s, d, sino,
-                               (const ViBox*)boxes, (TV*)vol, vol_scale);
+            hipLaunchKernelGGL((vi_staged_kernel<TV, false, kStagedZ, PAIR>), grid, dim3(SX * SY), 0, s, d, sino,
+                               (const ViBox*)boxes, (TV*)vol, vol_scale, post);
         e = hipGetLastError();
     }
     if (own) {
@@ -439,15 +534,30 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
     const DevGeom& d = g->dev;
     if (sino.dtype != TOMO_F32 && sino.dtype != TOMO_F16 && sino.dtype != TOMO_BF16)
         return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");
+    if (vol.dtype != TOMO_F32 && vol.dtype != TOMO_F16 && vol.dtype != TOMO_BF16)
+        return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");
     // byte offsets within one view are 32-bit
-    if ((size_t)d.nu * d.nv >= (1ull << 30)) return fail(TOMO_ERR_UNSUPPORTED, "VI needs views of fewer than 2^30 cells");
-    // A 16-bit sinogram is widened (value = stored / scale) into an FP32
-    // copy from the stream-ordered pool, released after the backprojection
-    // on the same stream: the staged kernel is bound by VALU issue, so 16-bit
-    // taps would not make it faster.
+    if ((size_t)(d.nu + 1) * d.nv >= (1ull << 30))
+        return fail(TOMO_ERR_UNSUPPORTED, "VI needs views of fewer than 2^30 cells");
+    // An FP16 sinogram becomes u-pair words, a BF16 one is widened (value =
+    // stored / scale) to FP32; either copy comes from the stream-ordered pool
+    // and is released after the backprojection on the same stream.
+    const bool pair = sino.dtype == TOMO_F16;
     const float* taps = (const float*)sino.p;
     void* wide = nullptr;
-    if (sino.dtype != TOMO_F32) {
+    if (pair) {
+        const size_t rows = (size_t)d.nv * d.nphi;
+        if (rows >= (1ull << 31)) return fail(TOMO_ERR_UNSUPPORTED, "VI needs fewer than 2^31 detector rows");
+        TOMO_HIP(hipMallocAsync(&wide, rows * (size_t)(d.nu + 1) * sizeof(uint32_t), s));
+        if (rows > 0) hipLaunchKernelGGL(vi_pair_kernel, dim3((unsigned)rows), dim3(256), 0, s,
+                                         (const uint16_t*)sino.p, (uint32_t*)wide, d.nu);
+        const hipError_t le = hipGetLastError();
+        if (le != hipSuccess) {
+            (void)hipFreeAsync(wide, s);
+            TOMO_HIP(le);
+        }
+        taps = (const float*)wide;
+    } else if (sino.dtype != TOMO_F32) {
         TOMO_HIP(hipMallocAsync(&wide, g->nsino() * sizeof(float), s));
         const int rc = launch_convert(sino, MutBuf{wide, TOMO_F32, 1.0f}, g->nsino(), s);
         if (rc) {
@@ -456,15 +566,17 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk) 
         }
         taps = (const float*)wide;
     }
+    const float post = pair ? 1.0f / sino.scale : 1.0f;
     hipError_t e = hipSuccess;
+#define TOMO_VI_STAGED(TV)                                                                              \
+    e = pair ? launch_vi_staged<TV, true>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, post, s)     \
+             : launch_vi_staged<TV, false>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, post, s)
     switch (vol.dtype) {
-        case TOMO_F32: e = launch_vi_staged<float>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, s); break;
-        case TOMO_F16: e = launch_vi_staged<__half>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, s); break;
-        case TOMO_BF16: e = launch_vi_staged<__hip_bfloat16>(d, fdk, taps, g->d_vi_boxes, vol.p, vol.scale, s); break;
-        default:
-            if (wide) (void)hipFreeAsync(wide, s);
-            return fail(TOMO_ERR_UNSUPPORTED, "unsupported element type");
+        case TOMO_F32: TOMO_VI_STAGED(float); break;
+        case TOMO_F16: TOMO_VI_STAGED(__half); break;
+        default: TOMO_VI_STAGED(__hip_bfloat16); break;
     }
+#undef TOMO_VI_STAGED
     if (wide) TOMO_HIP(hipFreeAsync(wide, s));
     TOMO_HIP(e);
     return TOMO_OK;
