@@ -157,7 +157,10 @@ int tomo_project(tomo_ctx* ctx, const tomo_geom* geom, int projector, int flags,
 
 /* volume <- B sinogram.  backprojector = TOMO_BP_VI.
  * Replaces VIBackProjector_compute_CUDA_mem_{GPU,CPU}::doBackProjection
- * (C/VIBackProjector_compute_CUDA_mem_GPU.cu:26-111, ..._mem_CPU.cu:19-690). */
+ * (C/VIBackProjector_compute_CUDA_mem_GPU.cu:26-111, ..._mem_CPU.cu:19-690).
+ * An F32 or BF16 sinogram is interpolated in FP32 (BF16 taps widened first);
+ * an F16 sinogram with FP16 u-weights in steps of 1/2048 (two taps per
+ * v_dot2_f32_f16) and FP32 sums.                                           */
 int tomo_backproject(tomo_ctx* ctx, const tomo_geom* geom, int backprojector, int flags,
                      const void* sinogram, int sinogram_dtype, float sinogram_scale,
                      void* volume, int volume_dtype, float volume_scale,
