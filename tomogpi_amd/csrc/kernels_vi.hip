@@ -92,14 +92,15 @@ constexpr int SX = kViStageX, SY = kViStageY;  // voxel columns per workgroup; a
 // extra cycles on bank conflicts.
 constexpr int kStagePitch = 75;
 
-// Staging buffer of a workgroup of 16-voxel columns: up to 44 rows of 64
-// taps (at 512^3 / 512 views the boxes need at most 41); two of them are
-// 26.4 KB of LDS, 6 workgroups per CU (158.4 of 160 KB).
+// Staging buffer of a workgroup of 16-voxel columns: up to 38 rows of 64
+// taps (at 512^3 / 512 views about 0.1% of the boxes need 39 or 40 and read
+// their taps directly); two of them are 22.8 KB of LDS, 7 workgroups per CU
+// (FDK: 6, its VGPRs).
 template <int ZS>
 struct StageShape {
     static_assert(ZS == 16, "staging buffer sized for 16-voxel columns");
     static_assert(SX == 32 && SY == 8, "four 8 x 8 waves per workgroup");
-    static constexpr int rows = 44;                  // staged rows per view
+    static constexpr int rows = 38;                  // staged rows per view
     static constexpr int taps = rows * kStagePitch;  // floats per buffer
 };
 
@@ -410,7 +411,7 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
 
 // post: 1 for FP32 taps, 1 / storage scale for FP16 u-pairs.
 template <typename TV, bool FDK, int ZS, bool PAIR>
-__global__ __launch_bounds__(SX* SY, 6) void vi_staged_kernel(DevGeom g, const float* __restrict__ sino,
+__global__ __launch_bounds__(SX* SY, FDK ? 6 : 7) void vi_staged_kernel(DevGeom g, const float* __restrict__ sino,
                                                          const ViBox* __restrict__ boxes, TV* __restrict__ vol,
                                                          float vol_scale, float post) {
     __shared__ float stage[2 * StageShape<ZS>::taps];
@@ -436,7 +437,7 @@ __global__ __launch_bounds__(256) void vi_pair_kernel(const uint16_t* __restrict
 // Shared launch (FP32 sinograms and volumes): tile grid (ceil(nx/SX),
 // ceil(ny/SY), ceil(nz/16)) on the sinogram `in` with the problem's box table
 // `aux` (launch_vi_boxes); the stored value is (delta * sum) * a.
-__global__ __launch_bounds__(SX* SY, 6) void vi_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+__global__ __launch_bounds__(SX* SY, 7) void vi_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
     __shared__ float stage[2 * StageShape<kViTileZ>::taps];
     TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
