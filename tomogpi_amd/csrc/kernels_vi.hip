@@ -269,7 +269,12 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     const int nu = g.nu, nv = g.nv;
     const int rl = PAIR ? nu + 1 : nu;  // words per detector row
     const size_t plane = (size_t)rl * nv;
-    const ViBox* bt = boxes + (size_t)(by * ((g.nx + SX - 1) / SX) + bx) * g.nphi;
+    // The box and view tables are read-only here and indexed by wave-uniform
+    // values: read through the constant address space they come in with scalar
+    // loads, and the staging plan's branches are uniform.
+    typedef const __attribute__((address_space(4))) ViBox* ConstBox;
+    typedef const __attribute__((address_space(4))) double* ConstView;
+    const ConstBox bt = (ConstBox)(boxes + (size_t)(by * ((g.nx + SX - 1) / SX) + bx) * g.nphi);
     float acc[ZS];
 #pragma unroll
     for (int k = 0; k < ZS; ++k) acc[k] = 0.0f;
@@ -316,7 +321,8 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     };
     const bool lo_amax = zp0 < 0.0f, hi_amax = zp0 + (float)(ZS - 1) > 0.0f;
     const uint32_t view_bytes = (uint32_t)(plane * sizeof(float));
-    StagePlan nxt = stage_plan<ZS>(bt[0], zp0, cv, lo_amax, hi_amax);
+    auto box_of = [&](int p) { return ViBox{bt[p].u0, bt[p].u1, bt[p].amin, bt[p].amax}; };
+    StagePlan nxt = stage_plan<ZS>(box_of(0), zp0, cv, lo_amax, hi_amax);
     copy_box(nxt, 0, 0);
     for (int p = 0; p < g.nphi; ++p) {
         // this wave's copies of view p have landed; every wave is done with view p - 1
@@ -324,11 +330,15 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
         __syncthreads();
         const StagePlan now = nxt;
         if (p + 1 < g.nphi) {
-            nxt = stage_plan<ZS>(bt[p + 1], zp0, cv, lo_amax, hi_amax);
+            nxt = stage_plan<ZS>(box_of(p + 1), zp0, cv, lo_amax, hi_amax);
             copy_box(nxt, p + 1, (p + 1) & 1);
         }
         if (active) {
-            const ColumnView cvw = column_view<FDK>(g, g.view + (size_t)p * kViewStride, xp, yp);
+            const ConstView vt = (ConstView)(g.view + (size_t)p * kViewStride);
+            double c[kViewStride];
+#pragma unroll
+            for (int e = 0; e < kViewStride; ++e) c[e] = vt[e];
+            const ColumnView cvw = column_view<FDK>(g, c, xp, yp);
             const float uf = floorf(cvw.u);
             const float fu = cvw.u - uf;
             const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
