@@ -36,7 +36,7 @@
 extern "C" {
 #endif
 
-#define TOMOGPI_ABI_VERSION 3
+#define TOMOGPI_ABI_VERSION 4
 
 /* ---- status codes ------------------------------------------------------ */
 #define TOMO_OK                0
@@ -79,6 +79,13 @@ extern "C" {
                                         34-35): f = sum_p W_p bilinear(g_p),
                                         W_p = alphaC (FDD/du) / w^2 /
                                         sqrt(betaC + u'^2 + v'^2), no delta   */
+#define TOMO_FLAG_INDEX_READS   0x10 /* RS, clean Siddon: read the z-pair copies
+                                        through 64-bit element indices, the
+                                        form every copy of 4 GiB or more
+                                        takes (1024^3 in FP32 or 16-bit),
+                                        also where 32-bit buffer offsets
+                                        apply; same results.  Lets that form
+                                        be checked at small sizes.           */
 
 typedef struct tomo_ctx  tomo_ctx;
 typedef struct tomo_geom tomo_geom;
@@ -223,10 +230,19 @@ typedef struct tomo_gd_params {
 } tomo_gd_params;
 
 typedef struct tomo_gd_iter_stat {
-    double residual_sq;              /* sum (y - P x)^2 before the update    */
+    double residual_sq;              /* sum (y - P x)^2 before the update
+                                        (J_MC of Iteration.dat)              */
     double norm_dj;                  /* sum dJ^2 (0 when the step is fixed)   */
     double norm_dj_proj;             /* 2 sum (P dJ)^2 + lambda normdJProjReg */
     double alpha;                    /* step used                            */
+    double j_reg;                    /* sum (L x)^2 before the update when
+                                        lambda != 0, else 0 (JReg of the GG
+                                        beta=2 regulariser, C/Generalized
+                                        GaussianRegularizer_CPU.cu:178); 0 in
+                                        served requests                      */
+    double beta;                     /* conjugate gradient: |g|^2/|g_prev|^2
+                                        (0 at the first iteration and in the
+                                        gradient loops)                      */
 } tomo_gd_iter_stat;
 
 /* Runs the loop on device-resident data (or host data with TOMO_HOST_PTR in
@@ -285,8 +301,8 @@ int tomo_sirt_run(tomo_ctx* ctx, const tomo_geom* geom, const tomo_sirt_params* 
  * A server owns a worker thread and a stream on the context's device.  Each
  * request is one simple-gradient reconstruction (the loop of tomo_gd_run,
  * zero initial volume) of its own geometry and size.  Requests are admitted
- * in arrival order while the device memory they need fits the budget and a
- * slot is free; admitted requests keep volumes, sinograms, geometry and loop
+ * in arrival order while the device memory they need fits (free memory or a
+ * fixed budget) and a slot is free; admitted requests keep volumes, sinograms, geometry and loop
  * scalars resident on the device.  The server advances every admitted request
  * by one or more iterations per round (more for cheaper requests), each
  * operator of an iteration being one shared persistent launch over all of
@@ -307,7 +323,11 @@ typedef struct tomo_server tomo_server;
 
 typedef struct tomo_serve_params {
     uint64_t hbm_budget_bytes;  /* device bytes admitted requests may hold;
-                                   0 = free memory at creation less 1/16    */
+                                   0 = live: a request is admitted while its
+                                   footprint fits the device's free memory
+                                   (hipMemGetInfo at admission, plus memory
+                                   the server's pool holds unused) less 1/16
+                                   of the device                            */
     int32_t  max_active;        /* requests resident at once, 1..64         */
     int32_t  use_graph;         /* replay each round from a captured graph  */
     int32_t  schedule;          /* TOMO_SCHEDULE_*                          */

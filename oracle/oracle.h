@@ -108,8 +108,11 @@ typedef struct og_gd_params {
     double  alpha_min_init;
     int32_t matched;      /* 1: use the exact transpose of P as B (tests) */
 } og_gd_params;
+/* j_reg = sum (L x)^2 of the iterate before the update when lambda != 0
+ * (the GG beta=2 criterion, C/GeneralizedGaussianRegularizer_CPU.cu:178),
+ * beta = the conjugate gradient's |g|^2/|g_prev|^2 (0 otherwise).          */
 typedef struct og_gd_stat {
-    double residual_sq, norm_dj, norm_dj_proj, alpha;
+    double residual_sq, norm_dj, norm_dj_proj, alpha, j_reg, beta;
 } og_gd_stat;
 int og_gd_run(const og_geom* g, const og_gd_params* p, const float* sino,
               float* vol, og_gd_stat* stats);

@@ -272,7 +272,7 @@ static int gd_loop(const og_geom* g, const og_gd_params* p, const float* sino, c
     if (!est || !dj || !t1 || !t2) { free(est); free(dj); free(t1); free(t2); return 1; }
     double alpha_min = p->alpha_min_init;
     for (int k = 1; k <= p->iters; ++k) {
-        og_gd_stat st = {0, 0, 0, 0};
+        og_gd_stat st = {0, 0, 0, 0, 0, 0};
         project(g, p, vol, est);
         for (size_t i = 0; i < ns; ++i) est[i] = sino[i] - est[i];
         st.residual_sq = sq_norm(est, ns);
@@ -283,6 +283,7 @@ static int gd_loop(const og_geom* g, const og_gd_params* p, const float* sino, c
         double norm_reg = 0.0;
         if (p->lambda != 0.0) {
             og_laplacian_f32(g->nx, g->ny, g->nz, vol, t1);
+            st.j_reg = sq_norm(t1, nv);
             og_laplacian_f32(g->nx, g->ny, g->nz, t1, t2);
             const float w = (float)(2.0 * (double)(float)p->lambda);
             for (size_t i = 0; i < nv; ++i) dj[i] = dj[i] + w * t2[i];
@@ -340,7 +341,7 @@ int og_cg_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
     if (!est || !gr || !d || !t1 || !t2) { free(est); free(gr); free(d); free(t1); free(t2); return 1; }
     double gg_prev = 0.0;
     for (int k = 1; k <= p->iters; ++k) {
-        og_gd_stat st = {0, 0, 0, 0};
+        og_gd_stat st = {0, 0, 0, 0, 0, 0};
         project(g, p, vol, est);
         for (size_t i = 0; i < ns; ++i) est[i] = sino[i] - est[i];
         st.residual_sq = sq_norm(est, ns);
@@ -348,6 +349,7 @@ int og_cg_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
         for (size_t i = 0; i < nv; ++i) gr[i] = gr[i] * -2.0f;
         if (p->lambda != 0.0) {
             og_laplacian_f32(g->nx, g->ny, g->nz, vol, t1);
+            st.j_reg = sq_norm(t1, nv);
             og_laplacian_f32(g->nx, g->ny, g->nz, t1, t2);
             const float w = (float)(2.0 * (double)(float)p->lambda);
             for (size_t i = 0; i < nv; ++i) gr[i] = gr[i] + w * t2[i];
@@ -356,7 +358,8 @@ int og_cg_run(const og_geom* g, const og_gd_params* p, const float* sino, float*
         if (k == 1) {
             for (size_t i = 0; i < nv; ++i) d[i] = gr[i];
         } else {
-            const float bf = (float)(gg / gg_prev);
+            st.beta = gg / gg_prev;
+            const float bf = (float)st.beta;
             for (size_t i = 0; i < nv; ++i) d[i] = bf * d[i] + gr[i];
         }
         gg_prev = gg;

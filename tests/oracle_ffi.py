@@ -57,7 +57,7 @@ class OgSirtParams(C.Structure):
 
 class OgGdStat(C.Structure):
     _fields_ = [("residual_sq", C.c_double), ("norm_dj", C.c_double),
-                ("norm_dj_proj", C.c_double), ("alpha", C.c_double)]
+                ("norm_dj_proj", C.c_double), ("alpha", C.c_double), ("j_reg", C.c_double), ("beta", C.c_double)]
 
 
 @dataclass
@@ -284,7 +284,7 @@ class Oracle:
             rc = self.lib().og_wgd_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(w), _f32p(v), stats)
         assert rc == 0
         return v, [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj,
-                        alpha=x.alpha) for x in stats[:iters]]
+                        alpha=x.alpha, j_reg=x.j_reg, beta=x.beta) for x in stats[:iters]]
 
     def cg(self, sino, vol0=None, pair=0, cube=False, iters=3, lam=0.0, positivity=0, matched=False):
         s = np.ascontiguousarray(sino, dtype=np.float32)
@@ -294,7 +294,7 @@ class Oracle:
         stats = (OgGdStat * max(iters, 1))()
         assert self.lib().og_cg_run(C.byref(self._g), C.byref(p), _f32p(s), _f32p(v), stats) == 0
         return v, [dict(residual_sq=x.residual_sq, norm_dj=x.norm_dj, norm_dj_proj=x.norm_dj_proj,
-                        alpha=x.alpha) for x in stats[:iters]]
+                        alpha=x.alpha, j_reg=x.j_reg, beta=x.beta) for x in stats[:iters]]
 
     def sirt_weights(self, pair=0, cube=False, relax=1.0):
         p = OgSirtParams(pair=pair, cube=int(cube), iters=0, relax=relax, positivity=0)

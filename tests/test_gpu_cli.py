@@ -123,6 +123,26 @@ def test_cli_iterative_reconstruction(tmp_path):
     assert np.linalg.norm(final - ref) / np.linalg.norm(ref) <= 1e-6
     assert (res[0] / "configCT").exists() and (res[0] / "Iteration.dat").exists()
     assert "Gradient iteration 3" in out
+    # finalVolume.v (the loop's own save, C/Iter3D_Algo_Gradient.cu:214) holds the same volume
+    assert np.array_equal(np.fromfile(res[0] / "finalVolume.v", dtype=np.float32).reshape(truth.shape), final)
+    # Iteration.dat in the reference's layout (C/Config/ConfigIteration.cpp:268-303): seven
+    # setw(20) left-aligned header names, then IterNb J J_MC JReg lambda |alpha| beta eam
+    lines = (res[0] / "Iteration.dat").read_text().splitlines()
+    names = ["IterNb", "J", "J_MC", "JReg", "lambda", "alpha", "beta"]
+    assert lines[0] == "".join(f"{n:<20}" for n in names)
+    assert len(lines) == 4
+    for k, (line, st) in enumerate(zip(lines[1:], stats), start=1):
+        assert len(line) == 8 * 20 and all(line[20 * c] != " " for c in range(8)), line
+        cols = [float(line[20 * c:20 * c + 20]) for c in range(8)]
+        assert cols[0] == k and cols[4] == 0.01 and cols[6] == 0.0 and cols[7] == 0.0
+        assert cols[2] == pytest.approx(st["residual_sq"], rel=1e-5)       # 6 significant digits
+        assert cols[3] == pytest.approx(st["j_reg"], rel=1e-5) and cols[3] > 0
+        assert cols[1] == pytest.approx(cols[2] + 0.01 * cols[3], rel=1e-5)
+        assert cols[5] == pytest.approx(abs(st["alpha"]), rel=1e-5)
+    # the timings.log line, appended in the working directory (C/Iter3D_Algo_Gradient.cu:207-210)
+    tl = (tmp_path / "timings.log").read_text().splitlines()
+    assert len(tl) == 1 and tl[0].startswith("ngpu 1 nstreams 1 storage 0 precision 1 temps gradient ")
+    assert tl[0].endswith("ms")
 
 
 def test_cli_conjugate_gradient(tmp_path):

@@ -49,8 +49,9 @@ def test_gd_loop_matches_oracle(small, pair, flags, lam, kopt, positivity):
                      positivity=positivity)
     torch.cuda.synchronize()
     for a, b in zip(stats, ref_stats):
-        for k in ("alpha", "residual_sq", "norm_dj", "norm_dj_proj"):
+        for k in ("alpha", "residual_sq", "norm_dj", "norm_dj_proj", "j_reg", "beta"):
             assert a[k] == pytest.approx(b[k], rel=1e-9, abs=1e-12), (k, a, b)
+        assert (a["j_reg"] > 0) == (lam != 0)
     assert rel_l2(vol.cpu().numpy(), ref_vol) <= 1e-6
     assert stats[-1]["residual_sq"] < stats[0]["residual_sq"]
 
@@ -172,7 +173,47 @@ def test_cg_loop_matches_oracle(small, pair, lam):
     stats = G.gd_run(torch.from_numpy(sino).cuda(), vol, pair=pair, iters=5, lam=lam, conjugate=True)
     torch.cuda.synchronize()
     for a, b in zip(stats, ref_stats):
-        for k in ("alpha", "residual_sq", "norm_dj", "norm_dj_proj"):
+        for k in ("alpha", "residual_sq", "norm_dj", "norm_dj_proj", "j_reg", "beta"):
             assert a[k] == pytest.approx(b[k], rel=1e-8, abs=1e-12), (k, a, b)
     assert rel_l2(vol.cpu().numpy(), ref) <= 1e-6
     assert stats[-1]["residual_sq"] < stats[0]["residual_sq"]
+
+
+def test_gd_two_threads_two_streams_one_context(small, ctx):
+    """Two threads, each on its own stream, run tomo_gd_run on one context at
+    once (RSVI and SIDDONVI); each result equals the same call run alone, bit
+    for bit (the loops' reductions take their scratch per call)."""
+    import threading
+    g, o, G = small
+    truth = ellipsoids(g.nx, g.ny, g.nz, count=6, seed=7, amin=2.0, amax=9.0)
+    jobs = [(lib.PAIR_RSVI, torch.from_numpy(o.rs(truth)).cuda()),
+            (lib.PAIR_SIDDONVI, torch.from_numpy(o.siddon(truth)).cuda())]
+    kw = dict(iters=6, kopt=7, lam=0.01)
+    alone = []
+    for pair, y in jobs:
+        v = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+        st = G.gd_run(y, v, pair=pair, **kw)
+        alone.append((v, st))
+    torch.cuda.synchronize()
+    for rep in range(3):
+        outs = [torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda") for _ in jobs]
+        streams = [torch.cuda.Stream() for _ in jobs]
+        res, errs = [None, None], []
+        torch.cuda.synchronize()
+
+        def run(i):
+            try:
+                res[i] = G.gd_run(jobs[i][1], outs[i], pair=jobs[i][0], stream=streams[i], **kw)
+            except Exception as e:  # reported below
+                errs.append(e)
+
+        th = [threading.Thread(target=run, args=(i,)) for i in range(len(jobs))]
+        for t in th:
+            t.start()
+        for t in th:
+            t.join()
+        torch.cuda.synchronize()
+        assert not errs, errs
+        for i in range(len(jobs)):
+            assert torch.equal(outs[i], alone[i][0]), (rep, i)
+            assert res[i] == alone[i][1], (rep, i)

@@ -358,3 +358,111 @@ def test_host_pointer_path(setup):
     out = np.zeros(g.vol_shape, dtype=np.float32)
     G.backproject(sino, out)
     assert np.array_equal(out, o.vi(sino))
+
+
+# ---- the reference's C-path Siddon (TOMO_FLAG_SIDDON_COMPAT) -------------
+# SiddonProjector_compute_C_mem_CPU.cpp:20-276 runs only at 256^3 (it indexes
+# % 256); the compat kernel restates its float arithmetic and quirks and must
+# equal the oracle's restatement bit for bit.
+
+def test_siddon_compat_bit_exact_coarse_detector(ctx):
+    """256^3, 32 x 32 detector cells of 16 mm, 8 views."""
+    g = scaled_geometry(256, 8, nu=32, nv=32, pixel_mm=16.0)
+    G = lib.Geometry(ctx, to_desc(g))
+    try:
+        vol = np.random.default_rng(31).random(g.vol_shape, dtype=np.float32)
+        ref = Oracle(g).siddon_compat(vol)
+        out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        G.project(dev(vol), out, lib.PROJ_SIDDON, lib.FLAG_SIDDON_COMPAT)
+        got = out.cpu().numpy()
+        assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+    finally:
+        G.close()
+
+
+# ---- index readers (the form of copies of 4 GiB or more) ----------------
+@pytest.mark.parametrize("dt", ["f32", "bf16"])
+@pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon"])
+def test_index_reads_bit_exact(setup, op, dt):
+    """TOMO_FLAG_INDEX_READS makes the projectors read their z-pair copies
+    through 64-bit element indices, as every 1024^3 volume does; the results
+    equal the oracle's (and the buffer readers') bit for bit."""
+    g, o, G = setup
+    rng = np.random.default_rng(8)
+    vol = ellipsoids(g.nx, g.ny, g.nz, count=12, seed=5, amin=2.0, amax=12.0)
+    vol = vol + rng.random(g.vol_shape, dtype=np.float32)
+    dv = dev(vol)
+    if dt == "bf16":
+        dv = dv.to(torch.bfloat16)
+        vol = dv.float().cpu().numpy()
+    ref = o.siddon(vol) if op == "siddon" else o.rs(vol, cube=op == "rs_cube")
+    flags = lib.FLAG_INDEX_READS | (lib.FLAG_RS_FOV_CUBE if op == "rs_cube" else 0)
+    out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+    G.project(dv, out, lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS, flags)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+# ---- the 256^3 Shepp-Logan workload at its own size ------------------------
+# 256^3 Shepp-Logan (densities 0..2), 256 views over 360 deg, 256 x 256
+# detector of 2 mm, FOD 500 mm, FDD 1000 mm: one projection of each kind and
+# one backprojection against the oracle, bit for bit, and the adjoint test.
+
+@pytest.fixture(scope="module")
+def c0(ctx):
+    g = scaled_geometry(256, 256)
+    print("[case] 256^3 Shepp-Logan, 256 views", flush=True)
+    G = lib.Geometry(ctx, to_desc(g))
+    yield g, Oracle(g), G, shepp_logan(256)
+    G.close()
+
+
+@pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "siddon_compat"])
+def test_c0_projection_bit_exact(c0, op):
+    g, o, G, vol = c0
+    print(f"[case] c0 projection {op}", flush=True)
+    if op == "siddon_compat":
+        ref = o.siddon_compat(vol)
+    elif op == "siddon":
+        ref = o.siddon(vol)
+    else:
+        ref = o.rs(vol, cube=op == "rs_cube")
+    flags = {"rs": 0, "rs_cube": lib.FLAG_RS_FOV_CUBE, "siddon": 0, "siddon_compat": lib.FLAG_SIDDON_COMPAT}[op]
+    out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+    G.project(dev(vol), out, lib.PROJ_RS if op.startswith("rs") else lib.PROJ_SIDDON, flags)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+def test_c0_rsvi_backprojection_bit_exact(c0):
+    """VI of the RS sinogram of the phantom (the RSVI pair), bit for bit."""
+    g, o, G, vol = c0
+    sino = o.rs(vol)
+    ref = o.vi(sino)
+    out = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+    G.backproject(dev(sino), out)
+    got = out.cpu().numpy()
+    assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
+
+
+@pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "vi"])
+def test_c0_adjoint_dot(c0, ctx, op):
+    """<A x, y> = <x, A^T y> at 256^3 / 256 views, x, y ~ U[0,1) (seeds 1, 2)."""
+    g, o, G, _ = c0
+    x = dev(np.random.default_rng(1).random(g.vol_shape, dtype=np.float32))
+    y = dev(np.random.default_rng(2).random(g.sino_shape, dtype=np.float32))
+    if op == "vi":
+        bx = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.backproject(y, bx)
+        bty = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        G.backproject_adjoint(x, bty)
+        lhs, rhs = lib.dot(ctx, bx, x), lib.dot(ctx, y, bty)
+    else:
+        proj = lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS
+        flags = lib.FLAG_RS_FOV_CUBE if op == "rs_cube" else 0
+        ax = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        G.project(x, ax, proj, flags)
+        aty = torch.empty(g.vol_shape, dtype=torch.float32, device="cuda")
+        G.project_adjoint(y, aty, proj, flags)
+        lhs, rhs = lib.dot(ctx, ax, y), lib.dot(ctx, x, aty)
+    assert abs(lhs - rhs) / max(abs(lhs), abs(rhs)) <= 1e-5, (lhs, rhs)

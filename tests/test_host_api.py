@@ -28,7 +28,7 @@ def test_library_exports_every_declared_symbol():
     missing = [n for n in names if not hasattr(L, n)]
     assert not missing, missing
     assert sorted(lib.EXPORTS) == names
-    assert L.tomo_abi_version() == 3
+    assert L.tomo_abi_version() == 4
 
 
 def test_config_ct_of_shipped_dataset():

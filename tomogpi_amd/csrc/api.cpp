@@ -84,9 +84,21 @@ int run_operator(tomo_ctx* ctx, const tomo_geom* g, int flags, const void* in, i
     return TOMO_OK;
 }
 
-int do_rs(const tomo_geom* g, int cube, Buf in, MutBuf out, hipStream_t s) { return launch_rs(g, cube, in, out, s); }
-int do_siddon(const tomo_geom* g, int compat, Buf in, MutBuf out, hipStream_t s) {
-    return launch_siddon(g, compat, in, out, s);
+// Projector modes: bit 0 = cube FOV (RS) / compat (Siddon), bit 1 = index reads.
+constexpr int kModeIndexReads = 2;
+int proj_mode(int flags, int fov_or_compat) {
+    return ((flags & fov_or_compat) ? 1 : 0) | ((flags & TOMO_FLAG_INDEX_READS) ? kModeIndexReads : 0);
+}
+ProjExtra extra_of(int mode) {
+    ProjExtra x;
+    x.index_reads = (mode & kModeIndexReads) != 0;
+    return x;
+}
+int do_rs(const tomo_geom* g, int mode, Buf in, MutBuf out, hipStream_t s) {
+    return launch_rs(g, mode & 1, in, out, s, extra_of(mode));
+}
+int do_siddon(const tomo_geom* g, int mode, Buf in, MutBuf out, hipStream_t s) {
+    return launch_siddon(g, mode & 1, in, out, s, extra_of(mode));
 }
 int do_vi(const tomo_geom* g, int fdk, Buf in, MutBuf out, hipStream_t s) { return launch_vi(g, in, out, s, fdk); }
 int do_rs_adj(const tomo_geom* g, int cube, Buf in, MutBuf out, hipStream_t s) {
@@ -151,9 +163,6 @@ int tomo_ctx_create(int device, tomo_ctx** out) {
 
 int tomo_ctx_destroy(tomo_ctx* ctx) {
     if (!ctx) return TOMO_OK;
-    (void)hipSetDevice(ctx->device);
-    if (ctx->d_partial) (void)hipFree(ctx->d_partial);
-    if (ctx->h_result) (void)hipHostFree(ctx->h_result);
     delete ctx;
     return TOMO_OK;
 }
@@ -244,11 +253,10 @@ int tomo_project(tomo_ctx* ctx, const tomo_geom* g, int projector, int flags, co
     if (!g) return fail(TOMO_ERR_INVALID_ARG, "null geometry");
     if (projector == TOMO_PROJ_RS)
         return run_operator(ctx, g, flags, volume, volume_dtype, volume_scale, g->nvox(), sinogram, sinogram_dtype,
-                            sinogram_scale, g->nsino(), stream, do_rs, (flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0);
+                            sinogram_scale, g->nsino(), stream, do_rs, proj_mode(flags, TOMO_FLAG_RS_FOV_CUBE));
     if (projector == TOMO_PROJ_SIDDON)
         return run_operator(ctx, g, flags, volume, volume_dtype, volume_scale, g->nvox(), sinogram, sinogram_dtype,
-                            sinogram_scale, g->nsino(), stream, do_siddon,
-                            (flags & TOMO_FLAG_SIDDON_COMPAT) ? 1 : 0);
+                            sinogram_scale, g->nsino(), stream, do_siddon, proj_mode(flags, TOMO_FLAG_SIDDON_COMPAT));
     return fail(TOMO_ERR_INVALID_ARG, "unknown projector");
 }
 

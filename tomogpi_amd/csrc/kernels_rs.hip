@@ -414,17 +414,17 @@ __global__ __launch_bounds__(BX* BY) void rs_count_kernel(DevGeom g, int cube, u
 
 }  // namespace
 
-int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s) {
+int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s, const ProjExtra& x) {
     const DevGeom& d = g->dev;
     const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
     const float inv = 1.0f / vol.scale;
     const uint32_t es = vol.dtype == TOMO_F32 ? 4u : 2u;
     ZPairs zp;
-    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s);
+    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s, x);
     const void* cx = zp.qx;
     const void* cy = zp.qy;
     if (le == hipSuccess) {
-        const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, es).buf;
+        const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, es).buf && !x.index_reads;
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)

@@ -592,7 +592,7 @@ __global__ __launch_bounds__(BX* BY) void siddon_compat_kernel(DevGeom g, const 
 
 }  // namespace
 
-int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStream_t s) {
+int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStream_t s, const ProjExtra& x) {
     const DevGeom& d = g->dev;
     if (compat && (d.nx != 256 || d.ny != 256 || d.nz != 256 || d.nu > 256 || d.nv > 256))
         return fail(TOMO_ERR_UNSUPPORTED,
@@ -619,9 +619,9 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
         return fail(TOMO_ERR_UNSUPPORTED, "volume too large for the 32-bit traversal index of the Siddon projector");
     const size_t qx_bytes = qx_elems * es, qy_bytes = qy_elems * es;
     ZPairs zp;
-    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s);
+    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s, x);
     if (le == hipSuccess) {
-        const bool buf = qx_bytes < (1ull << 32) && qy_bytes < (1ull << 32);
+        const bool buf = qx_bytes < (1ull << 32) && qy_bytes < (1ull << 32) && !x.index_reads;
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)

@@ -565,47 +565,55 @@ int launch_laplacian(const float* x, float* y, int nx, int ny, int nz, hipStream
     return TOMO_OK;
 }
 
-static int ensure_partial(tomo_ctx* ctx) {
-    if (!ctx->d_partial) {
-        TOMO_HIP(hipMalloc((void**)&ctx->d_partial, sizeof(double) * (kReduceBlocks + 1)));
-        ctx->partial_cap = kReduceBlocks;
-    }
-    if (!ctx->h_result) TOMO_HIP(hipHostMalloc((void**)&ctx->h_result, sizeof(double)));
-    return TOMO_OK;
-}
-
-static int finish_reduce(tomo_ctx* ctx, int blocks, double* out, hipStream_t s) {
-    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(kBlock), 0, s, ctx->d_partial, blocks,
-                       ctx->d_partial + kReduceBlocks);
-    TOMO_HIP(hipGetLastError());
-    TOMO_HIP(hipMemcpyAsync(ctx->h_result, ctx->d_partial + kReduceBlocks, sizeof(double), hipMemcpyDeviceToHost, s));
-    TOMO_HIP(hipStreamSynchronize(s));
-    *out = *ctx->h_result;
+// Each reduction takes its partial sums from the stream-ordered pool for the
+// duration of the call, so calls from several threads on one context (each
+// on its own stream) never share them.  The final value is copied to the
+// host and the stream synchronised before the call returns.
+static int finish_reduce(double* part, int blocks, double* out, hipStream_t s) {
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(kBlock), 0, s, part, blocks, part + kReduceBlocks);
+    hipError_t e = hipGetLastError();
+    double h = 0.0;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, part + kReduceBlocks, sizeof(double), hipMemcpyDeviceToHost, s);
+    const hipError_t f = hipFreeAsync(part, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    TOMO_HIP(e);
+    TOMO_HIP(f);
+    *out = h;
     return TOMO_OK;
 }
 
 int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s) {
-    int rc = ensure_partial(ctx);
-    if (rc) return rc;
+    (void)ctx;
+    double* part = nullptr;
+    TOMO_HIP(hipMallocAsync((void**)&part, sizeof(double) * (kReduceBlocks + 1), s));
     unsigned blocks = grid_for(n);
     if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
     const float inv = 1.0f / x.scale;
-    TOMO_DISPATCH_STORAGE(x.dtype, T, {
-        hipLaunchKernelGGL((sq_partial_kernel<T>), dim3(blocks), dim3(kBlock), 0, s, (const T*)x.p, inv, n,
-                           ctx->d_partial);
-    });
-    TOMO_HIP(hipGetLastError());
-    return finish_reduce(ctx, (int)blocks, out, s);
+    switch (x.dtype) {
+        case TOMO_F32:
+            hipLaunchKernelGGL((sq_partial_kernel<float>), dim3(blocks), dim3(kBlock), 0, s, (const float*)x.p, inv,
+                               n, part);
+            break;
+        case TOMO_F16:
+            hipLaunchKernelGGL((sq_partial_kernel<__half>), dim3(blocks), dim3(kBlock), 0, s, (const __half*)x.p,
+                               inv, n, part);
+            break;
+        default:
+            hipLaunchKernelGGL((sq_partial_kernel<__hip_bfloat16>), dim3(blocks), dim3(kBlock), 0, s,
+                               (const __hip_bfloat16*)x.p, inv, n, part);
+            break;
+    }
+    return finish_reduce(part, (int)blocks, out, s);
 }
 
 int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s) {
-    int rc = ensure_partial(ctx);
-    if (rc) return rc;
+    (void)ctx;
+    double* part = nullptr;
+    TOMO_HIP(hipMallocAsync((void**)&part, sizeof(double) * (kReduceBlocks + 1), s));
     unsigned blocks = grid_for(n);
     if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
-    hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, a, b, n, ctx->d_partial);
-    TOMO_HIP(hipGetLastError());
-    return finish_reduce(ctx, (int)blocks, out, s);
+    hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, a, b, n, part);
+    return finish_reduce(part, (int)blocks, out, s);
 }
 
 int launch_ellipsoids(const tomo_geom* g, int mode, const double* d_table, int count, float* vol, hipStream_t s) {

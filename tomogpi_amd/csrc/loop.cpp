@@ -50,7 +50,7 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
     double alpha_min = pr.alpha_min_init;
     int rc;
     for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
-        tomo_gd_iter_stat st{0, 0, 0, 0};
+        tomo_gd_iter_stat st{0, 0, 0, 0, 0, 0};
         if ((rc = project(g, pr, vol, e, s))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
@@ -62,6 +62,7 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
             float* a = (float*)t1.p;
             float* b = (float*)t2.p;
             if ((rc = launch_laplacian(vol, a, g->info.nx, g->info.ny, g->info.nz, s))) return rc;
+            if ((rc = reduce_sq_norm(ctx, Buf{a, TOMO_F32, 1.0f}, nv, &st.j_reg, s))) return rc;
             if ((rc = launch_laplacian(a, b, g->info.nx, g->info.ny, g->info.nz, s))) return rc;
             const float w = (float)(2.0 * (double)(float)pr.lambda);
             if ((rc = launch_axpy(d, b, w, 0, nv, s))) return rc;
@@ -117,7 +118,7 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
     double gg_prev = 0.0;
     int rc;
     for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
-        tomo_gd_iter_stat st{0, 0, 0, 0};
+        tomo_gd_iter_stat st{0, 0, 0, 0, 0, 0};
         if ((rc = project(g, pr, vol, e, s))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
@@ -125,6 +126,7 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         if ((rc = launch_scale(gr, -2.0f, nv, s))) return rc;
         if (pr.lambda != 0.0) {
             if ((rc = launch_laplacian(vol, a, nx, ny, nz, s))) return rc;
+            if ((rc = reduce_sq_norm(ctx, Buf{a, TOMO_F32, 1.0f}, nv, &st.j_reg, s))) return rc;
             if ((rc = launch_laplacian(a, b, nx, ny, nz, s))) return rc;
             if ((rc = launch_axpy(gr, b, (float)(2.0 * (double)(float)pr.lambda), 0, nv, s))) return rc;
         }
@@ -133,7 +135,8 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         if (k == 1) {
             TOMO_HIP(hipMemcpyAsync(d, gr, nv * sizeof(float), hipMemcpyDeviceToDevice, s));
         } else {
-            if ((rc = launch_scale(d, (float)(gg / gg_prev), nv, s))) return rc;
+            st.beta = gg / gg_prev;
+            if ((rc = launch_scale(d, (float)st.beta, nv, s))) return rc;
             if ((rc = launch_axpy(d, gr, 1.0f, 0, nv, s))) return rc;
         }
         gg_prev = gg;

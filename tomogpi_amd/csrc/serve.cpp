@@ -2,12 +2,13 @@
 // reconstructions of uneven sizes on one device.
 //
 // A worker thread owns one stream.  Requests wait in arrival order; the head
-// request is admitted when its device footprint fits the memory budget and
-// a slot is free (no reordering, so a large request is never overtaken
-// indefinitely).  Admission carves one stream-ordered allocation into the
-// request's resident state (view tables, volume x, gradient dJ, Laplacian
-// work volumes, z-pair copies, sinograms y / P x / residual / P dJ, the VI
-// tap-quad copy, FP64 partial sums, loop scalars) and stages its inputs.
+// request is admitted when its device footprint fits the free device memory
+// (or a fixed budget) and a slot is free (no reordering, so a large request
+// is never overtaken indefinitely).  Admission carves one stream-ordered
+// allocation into the request's resident state (view tables, volume x,
+// gradient dJ, Laplacian work volumes, z-pair copies, sinograms y / P x /
+// residual / P dJ, the VI box table, FP64 partial sums, loop scalars) and
+// stages its inputs.
 //
 // A round advances every admitted request by one or more iterations of the
 // loop of tomo_gd_run (C/Iter3D_Algo_Gradient.cu:25-223 with the beta=2
@@ -207,6 +208,7 @@ int admit(tomo_server* srv, Request& q) {
     int rc;
     // the VI box table of the geometry, read by every round's backprojection
     if ((rc = launch_vi_boxes(q.geom.dev, q.tq, s))) return rc;
+    q.geom.d_vi_boxes = q.tq;  // the single-problem VI below uses it too
     if (sirt) {
         // R = 1/(P 1) into pd, C = relaxation/(B 1) into t1, with the
         // single-problem operators (the same arithmetic as tomo_sirt_run).
@@ -544,18 +546,30 @@ int capture_graphs(tomo_server* srv) {
 }
 
 // Copies the result out and releases the request's memory (worker, no lock).
+// The block is released whatever the copies return.
 int retire(tomo_server* srv, Request& q) {
     hipStream_t s = srv->stream;
     const size_t nv = q.geom.nvox();
-    TOMO_HIP(hipMemcpyAsync(q.req.volume, q.x, nv * 4, q.host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s));
+    hipError_t e = hipMemcpyAsync(q.req.volume, q.x, nv * 4, q.host ? hipMemcpyDeviceToHost : hipMemcpyDeviceToDevice, s);
     q.hist_host.assign((size_t)q.req.gd.gradient_iteration_nb * 4, 0.0);
-    if (!q.hist_host.empty())
-        TOMO_HIP(hipMemcpyAsync(q.hist_host.data(), q.hist, q.hist_host.size() * 8, hipMemcpyDeviceToHost, s));
-    TOMO_HIP(hipFreeAsync(q.block, s));
-    TOMO_HIP(hipStreamSynchronize(s));
+    if (e == hipSuccess && !q.hist_host.empty())
+        e = hipMemcpyAsync(q.hist_host.data(), q.hist, q.hist_host.size() * 8, hipMemcpyDeviceToHost, s);
+    const hipError_t f = hipFreeAsync(q.block, s);
+    const hipError_t y = hipStreamSynchronize(s);
     q.block = nullptr;
+    TOMO_HIP(e);
+    TOMO_HIP(f);
+    TOMO_HIP(y);
     if (!q.hist_host.empty()) q.residual_sq = q.hist_host[q.hist_host.size() - 4];
     return TOMO_OK;
+}
+
+// Releases a request's block without copying anything out (failed rounds).
+void drop_block(tomo_server* srv, Request& q) {
+    if (!q.block) return;
+    (void)hipFreeAsync(q.block, srv->stream);
+    (void)hipStreamSynchronize(srv->stream);
+    q.block = nullptr;
 }
 
 // Caller holds srv->mu.
@@ -577,6 +591,28 @@ void finish_request(tomo_server* srv, Request& q, int status, const std::string&
     }
 }
 
+// Bytes further requests may take now.  With a fixed budget: what the budget
+// leaves.  Otherwise (live admission) the device's free memory, plus what the
+// server's pool holds without handing it out (blocks of finished requests
+// stay cached there), less a reserve of 1/16 of the device; the budget shown
+// in the statistics follows it.  Caller holds srv->mu; the worker's device is
+// current.
+uint64_t admission_room(tomo_server* srv) {
+    if (srv->params.hbm_budget_bytes)
+        return srv->budget > srv->bytes_in_use ? srv->budget - srv->bytes_in_use : 0;
+    size_t free_b = 0, total_b = 0;
+    if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return 0;
+    uint64_t reserved = 0, used = 0;
+    if (hipMemPoolGetAttribute(srv->pool, hipMemPoolAttrReservedMemCurrent, &reserved) != hipSuccess ||
+        hipMemPoolGetAttribute(srv->pool, hipMemPoolAttrUsedMemCurrent, &used) != hipSuccess)
+        reserved = used = 0;
+    const uint64_t avail = (uint64_t)free_b + (reserved > used ? reserved - used : 0);
+    const uint64_t keep = (uint64_t)total_b / 16;
+    const uint64_t room = avail > keep ? avail - keep : 0;
+    srv->budget = srv->bytes_in_use + room;
+    return room;
+}
+
 void worker_main(tomo_server* srv) {
     (void)hipSetDevice(srv->ctx->device);
     std::vector<std::shared_ptr<Request>> running;  // admitted, owned by this thread
@@ -585,20 +621,26 @@ void worker_main(tomo_server* srv) {
         {
             std::unique_lock<std::mutex> lk(srv->mu);
             for (;;) {
-                // Admission: the head of the queue while it fits.
+                // Admission: the head of the queue while it fits the room
+                // measured at the start of this pass, less what the pass has
+                // admitted already (allocated after the lock is released).
+                uint64_t room = srv->queue.empty() ? 0 : admission_room(srv);
                 while (!srv->queue.empty() && !srv->broken) {
                     auto q = srv->queue.front();
                     const uint64_t need =
                         layout_of(q->geom, q->req.gd, q->req.algorithm, q->req.storage == TOMO_BF16 ? 2 : 4).total;
                     if ((int)(running.size() + admitted.size()) >= srv->params.max_active) break;
-                    if (srv->bytes_in_use + need > srv->budget) {
+                    if (need > room) {
                         if (!running.empty() || !admitted.empty()) break;
-                        // Larger than the whole budget: it can never be admitted.
+                        // Larger than all the memory it could ever be given.
                         srv->queue.pop_front();
-                        finish_request(srv, *q, TOMO_ERR_OUT_OF_MEMORY, "request exceeds the server's memory budget");
+                        finish_request(srv, *q, TOMO_ERR_OUT_OF_MEMORY,
+                                       srv->params.hbm_budget_bytes ? "request exceeds the server's memory budget"
+                                                                    : "request exceeds the free device memory");
                         srv->cv_done.notify_all();
                         continue;
                     }
+                    room -= need;
                     srv->queue.pop_front();
                     q->bytes = need;
                     srv->bytes_in_use += need;
@@ -639,7 +681,10 @@ void worker_main(tomo_server* srv) {
             ++srv->rounds;
             if (rc) {
                 srv->broken = true;
-                for (auto& q : running) finish_request(srv, *q, rc, "round failed");
+                for (auto& q : running) {
+                    drop_block(srv, *q);
+                    finish_request(srv, *q, rc, "round failed");
+                }
                 running.clear();
                 for (auto& q : srv->queue) finish_request(srv, *q, rc, "server failed");
                 srv->queue.clear();
@@ -684,9 +729,7 @@ extern "C" int tomo_serve_create(tomo_ctx* ctx, const tomo_serve_params* params,
     srv->params = *params;
     srv->sub_rounds = params->sub_rounds ? params->sub_rounds : kMaxSubRounds;
     srv->t0 = std::chrono::steady_clock::now();
-    size_t free_b = 0, total_b = 0;
-    TOMO_HIP(hipMemGetInfo(&free_b, &total_b));
-    srv->budget = params->hbm_budget_bytes ? params->hbm_budget_bytes : free_b - free_b / 16;
+    srv->budget = params->hbm_budget_bytes;  // 0: live admission (admission_room)
     // A pool of its own, whose memory stays cached between requests
     // (stream-ordered reuse without returning it to the device).
     hipMemPoolProps pp;
@@ -828,6 +871,8 @@ extern "C" int tomo_serve_iter_stats(tomo_server* srv, uint64_t id, tomo_gd_iter
         stats[i].norm_dj = q.hist_host[4 * i + 1];
         stats[i].norm_dj_proj = q.hist_host[4 * i + 2];
         stats[i].alpha = q.hist_host[4 * i + 3];
+        stats[i].j_reg = 0.0;
+        stats[i].beta = 0.0;
     }
     return TOMO_OK;
 }

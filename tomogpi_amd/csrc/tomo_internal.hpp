@@ -47,11 +47,6 @@ constexpr int kViewStride = 8;
 
 struct tomo_ctx {
     int device = 0;
-    hipStream_t own_stream = nullptr;
-    // Scratch for reductions: partial sums (double) and the final value.
-    double* d_partial = nullptr;
-    size_t partial_cap = 0;
-    double* h_result = nullptr;  // pinned
 };
 
 struct tomo_geom {
@@ -95,9 +90,21 @@ struct MutBuf {
 // fdk: FDK-weighted form (back_fdk) instead of delta * sum.
 int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk = 0);
 int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s);
-int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s);
+// Projector options beyond the FOV / compat mode.  index_reads: read the
+// z-pair copies through 64-bit element indices even where 32-bit buffer
+// offsets apply (the form every copy of 4 GiB or more takes).  qx, qy: z-pair
+// copies in memory the caller provides (zpair_elems_x / _y elements of the
+// volume's element size each); null: the launcher allocates them from the
+// stream-ordered pool for the call.
+struct ProjExtra {
+    bool index_reads = false;
+    void* qx = nullptr;
+    void* qy = nullptr;
+};
+int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s, const ProjExtra& x = ProjExtra{});
 int launch_rs_adjoint(const tomo_geom* g, int cube, Buf sino, MutBuf vol, hipStream_t s);
-int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStream_t s);
+int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStream_t s,
+                  const ProjExtra& x = ProjExtra{});
 int launch_siddon_adjoint(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s);
 // Work counters (roofline accounting): sum of L_r (RS) and number of
 // positive chords (Siddon).  Synchronous.

@@ -29,6 +29,9 @@
 //               offer) reconstruction from the same sinogram, starting
 //               from configCT:initVolumeName (or zero), writes
 //               <workdir>/TomoIter/Results/Result_<date>/final_Volume.v
+//               (unless --no_save) and finalVolume.v, Iteration.dat and the
+//               copied config files there, and appends a line to
+//               ./timings.log
 // Every failure prints a message and returns a nonzero status.
 #include <sys/stat.h>
 
@@ -39,6 +42,7 @@
 #include <cstring>
 #include <ctime>
 #include <fstream>
+#include <iomanip>
 #include <string>
 #include <vector>
 
@@ -140,8 +144,16 @@ int main(int argc, char** argv) {
     if (flag(argc, argv, "no_load", nullptr)) o.load = false;
     if (flag(argc, argv, "no_save", nullptr)) o.save = false;
     o.have_op = flag(argc, argv, "op", &o.op);
-    if (flag(argc, argv, "ngpu", &v) || flag(argc, argv, "nstreams", &v))
-        std::printf("note: --ngpu/--nstreams ignored (one GPU, one stream)\n");
+    // Devices and streams of the projection (configComputeArchitecture
+    // gpuNbP / streamsNbP, overridden by --ngpu / --nstreams): recorded in
+    // timings.log; the work runs on one GPU and one stream whatever they say.
+    int gpu_nb = std::atoi(cfg(work, "configComputeArchitecture", "gpuNbP", "1").c_str());
+    int streams_nb = std::atoi(cfg(work, "configComputeArchitecture", "streamsNbP", "1").c_str());
+    if (flag(argc, argv, "ngpu", &v)) gpu_nb = std::atoi(v.c_str());
+    if (flag(argc, argv, "nstreams", &v)) streams_nb = std::atoi(v.c_str());
+    if (gpu_nb != 1 || streams_nb != 1)
+        std::printf("note: %d GPU(s) / %d stream(s) requested; the work runs on one GPU and one stream\n", gpu_nb,
+                    streams_nb);
 
     const bool device = o.storage == "mem_GPU";
     int proj = TOMO_PROJ_RS, pflags = 0;
@@ -304,17 +316,45 @@ int main(int argc, char** argv) {
         else
             CHECK(tomo_gd_run(ctx, geom, &gp, sino.data(), vol.data(), st.data(), nullptr), "gradient descent");
         t1 = clk::now();
-        FILE* log = std::fopen((res + "Iteration.dat").c_str(), "w");
-        if (log) std::fprintf(log, "IterNb residual_sq normdJ normdJProj alpha\n");
+        // Iteration.dat as ConfigIteration::createIterationLogFile / updateLogFile
+        // write it (C/Config/ConfigIteration.cpp:268-303): a header of seven
+        // setw(20) left-aligned names, then per iteration IterNb J J_MC JReg
+        // lambda |alpha| beta eam in default ostream formatting.  J_MC is the
+        // data term sum (y - P x)^2 (the reference leaves its own at 0: the
+        // assignment at C/Iter3D_Algo_Gradient.cu:88 is commented out), JReg
+        // = sum (L x)^2, J = J_MC + lambda JReg, beta the conjugate
+        // gradient's, eam 0 (no reference volume is loaded).
+        std::ofstream log((res + "Iteration.dat").c_str(), std::ios::out);
+        if (log.is_open()) {
+            log << std::setw(20) << std::left << "IterNb" << std::setw(20) << std::left << "J" << std::setw(20)
+                << std::left << "J_MC" << std::setw(20) << std::left << "JReg" << std::setw(20) << std::left
+                << "lambda" << std::setw(20) << std::left << "alpha" << std::setw(20) << std::left << "beta"
+                << std::endl;
+        }
         for (int k = 0; k < iters; ++k) {
             std::printf("Gradient iteration %d: |y-Px|^2 = %.9g  alpha = %.9g\n", k + 1, st[k].residual_sq,
                         st[k].alpha);
-            if (log)
-                std::fprintf(log, "%d %.17g %.17g %.17g %.17g\n", k + 1, st[k].residual_sq, st[k].norm_dj,
-                             st[k].norm_dj_proj, st[k].alpha);
+            if (!log.is_open()) continue;
+            const double j_mc = st[k].residual_sq, j_reg = st[k].j_reg;
+            const double j = j_mc + lambda * j_reg;
+            log << std::setw(20) << std::left << (unsigned short)(k + 1) << std::setw(20) << std::left << j
+                << std::setw(20) << std::left << j_mc << std::setw(20) << std::left << j_reg << std::setw(20)
+                << std::left << lambda << std::setw(20) << std::left << std::fabs(st[k].alpha) << std::setw(20)
+                << std::left << st[k].beta << std::setw(20) << std::left << 0.0 << std::endl;
         }
-        if (log) std::fclose(log);
-        if (o.save) CHECK(write_file(res + "final_Volume.v", vol, nvox), "final_Volume.v");
+        log.close();
+        // The loop's own outputs (C/Iter3D_Algo_Gradient.cu:202-214): the
+        // "timings.log" line appended in the working directory, and
+        // finalVolume.v in the results directory whatever --no_save says.
+        const long gms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();
+        std::printf("Gradient computation :%ldms\n", gms);
+        {
+            std::ofstream tl("timings.log", std::ios::app);
+            tl << "ngpu " << gpu_nb << " nstreams " << streams_nb << " storage " << (device ? 1 : 0) << " precision "
+               << (o.precision == "half" ? 0 : o.precision == "double" ? 2 : 1) << " temps gradient " << gms
+               << "ms\n";
+        }
+        CHECK(write_file(res + "finalVolume.v", vol, nvox), "finalVolume.v");
         std::printf("results in %s\n", res.c_str());
     }
     const long ms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();

@@ -23,22 +23,34 @@ __device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int
 // Fills qx and qy from an x-fastest volume of es-byte elements (es 4 or 2).
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s);
 
-// Both copies in the stream-ordered pool for the duration of one operator
-// call: alloc() then release() on the same stream.
+// Both copies for the duration of one operator call, filled from the volume:
+// alloc() then release() on the same stream.  The copies come from the
+// stream-ordered pool, or are the caller's (ProjExtra::qx, qy), which
+// release() leaves alone.
 struct ZPairs {
     void* qx = nullptr;
     void* qy = nullptr;
-    hipError_t alloc(const void* vol, size_t es, int nx, int ny, int nz, hipStream_t s) {
-        hipError_t e = hipMallocAsync(&qx, zpair_elems_x(nx, ny, nz) * es, s);
-        if (e == hipSuccess) e = hipMallocAsync(&qy, zpair_elems_y(nx, ny, nz) * es, s);
+    bool owned = false;
+    hipError_t alloc(const void* vol, size_t es, int nx, int ny, int nz, hipStream_t s,
+                     const ProjExtra& x = ProjExtra{}) {
+        hipError_t e = hipSuccess;
+        if (x.qx && x.qy) {
+            qx = x.qx;
+            qy = x.qy;
+        } else {
+            owned = true;
+            e = hipMallocAsync(&qx, zpair_elems_x(nx, ny, nz) * es, s);
+            if (e == hipSuccess) e = hipMallocAsync(&qy, zpair_elems_y(nx, ny, nz) * es, s);
+        }
         if (e == hipSuccess) e = launch_zpair(vol, qx, qy, es, nx, ny, nz, s);
         return e;
     }
     hipError_t release(hipStream_t s) {
         hipError_t e = hipSuccess, e2 = hipSuccess;
-        if (qy) e = hipFreeAsync(qy, s);
-        if (qx) e2 = hipFreeAsync(qx, s);
+        if (owned && qy) e = hipFreeAsync(qy, s);
+        if (owned && qx) e2 = hipFreeAsync(qx, s);
         qx = qy = nullptr;
+        owned = false;
         return e != hipSuccess ? e : e2;
     }
 };

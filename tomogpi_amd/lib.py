@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(PKG, "libtomogpi_mi355x.so")
 TOMO_OK = 0
 F32, F16, BF16, F64 = 0, 1, 2, 3
 PROJ_RS, PROJ_SIDDON, BP_VI = 0, 1, 0
-FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR, FLAG_FDK = 0x1, 0x2, 0x4, 0x8
+FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR, FLAG_FDK, FLAG_INDEX_READS = 0x1, 0x2, 0x4, 0x8, 0x10
 PAIR_RSVI, PAIR_SIDDONVI = 0, 1
 SCHEDULE_DIRECT, SCHEDULE_RECURRENT = 0, 1
 ALGO_GD, ALGO_SIRT = 0, 1
@@ -79,7 +79,7 @@ class GdParams(C.Structure):
 
 class GdIterStat(C.Structure):
     _fields_ = [("residual_sq", C.c_double), ("norm_dj", C.c_double), ("norm_dj_proj", C.c_double),
-                ("alpha", C.c_double)]
+                ("alpha", C.c_double), ("j_reg", C.c_double), ("beta", C.c_double)]
 
 
 class ServeParams(C.Structure):
@@ -387,7 +387,8 @@ class Geometry:
         else:
             fn = self.lib.tomo_cg_run if conjugate else self.lib.tomo_gd_run
             check(fn(self.ctx.h, self.h, C.byref(p), C.c_void_p(ps), C.c_void_p(pv), stats, st))
-        return [dict(residual_sq=s.residual_sq, norm_dj=s.norm_dj, norm_dj_proj=s.norm_dj_proj, alpha=s.alpha)
+        return [dict(residual_sq=s.residual_sq, norm_dj=s.norm_dj, norm_dj_proj=s.norm_dj_proj, alpha=s.alpha,
+                     j_reg=s.j_reg, beta=s.beta)
                 for s in stats[:iters]]
 
     def sirt_run(self, sino, vol, pair=PAIR_RSVI, flags=0, iters=20, relaxation=1.0, positivity=0, stream=None):
