@@ -10,10 +10,18 @@ volume followed by a VI backprojection of the sinogram, device-resident
 GUPS = voxel-view updates / s = 2 N^3 Nphi / t_step.  The RS projector is
 timed alongside.  Kernel times come from HIP events recorded on the stream
 the kernels run on.  One JSON line is printed, with these further legs:
-  mixed    FP16 / BF16 storage forms against FP32 (speed, error vs FP64)
-  serving  mixed-size reconstruction stream, served vs sequential, p50/p99
-  batched  uneven SIRT problems in shared launches vs one at a time
-  cpu_baseline  the FP32 oracle on the host cores (views 0..3)
+  loop       10 gradient iterations at 512^3: SIDDONVI and RSVI in FP32,
+             SIDDONVI in its FP16 / BF16 mixed-precision forms
+  mixed      FP16 / BF16 storage forms against FP32 (speed, error vs FP64
+             and vs FP32 on the GPU, relative L2 and Linf)
+  mixed_c2   the 720-view, 768x512-detector workload at 512^3 and 1024^3:
+             FP32 checked against the oracle, then the 16-bit forms
+  serving    mixed-size reconstruction stream, served vs sequential, p50/p99
+  batched    uneven SIRT problems in shared launches vs one at a time
+  cpu_baseline  the FP32 oracle on the host cores (views 0..3), and the
+             reference C-path Siddon on one thread
+The GPU is released before the process ends: every server is closed and
+the device synchronised.
 """
 from __future__ import annotations
 
@@ -40,12 +48,18 @@ def parse():
     ap.add_argument("--nphi", type=int, default=512)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-views", type=int, default=4, help="views in the CPU baseline sample")
-    ap.add_argument("--serving-requests", type=int, default=24,
+    ap.add_argument("--serving-requests", type=int, default=200,
                     help="requests in the serving sample (0 skips the serving leg)")
-    ap.add_argument("--serving-rates", default="2,5,10,20,35,50", help="Poisson arrival rates (req/s)")
-    ap.add_argument("--serving-schedule", choices=["direct", "recurrent"], default="recurrent")
+    ap.add_argument("--serving-rates", default="10,50", help="Poisson arrival rates (req/s)")
+    ap.add_argument("--serving-schedules", default="recurrent,direct",
+                    help="schedules of the FP32-copy serving runs (the direct one on --serving-direct-requests)")
+    ap.add_argument("--serving-direct-requests", type=int, default=64)
+    ap.add_argument("--serving-bf16-rate", type=float, default=50.0, help="rate of the BF16-copy run (0: none)")
     ap.add_argument("--mixed-oracle-views", type=int, default=8,
-                    help="views of the FP64 oracle check of the mixed forms (0 skips the mixed leg)")
+                    help="views of the FP64 oracle check of the mixed forms at 512^3 / 512 views "
+                         "(0 skips the mixed legs)")
+    ap.add_argument("--c2-oracle-views", type=int, default=32,
+                    help="views of the oracle checks of the 720-view workload at 512^3 (a quarter at 1024^3)")
     ap.add_argument("--c2-sizes", default="512,1024",
                     help="edges of the mixed-precision workload (720 views, 768x512 detector); empty skips it")
     ap.add_argument("--batch-problems", type=int, default=64,
@@ -66,6 +80,20 @@ def load_traffic(n, nphi):
         return {}
     for line in open(path):
         if line.startswith("traffic_json:"):
+            d = json.loads(line.split(":", 1)[1])
+            if d.get("n") == n and d.get("nphi") == nphi:
+                return d.get("kernels", {})
+    return {}
+
+
+def load_valu(n, nphi):
+    """VALU issue fractions per kernel (profiles/valu_issue_512.md), for the
+    workload they were measured on."""
+    path = os.path.join(ROOT, "profiles", "valu_issue_512.md")
+    if not os.path.exists(path):
+        return {}
+    for line in open(path):
+        if line.startswith("valu_json:"):
             d = json.loads(line.split(":", 1)[1])
             if d.get("n") == n and d.get("nphi") == nphi:
                 return d.get("kernels", {})
@@ -123,6 +151,11 @@ def cpu_baseline(vol_host, n, nphi, views, threads):
     _, t0_vi = clock(lambda: o0.vi(s0))
     _, t0_rs = clock(lambda: o0.rs(v0))
     upd0 = 256.0 ** 3 * 256
+    # The reference's own C path (SiddonProjector_compute_C_mem_CPU.cpp:20-276, single-threaded as there), on
+    # views 0..compat_views-1 of the same 256-view acquisition.
+    compat_views = 32
+    oc = sub(256, compat_views, 256)
+    _, tc = clock(lambda: oc.siddon_compat(v0, threads=1))
     model = "unknown"
     try:
         for line in open("/proc/cpuinfo"):
@@ -142,6 +175,9 @@ def cpu_baseline(vol_host, n, nphi, views, threads):
         "vi_gups": round(upd / t_vi / 1e9, 4),
         "rs_gups": round(upd / t_rs / 1e9, 4),
         "siddon_1thread_gups": round(float(n) ** 3 / t_sid1 / 1e9, 4),
+        "siddon_compat_1thread": {"s": round(tc, 3), "gups": round(256.0 ** 3 * compat_views / tc / 1e9, 4),
+                                  "sample": f"reference C-path Siddon (compat form of the oracle), one thread, "
+                                            f"256^3 Shepp-Logan, views 0..{compat_views - 1} of 256"},
         "c0_256_full": {"siddon_s": round(t0_sid, 3), "vi_s": round(t0_vi, 3), "rs_s": round(t0_rs, 3),
                         "siddon_gups": round(upd0 / t0_sid / 1e9, 4), "vi_gups": round(upd0 / t0_vi / 1e9, 4),
                         "rs_gups": round(upd0 / t0_rs / 1e9, 4),
@@ -150,7 +186,7 @@ def cpu_baseline(vol_host, n, nphi, views, threads):
     }
 
 
-def serving_leg(ctx, n_req, rates, schedule, seed=42):
+def serving_leg(ctx, n_req, rates, schedules, seed=42, bf16_rate=None, direct_requests=64):
     """Mixed-size reconstruction stream: the plain sequential FP32 loop
     (tomo_gd_run, one request after another) against the serving loop at
     Poisson arrival rates.  Requests: edge n in {128,256,384,512} with
@@ -158,10 +194,18 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
     gradient iterations with the quadratic regulariser (lambda 0.01, every
     step optimal), pairs alternating RSVI / SIDDONVI, 16-ellipsoid phantom
     seeded by the request id.  Sinograms y = P(phantom) are made on the
-    device before the clock starts (excluded from latency).  The serving
-    loop runs with FP32 operator copies at every rate, and once more with
-    BF16 copies at the highest rate (FP32 state and sums; the error against
-    the sequential FP32 results is reported)."""
+    device before the clock starts (excluded from latency).
+
+    The sequential form's service time depends only on the request's
+    configuration (edge, views, pair): each of the stream's configurations is
+    run once (after a warm-up call) and its time stands for every request of
+    that configuration; the sequential throughput is n_req / sum of service
+    times and its latencies come from a FIFO queue at the same arrivals.  The
+    sequential results of those runs are the references the served results
+    of the same requests are compared with.  The serving loop runs FP32
+    operator copies with each schedule of `schedules` at every rate, and
+    BF16 copies at `bf16_rate`; the direct schedule (no recurrence) runs the
+    first `direct_requests` requests only."""
     import numpy as np
     import torch
     from tomogpi_amd import lib
@@ -179,39 +223,50 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
         y = torch.empty(G.sino_shape, dtype=torch.float32, device="cuda")
         G.project(truth, y, lib.PROJ_RS if pair == lib.PAIR_RSVI else lib.PROJ_SIDDON)
         del truth
-        reqs.append(dict(n=n, nphi=nphi, desc=desc, G=G, pair=pair, y=y))
+        reqs.append(dict(n=n, nphi=nphi, desc=desc, G=G, pair=pair, y=y, key=(n, nphi, pair)))
     torch.cuda.synchronize()
     gd = dict(iters=20, kopt=21, lam=0.01)
-    # Sequential plain form: service time of each request.
-    service = []
-    for r in reqs:
+    # Sequential plain form: one timed run per configuration.
+    service_of, refs = {}, {}
+    for i, r in enumerate(reqs):
+        if r["key"] in service_of:
+            continue
         vol = torch.zeros(r["G"].vol_shape, dtype=torch.float32, device="cuda")
+        r["G"].gd_run(r["y"], vol, pair=r["pair"], iters=1, kopt=2, lam=0.01)   # warm-up (allocations, code)
+        vol.zero_()
         torch.cuda.synchronize()
         t0 = time.perf_counter()
         r["G"].gd_run(r["y"], vol, pair=r["pair"], **gd)
         torch.cuda.synchronize()
-        service.append(time.perf_counter() - t0)
-        r["ref"] = vol
+        service_of[r["key"]] = time.perf_counter() - t0
+        refs[i] = vol
+    service = [service_of[r["key"]] for r in reqs]
     seq_total = sum(service)
     out = {"requests": n_req, "sizes": {str(k): int((edges == k).sum()) for k in (128, 256, 384, 512)},
-           "schedule": "recurrent" if schedule == lib.SCHEDULE_RECURRENT else "direct",
-           "sequential_rps": round(n_req / seq_total, 4), "sequential_busy_s": round(seq_total, 3), "rates": [],
-           "bf16_copies": []}
-    runs = [(rate, lib.F32) for rate in rates] + [(max(rates), lib.BF16)]
-    for rate, storage in runs:
-        arrivals = np.cumsum(np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, size=n_req))
-        # sequential latency at these arrivals (FIFO, one at a time)
+           "configurations": len(service_of),
+           "sequential_rps": round(n_req / seq_total, 4), "sequential_busy_s": round(seq_total, 3),
+           "sequential_note": "tomo_gd_run per configuration (edge, views, pair), each timed once after a warm-up "
+                              "call; the stream's sequential time is the sum over its requests",
+           "runs": []}
+    runs = [(rate, lib.F32, sch, n_req) for sch in schedules for rate in rates
+            if sch == lib.SCHEDULE_RECURRENT]
+    if lib.SCHEDULE_DIRECT in schedules:
+        runs.append((max(rates), lib.F32, lib.SCHEDULE_DIRECT, min(direct_requests, n_req)))
+    if bf16_rate:
+        runs.append((bf16_rate, lib.BF16, lib.SCHEDULE_RECURRENT, n_req))
+    for rate, storage, sch, count in runs:
+        arrivals = np.cumsum(np.random.default_rng(seed + int(rate)).exponential(1.0 / rate, size=count))
         t_free, lat_seq = 0.0, []
-        for a_t, s_t in zip(arrivals, service):
+        for a_t, s_t in zip(arrivals, service[:count]):
             start = max(a_t, t_free)
             t_free = start + s_t
             lat_seq.append(t_free - a_t)
-        srv = lib.Server(ctx, max_active=64, use_graph=True, schedule=schedule)
-        outs = [torch.zeros(r["G"].vol_shape, dtype=torch.float32, device="cuda") for r in reqs]
+        srv = lib.Server(ctx, max_active=64, use_graph=True, schedule=sch)
+        outs = [torch.zeros(r["G"].vol_shape, dtype=torch.float32, device="cuda") for r in reqs[:count]]
         torch.cuda.synchronize()
         ids = []
         t0 = time.perf_counter()
-        for i, r in enumerate(reqs):
+        for i, r in enumerate(reqs[:count]):
             dt = arrivals[i] - (time.perf_counter() - t0)
             if dt > 0:
                 time.sleep(dt)
@@ -221,32 +276,38 @@ def serving_leg(ctx, n_req, rates, schedule, seed=42):
         st = srv.stats()
         srv.close()
         torch.cuda.synchronize()
-        err = max(float(torch.linalg.vector_norm(o - r["ref"]) / torch.linalg.vector_norm(r["ref"]))
-                  for o, r in zip(outs, reqs))
+        errs = [float(torch.linalg.vector_norm(outs[i] - v) / torch.linalg.vector_norm(v))
+                for i, v in refs.items() if i < count]
         lat = np.array(lat_seq)
+        seq_s = sum(service[:count])
         rec = {
             "rate_rps": rate, "copies": "bf16" if storage == lib.BF16 else "f32",
+            "schedule": "recurrent" if sch == lib.SCHEDULE_RECURRENT else "direct", "requests": count,
             "serving_rps": round(st["throughput_rps"], 4),
-            "speedup_vs_sequential": round(st["throughput_rps"] * seq_total / n_req, 3),
+            "speedup_vs_sequential": round(st["throughput_rps"] * seq_s / count, 3),
             "p50_s": round(st["latency_p50_s"], 3), "p99_s": round(st["latency_p99_s"], 3),
             "sequential_p50_s": round(float(np.percentile(lat, 50, method="inverted_cdf")), 3),
             "sequential_p99_s": round(float(np.percentile(lat, 99, method="inverted_cdf")), 3),
             "peak_active": st["peak_active"], "peak_gib": round(st["peak_bytes"] / 2**30, 2),
-            "max_rel_l2_vs_sequential": float(f"{err:.3e}")}
-        out["rates" if storage == lib.F32 else "bf16_copies"].append(rec)
-        print(f"[serving] rate {rate} copies {rec['copies']}: {rec}", file=sys.stderr, flush=True)
+            "max_rel_l2_vs_sequential": float(f"{max(errs):.3e}") if errs else None,
+            "compared": len(errs)}
+        out["runs"].append(rec)
+        print(f"[serving] {rec}", file=sys.stderr, flush=True)
         del outs
     for r in reqs:
         r["G"].close()
     return out
 
 
-def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
+def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views, check_fp32=False):
     """Mixed-precision forms of one acquisition: volumes and sinograms stored
     in FP16 (scaled so max|x| = 2^14) or BF16, FP32 accumulation, FP64
     geometry.  Speed of each operator against its FP32 time, and the error of
-    each 16-bit result against the FP64 oracle on views 0..oracle_views-1 (a
-    CPU-bounded subset of the same acquisition)."""
+    each 16-bit result against the FP64 oracle and against the FP32 result on
+    the GPU, on views 0..oracle_views-1 (a CPU-bounded subset of the same
+    acquisition): relative L2 and relative Linf (max |a - b| / max |b|).
+    With check_fp32 the FP32 GPU results on the subset are also compared with
+    the FP32 oracle (the forms the 16-bit speed-ups are measured against)."""
     import numpy as np
     import torch
     from tomogpi_amd import lib
@@ -263,18 +324,42 @@ def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
     o = Oracle(gsub)
     vol_host = vol.cpu().numpy()
     t0 = time.perf_counter()
-    ref_sid = o.siddon(vol_host.astype(np.float64), f64=True)
+    ref = {"siddon": o.siddon(vol_host.astype(np.float64), f64=True),
+           "rs": o.rs(vol_host.astype(np.float64), f64=True)}
     sino_sub = o.siddon(vol_host)  # FP32 sinogram subset, the backprojection input
-    ref_vi = o.vi(sino_sub.astype(np.float64), f64=True)
+    ref["vi"] = o.vi(sino_sub.astype(np.float64), f64=True)
+    ref32 = {}
+    if check_fp32:
+        ref32 = {"siddon": sino_sub, "rs": o.rs(vol_host), "vi": o.vi(sino_sub)}
     del vol_host
-    print(f"[mixed] n={n}: FP64 oracle on {oracle_views} views: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
+    print(f"[mixed] n={n}: oracle on {oracle_views} views: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
     dsub = dataclasses.replace(desc, projection_nb=oracle_views, num_stop_angle=2.0 * oracle_views,
                                den_stop_angle=float(nphi))
     Gs = lib.Geometry(ctx, dsub)
 
-    def rel(a, b):
-        return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    def errs(a, b):
+        a = np.asarray(a, dtype=np.float64)
+        b = np.asarray(b, dtype=np.float64)
+        return {"rel_l2": float(f"{np.linalg.norm(a - b) / np.linalg.norm(b):.3e}"),
+                "rel_linf": float(f"{np.abs(a - b).max() / np.abs(b).max():.3e}")}
+
+    # FP32 results on the subset (the reference point of the 16-bit forms)
+    sin32 = torch.from_numpy(sino_sub).cuda()
+    got32 = {}
+    for op in ("siddon", "rs"):
+        t = torch.empty(Gs.sino_shape, dtype=torch.float32, device="cuda")
+        Gs.project(vol, t, lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS)
+        got32[op] = t.cpu().numpy()
+    t = torch.empty(Gs.vol_shape, dtype=torch.float32, device="cuda")
+    Gs.backproject(sin32, t)
+    got32["vi"] = t.cpu().numpy()
+    del t
+    if check_fp32:
+        out["fp32_vs_fp32_oracle"] = {op: {"bit_exact": bool(np.array_equal(got32[op], ref32[op])),
+                                           **errs(got32[op], ref32[op])} for op in ("siddon", "rs", "vi")}
+        out["fp32_vs_fp64_oracle"] = {op: errs(got32[op], ref[op]) for op in ("siddon", "rs", "vi")}
+        print(f"[mixed] n={n} FP32 vs oracle: {out['fp32_vs_fp32_oracle']}", file=sys.stderr, flush=True)
 
     for name, tdt, code in (("f16", torch.float16, lib.F16), ("bf16", torch.bfloat16, lib.BF16)):
         sv = 2.0 ** 14 / vmax if name == "f16" else 1.0
@@ -293,23 +378,29 @@ def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views):
         torch.cuda.synchronize()
         del b16, s16
         # errors on the view subset
-        e_s = torch.empty(Gs.sino_shape, dtype=tdt, device="cuda")
-        Gs.project(v16, e_s, lib.PROJ_SIDDON, vol_scale=sv, sino_scale=ss)
-        sin16 = (torch.from_numpy(sino_sub).cuda() * ss).to(tdt)
+        got = {}
+        for op in ("siddon", "rs"):
+            e_s = torch.empty(Gs.sino_shape, dtype=tdt, device="cuda")
+            Gs.project(v16, e_s, lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS, vol_scale=sv, sino_scale=ss)
+            got[op] = (e_s.float() / ss).cpu().numpy()
+        sin16 = (sin32 * ss).to(tdt)
         e_b = torch.empty(Gs.vol_shape, dtype=tdt, device="cuda")
         Gs.backproject(sin16, e_b, sino_scale=ss, vol_scale=sb)
         torch.cuda.synchronize()
-        err_s = rel((e_s.float() / ss).cpu().numpy().astype(np.float64), ref_sid)
-        err_b = rel((e_b.float() / sb).cpu().numpy().astype(np.float64), ref_vi)
-        out[name] = {
+        got["vi"] = (e_b.float() / sb).cpu().numpy()
+        rec = {
             "siddon_ms": round(ms["siddon"], 3), "vi_ms": round(ms["vi"], 3), "rs_ms": round(ms["rs"], 3),
             "siddon_speedup": round(fp32_ms["siddon"] / ms["siddon"], 3),
             "vi_speedup": round(fp32_ms["vi"] / ms["vi"], 3), "rs_speedup": round(fp32_ms["rs"] / ms["rs"], 3),
             "siddonvi_pair_speedup": round((fp32_ms["siddon"] + fp32_ms["vi"]) / (ms["siddon"] + ms["vi"]), 3),
             "rsvi_pair_speedup": round((fp32_ms["rs"] + fp32_ms["vi"]) / (ms["rs"] + ms["vi"]), 3),
-            "siddon_rel_l2_vs_fp64": float(f"{err_s:.3e}"), "vi_rel_l2_vs_fp64": float(f"{err_b:.3e}"),
-            "error_sample": f"views 0..{oracle_views - 1}, 16-bit inputs and outputs"}
-        print(f"[mixed] n={n} {name}: {out[name]}", file=sys.stderr, flush=True)
+            "vs_fp64_oracle": {op: errs(got[op], ref[op]) for op in ("siddon", "rs", "vi")},
+            "vs_fp32_gpu": {op: errs(got[op], got32[op]) for op in ("siddon", "rs", "vi")},
+            "error_sample": f"views 0..{oracle_views - 1} of {nphi}, 16-bit inputs and outputs"}
+        rec["siddon_rel_l2_vs_fp64"] = rec["vs_fp64_oracle"]["siddon"]["rel_l2"]
+        rec["vi_rel_l2_vs_fp64"] = rec["vs_fp64_oracle"]["vi"]["rel_l2"]
+        out[name] = rec
+        print(f"[mixed] n={n} {name}: {rec}", file=sys.stderr, flush=True)
         del v16, e_s, e_b, sin16
         torch.cuda.empty_cache()
     Gs.close()
@@ -343,7 +434,8 @@ def c2_leg(ctx, sizes, stream, oracle_views):
               flush=True)
         del sino, back
         torch.cuda.empty_cache()
-        res = mixed_leg(ctx, G, desc, vol, fp32, stream, oracle_views if n <= 512 else max(1, oracle_views // 4))
+        res = mixed_leg(ctx, G, desc, vol, fp32, stream, oracle_views if n <= 512 else max(1, oracle_views // 4),
+                        check_fp32=True)
         res["fp32"] = {k: {"ms": round(v, 3), "gups": round(upd / v / 1e6, 2)} for k, v in fp32.items()}
         out[str(n)] = res
         del vol
@@ -421,6 +513,52 @@ def batch_leg(ctx, n_prob, iters=20, seed=7):
                             "note": "BF16 z-pair copies rounded from the FP32 state; VI, sums and state FP32"}}
 
 
+def loop_leg(G, vol, sino, iters, upd, stream):
+    """The workload's reconstruction: gradient descent with the quadratic
+    regulariser (lambda 0.01) from zero on y = P(phantom), every step optimal
+    (kopt = iterations + 1), for the SIDDONVI and RSVI pairs in FP32, and
+    the SIDDONVI pair in its FP16 and BF16 mixed-precision forms (each
+    operator reads a 16-bit copy of its FP32 input, FP32 accumulation and
+    state) with the per-iteration residual against the FP32 run's."""
+    import torch
+    from tomogpi_amd import lib
+    out = {}
+    for name, pair, proj in (("siddonvi", lib.PAIR_SIDDONVI, lib.PROJ_SIDDON), ("rsvi", lib.PAIR_RSVI, lib.PROJ_RS)):
+        G.project(vol, sino, proj, stream=stream)
+        y = sino.clone()
+        runs = [("f32", lib.F32)] + ([("f16", lib.F16), ("bf16", lib.BF16)] if pair == lib.PAIR_SIDDONVI else [])
+        base = None
+        for dname, dt in runs:
+            x = torch.zeros_like(vol)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            hist = G.gd_run(y, x, pair=pair, iters=iters, kopt=iters + 1, lam=0.01, stream=stream, storage=dt)
+            torch.cuda.synchronize()
+            t_loop = time.perf_counter() - t0
+            rec = {"iterations": iters, "lambda": 0.01, "kopt": iters + 1, "s": round(t_loop, 3),
+                   "ms_per_iteration": round(t_loop * 1e3 / iters, 2),
+                   "gups": round(3.0 * upd * iters / t_loop / 1e9, 2),
+                   "residual_sq_first": hist[0]["residual_sq"], "residual_sq_last": hist[-1]["residual_sq"]}
+            if base is None:
+                base = (hist, x, t_loop)
+                rec["note"] = ("per iteration two projections (P x, P dJ) and one VI backprojection (3 N^3 Nphi "
+                               "updates), plus the Laplacian terms, norms and step rule")
+            else:
+                r32 = [h["residual_sq"] for h in base[0]]
+                r16 = [h["residual_sq"] for h in hist]
+                rel = [abs(u - v) / v for u, v in zip(r16, r32)]
+                rec["residual_rel_diff_vs_f32"] = [float(f"{v:.3e}") for v in rel]
+                rec["max_residual_rel_diff_vs_f32"] = float(f"{max(rel):.3e}")
+                rec["volume_rel_l2_vs_f32"] = float(
+                    f"{float(torch.linalg.vector_norm(x - base[1]) / torch.linalg.vector_norm(base[1])):.3e}")
+                rec["speedup_vs_f32"] = round(base[2] / t_loop, 3)
+            out[name if dname == "f32" else f"{name}_{dname}"] = rec
+            print(f"[loop] {name} {dname}: {rec}", file=sys.stderr, flush=True)
+            del x
+        del y
+    return out
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -482,30 +620,47 @@ def main():
     b_vi = upd * 4 * 4 + float(n) ** 3 * 4
     b_sid = n_seg * 4 + nsino * 4
     b_rs = n_samp * 8 * 4 + nsino * 4
-    ops = {
-        "siddon": {"ms": round(sid_ms, 3), "gups": round(upd / sid_ms / 1e6, 2), "alg_bytes": b_sid,
-                   "hbm_frac": round(b_sid / (sid_ms * 1e-3) / HBM_PEAK, 4), "n_seg": n_seg},
-        "vi": {"ms": round(vi_ms, 3), "gups": round(upd / vi_ms / 1e6, 2), "alg_bytes": b_vi,
-               "hbm_frac": round(b_vi / (vi_ms * 1e-3) / HBM_PEAK, 4)},
-        "rs": {"ms": round(rs_ms, 3), "gups": round(upd / rs_ms / 1e6, 2), "alg_bytes": b_rs,
-               "hbm_frac": round(b_rs / (rs_ms * 1e-3) / HBM_PEAK, 4), "n_samp": n_samp},
-    }
+    # Three fractions per operator:
+    #   effective_gather_frac  algorithmic bytes (every chord, tap or sample a DRAM read) / t / 8 TB/s: the rate
+    #                          the gathers are served at, most of them from L2 / the Infinity Cache; above 1
+    #                          when the caches serve more than HBM could, so it is not an HBM occupancy
+    #   physical_hbm_frac      FETCH_SIZE + WRITE_SIZE bytes per launch (calibrated) / t / 8 TB/s
+    #   valu_issue_frac        VALU wave-instructions per launch x cycles per instruction / (4 SIMDs x CUs x
+    #                          kernel cycles), from the counters and the issue-rate probe (the kernels' bound)
+    measured = load_traffic(n, nphi)
+    valu = load_valu(n, nphi)
+    ops = {}
+    for name, ms, alg, key in (("siddon", sid_ms, b_sid, "siddon_kernel"), ("vi", vi_ms, b_vi, "vi_staged_kernel"),
+                               ("rs", rs_ms, b_rs, "rs_kernel")):
+        rec = {"ms": round(ms, 3), "gups": round(upd / ms / 1e6, 2), "alg_bytes": alg,
+               "effective_gather_frac": round(alg / (ms * 1e-3) / HBM_PEAK, 4)}
+        if key in measured:
+            rec["hbm_traffic_bytes"] = measured[key]
+            rec["physical_hbm_frac"] = round(measured[key] / (ms * 1e-3) / HBM_PEAK, 4)
+        if key in valu:
+            rec["valu_issue_frac"] = valu[key]["issue_frac"]
+            rec["valu_busy"] = valu[key].get("valu_busy")
+        ops[name] = rec
+    ops["siddon"]["n_seg"] = n_seg
+    ops["rs"]["n_samp"] = n_samp
     # compulsory traffic: the volume and the sinogram once each
     comp = float(n) ** 3 * 4 + nsino * 4
     for name, ms in (("siddon", sid_ms), ("vi", vi_ms), ("rs", rs_ms)):
         ops[name]["compulsory_frac"] = round(comp / (ms * 1e-3) / HBM_PEAK, 4)
+    # The roofline line is the step's dominant kernel, the Siddon projector: its algorithmic bytes (every positive
+    # chord counted as one 4-byte read, plus the sinogram write) over its HIP-event time.
     dom = "vi" if vi_ms >= sid_ms else "siddon"
     dom_bytes = b_vi if dom == "vi" else b_sid
     dom_ms = vi_ms if dom == "vi" else sid_ms
     dom_kernel = "vi_staged_kernel" if dom == "vi" else "siddon_kernel"
     achieved = dom_bytes / (dom_ms * 1e-3) / 1e9
-    measured = load_traffic(n, nphi)
-    for name, key in (("siddon", "siddon_kernel"), ("vi", "vi_staged_kernel"), ("rs", "rs_kernel")):
-        if key in measured:
-            ops[name]["hbm_traffic_bytes"] = measured[key]
     roofline = {"bound": "hbm", "kernel": dom_kernel,
                 "achieved": round(achieved, 1), "peak": HBM_PEAK / 1e9, "unit": "GB/s",
-                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": measured.get(dom_kernel)}
+                "frac": round(achieved * 1e9 / HBM_PEAK, 4), "traffic": measured.get(dom_kernel),
+                "basis": "algorithmic bytes per launch (every positive chord one 4-byte read, plus the sinogram) / "
+                         "launch time: an effective gather rate, mostly served by L2 and the Infinity Cache",
+                "physical_frac": ops[dom].get("physical_hbm_frac"),
+                "valu_issue_frac": ops[dom].get("valu_issue_frac")}
 
     mixed = None
     if a.mixed_oracle_views > 0:
@@ -514,36 +669,19 @@ def main():
 
     loop = None
     if a.loop_iters > 0:
-        # The workload's reconstruction: SIDDONVI gradient descent with the
-        # quadratic regulariser (lambda 0.01) from zero on y = P(phantom),
-        # every step optimal (kopt = iterations + 1).
-        siddon()
-        y = sino.clone()
-        x = torch.zeros_like(vol)
-        torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        hist = G.gd_run(y, x, pair=lib.PAIR_SIDDONVI, iters=a.loop_iters, kopt=a.loop_iters + 1, lam=0.01,
-                        stream=stream)
-        torch.cuda.synchronize()
-        t_loop = time.perf_counter() - t0
-        loop = {"pair": "SIDDONVI", "iterations": a.loop_iters, "lambda": 0.01, "kopt": a.loop_iters + 1,
-                "s": round(t_loop, 3), "ms_per_iteration": round(t_loop * 1e3 / a.loop_iters, 2),
-                "gups": round(3.0 * upd * a.loop_iters / t_loop / 1e9, 2),
-                "residual_sq_first": hist[0]["residual_sq"], "residual_sq_last": hist[-1]["residual_sq"],
-                "note": "per iteration two Siddon projections (P x, P dJ) and one VI backprojection (3 N^3 Nphi "
-                        "updates), plus the Laplacian terms, norms and step rule"}
-        print(f"[loop] {loop}", file=sys.stderr, flush=True)
-        del x, y
+        loop = loop_leg(G, vol, sino, a.loop_iters, upd, stream)
 
     serving = None
     del vol, sino, back
     torch.cuda.empty_cache()
     mixed_c2 = None
     if a.c2_sizes.strip() and a.mixed_oracle_views > 0:
-        mixed_c2 = c2_leg(ctx, [int(x) for x in a.c2_sizes.split(",")], stream, a.mixed_oracle_views)
+        mixed_c2 = c2_leg(ctx, [int(x) for x in a.c2_sizes.split(",")], stream, a.c2_oracle_views)
     if a.serving_requests > 0:
-        sched = lib.SCHEDULE_RECURRENT if a.serving_schedule == "recurrent" else lib.SCHEDULE_DIRECT
-        serving = serving_leg(ctx, a.serving_requests, [float(x) for x in a.serving_rates.split(",")], sched)
+        names = {"recurrent": lib.SCHEDULE_RECURRENT, "direct": lib.SCHEDULE_DIRECT}
+        scheds = [names[x] for x in a.serving_schedules.split(",") if x]
+        serving = serving_leg(ctx, a.serving_requests, [float(x) for x in a.serving_rates.split(",")], scheds,
+                              bf16_rate=a.serving_bf16_rate or None, direct_requests=a.serving_direct_requests)
 
     batched = None
     if a.batch_problems > 0:
@@ -580,6 +718,10 @@ def main():
         "serving": serving,
         "batched": batched,
     }
+    # release the device before printing the line: geometry, context, queued work
+    G.close()
+    torch.cuda.synchronize()
+    ctx.close()
     print(json.dumps(out), flush=True)
 
 

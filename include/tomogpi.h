@@ -227,6 +227,20 @@ typedef struct tomo_gd_params {
     double  lambda;                  /* regularizationParameter (GG, beta=2) */
     int32_t positivity;              /* positivity                           */
     double  alpha_min_init;          /* -0.0347711 in the reference          */
+    int32_t storage;                 /* element type the operators read
+                                        (the reference's --precision half,
+                                        C/Iter3D_half.cu:545-700): TOMO_F32
+                                        (0), or TOMO_F16 / TOMO_BF16: every
+                                        projection and backprojection reads a
+                                        16-bit copy of its FP32 input (FP16
+                                        scaled by the power of two that puts
+                                        max|x| in [2^13, 2^14), BF16
+                                        unscaled) and accumulates in FP32;
+                                        the iterate, gradient, residual and
+                                        norms stay FP32 / FP64.  Served
+                                        requests take tomo_serve_request.
+                                        storage instead (this field must be
+                                        0 there).                            */
 } tomo_gd_params;
 
 typedef struct tomo_gd_iter_stat {

@@ -217,3 +217,42 @@ def test_gd_two_threads_two_streams_one_context(small, ctx):
         for i in range(len(jobs)):
             assert torch.equal(outs[i], alone[i][0]), (rep, i)
             assert res[i] == alone[i][1], (rep, i)
+
+
+# Mixed-precision loop (tomo_gd_params.storage): every operator reads a 16-bit
+# copy of its FP32 input, FP32 accumulation, FP32 state.  Its per-iteration
+# residual sum (y - P x)^2 stays within these relative bounds of the FP32
+# loop's, and so does the final volume (rel-L2).  A CPU emulation with the
+# oracle's operators on rounded inputs measured 1.5e-5..4.2e-5 (FP16) and
+# 2.1e-4..4.8e-4 (BF16) for the residuals at 40^3 and 64^3, 10 iterations.
+MIXED_LOOP_TOL = {"f16": 1e-3, "bf16": 4e-3}
+
+
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+@pytest.mark.parametrize("pair", [lib.PAIR_RSVI, lib.PAIR_SIDDONVI])
+@pytest.mark.parametrize("conjugate", [False, True])
+def test_mixed_loop_residual_tracks_fp32(ctx, dt, pair, conjugate):
+    g = scaled_geometry(64, 48)
+    print(f"[case] mixed loop {dt} pair={pair} cg={conjugate}", flush=True)
+    o = Oracle(g)
+    G = lib.Geometry(ctx, to_desc(g))
+    try:
+        truth = ellipsoids(64, count=8, seed=4, amin=2.0, amax=19.0)
+        y = torch.from_numpy(o.rs(truth) if pair == lib.PAIR_RSVI else o.siddon(truth)).cuda()
+        kw = dict(pair=pair, iters=10, kopt=11, lam=0.01, conjugate=conjugate)
+        x32 = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+        s32 = G.gd_run(y, x32, **kw)
+        x16 = torch.zeros(g.vol_shape, dtype=torch.float32, device="cuda")
+        s16 = G.gd_run(y, x16, storage=lib.F16 if dt == "f16" else lib.BF16, **kw)
+        torch.cuda.synchronize()
+        r32 = np.array([s["residual_sq"] for s in s32])
+        r16 = np.array([s["residual_sq"] for s in s16])
+        rel = np.abs(r16 - r32) / r32
+        vol = rel_l2(x16.cpu().numpy(), x32.cpu().numpy())
+        print(f"[mixed loop] {dt} pair={pair} cg={conjugate}: residual rel diff per iteration "
+              f"{' '.join(f'{v:.1e}' for v in rel)}; final volume rel-L2 {vol:.2e}", flush=True)
+        assert rel.max() <= MIXED_LOOP_TOL[dt], rel
+        assert vol <= MIXED_LOOP_TOL[dt]
+        assert r16[-1] < 0.05 * r16[0]
+    finally:
+        G.close()

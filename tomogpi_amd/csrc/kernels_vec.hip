@@ -97,6 +97,34 @@ __global__ __launch_bounds__(kBlock) void dot_partial_kernel(const float* a, con
     if (threadIdx.x == 0) partial[blockIdx.x] = r;
 }
 
+__device__ __forceinline__ float block_max(float v) {
+    __shared__ float shm[kBlock / 64];
+    for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_down(v, off, 64));
+    const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+    if (lane == 0) shm[wid] = v;
+    __syncthreads();
+    float r = 0.0f;
+    if (threadIdx.x == 0)
+        for (int w = 0; w < kBlock / 64; ++w) r = fmaxf(r, shm[w]);
+    return r;
+}
+
+// max |x| (NaN is ignored by fmaxf), partials then one final block.
+__global__ __launch_bounds__(kBlock) void max_abs_partial_kernel(const float* x, size_t n, float* partial) {
+    float m = 0.0f;
+    for (size_t i = blockIdx.x * (size_t)kBlock + threadIdx.x; i < n; i += (size_t)gridDim.x * kBlock)
+        m = fmaxf(m, fabsf(x[i]));
+    const float r = block_max(m);
+    if (threadIdx.x == 0) partial[blockIdx.x] = r;
+}
+
+__global__ __launch_bounds__(kBlock) void final_max_kernel(const float* partial, int count, float* out) {
+    float m = 0.0f;
+    for (int i = threadIdx.x; i < count; i += kBlock) m = fmaxf(m, partial[i]);
+    const float r = block_max(m);
+    if (threadIdx.x == 0) *out = r;
+}
+
 __global__ __launch_bounds__(kBlock) void final_sum_kernel(const double* partial, int count, double* out) {
     double s = 0.0;
     for (int i = threadIdx.x; i < count; i += kBlock) s += partial[i];
@@ -614,6 +642,24 @@ int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* 
     if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
     hipLaunchKernelGGL(dot_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, a, b, n, part);
     return finish_reduce(part, (int)blocks, out, s);
+}
+
+int reduce_max_abs(const float* x, size_t n, float* out, hipStream_t s) {
+    float* part = nullptr;
+    TOMO_HIP(hipMallocAsync((void**)&part, sizeof(float) * (kReduceBlocks + 1), s));
+    unsigned blocks = grid_for(n);
+    if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
+    hipLaunchKernelGGL(max_abs_partial_kernel, dim3(blocks), dim3(kBlock), 0, s, x, n, part);
+    hipLaunchKernelGGL(final_max_kernel, dim3(1), dim3(kBlock), 0, s, part, (int)blocks, part + kReduceBlocks);
+    hipError_t e = hipGetLastError();
+    float h = 0.0f;
+    if (e == hipSuccess) e = hipMemcpyAsync(&h, part + kReduceBlocks, sizeof(float), hipMemcpyDeviceToHost, s);
+    const hipError_t f = hipFreeAsync(part, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
+    TOMO_HIP(e);
+    TOMO_HIP(f);
+    *out = h;
+    return TOMO_OK;
 }
 
 int launch_ellipsoids(const tomo_geom* g, int mode, const double* d_table, int count, float* vol, hipStream_t s) {

@@ -10,8 +10,11 @@
 //                 alpha = -normdJ / normdJProj;  alpha_min = max(alpha, alpha_min)
 //   else          alpha = alpha_min
 //   x += (float)alpha dJ, clamped at 0 if positivity
+// With a 16-bit storage type every operator reads a 16-bit copy of its FP32
+// input (the mixed-precision form); the rest of the loop is unchanged.
 // Volumes and sinograms stay on the device; only the FP64 norms return to the
 // host for the step rule.  The SIRT loop (tomo_sirt_run) follows.
+#include <algorithm>
 #include <cmath>
 
 #include "tomo_internal.hpp"
@@ -33,6 +36,57 @@ int project(const tomo_geom* g, const tomo_gd_params& pr, const float* vol, floa
     return launch_rs(g, (pr.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, in, out, s);
 }
 
+// Operator inputs of the mixed-precision loops (tomo_gd_params.storage
+// TOMO_F16 / TOMO_BF16): each operator reads a 16-bit copy of its FP32
+// input, made in `tmp` (room for the larger of a volume and a sinogram);
+// FP16 copies are scaled by the power of two that puts max|x| in
+// [2^13, 2^14), BF16 copies are unscaled.  Outputs stay FP32.  The copy is
+// reused by the next operator of the same stream, after this one read it.
+struct Mixed {
+    int dtype = TOMO_F32;
+    DevBuf tmp;
+    int init(int storage, size_t n) {
+        dtype = storage;
+        if (dtype != TOMO_F32) TOMO_HIP(hipMalloc(&tmp.p, n * 2));
+        return TOMO_OK;
+    }
+    int stage(const float* x, size_t n, Buf* in, hipStream_t s) {
+        if (dtype == TOMO_F32) {
+            *in = Buf{x, TOMO_F32, 1.0f};
+            return TOMO_OK;
+        }
+        float scale = 1.0f;
+        if (dtype == TOMO_F16) {
+            float m = 0.0f;
+            const int rc = reduce_max_abs(x, n, &m, s);
+            if (rc) return rc;
+            if (m > 0.0f && std::isfinite(m)) {
+                int e = 0;
+                (void)std::frexp(m, &e);  // m = f 2^e, f in [0.5, 1)
+                scale = std::ldexp(1.0f, std::min(100, std::max(-100, 14 - e)));
+            }
+        }
+        *in = Buf{tmp.p, dtype, scale};
+        return launch_convert(Buf{x, TOMO_F32, 1.0f}, MutBuf{tmp.p, dtype, scale}, n, s);
+    }
+};
+
+int project(const tomo_geom* g, const tomo_gd_params& pr, Mixed& mx, const float* vol, float* sino, hipStream_t s) {
+    Buf in{};
+    int rc = mx.stage(vol, g->nvox(), &in, s);
+    if (rc) return rc;
+    const MutBuf out{sino, TOMO_F32, 1.0f};
+    if (pr.pair == TOMO_PAIR_SIDDONVI) return launch_siddon(g, (pr.flags & TOMO_FLAG_SIDDON_COMPAT) ? 1 : 0, in, out, s);
+    return launch_rs(g, (pr.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, in, out, s);
+}
+
+int backproject(const tomo_geom* g, Mixed& mx, const float* sino, float* vol, hipStream_t s) {
+    Buf in{};
+    int rc = mx.stage(sino, g->nsino(), &in, s);
+    if (rc) return rc;
+    return launch_vi(g, in, MutBuf{vol, TOMO_F32, 1.0f}, s);
+}
+
 }  // namespace
 
 int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const float* sino, const float* weight,
@@ -45,17 +99,19 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         TOMO_HIP(hipMalloc(&t1.p, nv * sizeof(float)));
         TOMO_HIP(hipMalloc(&t2.p, nv * sizeof(float)));
     }
+    Mixed mx;
+    int rc = mx.init(pr.storage, std::max(nv, ns));
+    if (rc) return rc;
     float* e = (float*)est.p;
     float* d = (float*)dj.p;
     double alpha_min = pr.alpha_min_init;
-    int rc;
     for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
         tomo_gd_iter_stat st{0, 0, 0, 0, 0, 0};
-        if ((rc = project(g, pr, vol, e, s))) return rc;
+        if ((rc = project(g, pr, mx, vol, e, s))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
         if (weight && (rc = launch_mul(e, weight, ns, s))) return rc;  // weighted form: r = W (y - P x)
-        if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{d, TOMO_F32, 1.0f}, s))) return rc;
+        if ((rc = backproject(g, mx, e, d, s))) return rc;
         if ((rc = launch_scale(d, -2.0f, nv, s))) return rc;
         double norm_reg = 0.0;
         if (pr.lambda != 0.0) {
@@ -75,7 +131,7 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         }
         double alpha;
         if (k < pr.optimal_step_iteration_nb) {
-            if ((rc = project(g, pr, d, e, s))) return rc;
+            if ((rc = project(g, pr, mx, d, e, s))) return rc;
             if ((rc = reduce_sq_norm(ctx, Buf{d, TOMO_F32, 1.0f}, nv, &st.norm_dj, s))) return rc;
             double pe = 0.0;
             if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &pe, s))) return rc;
@@ -115,14 +171,16 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
     float* a = (float*)t1.p;
     float* b = (float*)t2.p;
     const int nx = g->info.nx, ny = g->info.ny, nz = g->info.nz;
+    Mixed mx;
+    int rc = mx.init(pr.storage, std::max(nv, ns));
+    if (rc) return rc;
     double gg_prev = 0.0;
-    int rc;
     for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
         tomo_gd_iter_stat st{0, 0, 0, 0, 0, 0};
-        if ((rc = project(g, pr, vol, e, s))) return rc;
+        if ((rc = project(g, pr, mx, vol, e, s))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
-        if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{gr, TOMO_F32, 1.0f}, s))) return rc;
+        if ((rc = backproject(g, mx, e, gr, s))) return rc;
         if ((rc = launch_scale(gr, -2.0f, nv, s))) return rc;
         if (pr.lambda != 0.0) {
             if ((rc = launch_laplacian(vol, a, nx, ny, nz, s))) return rc;
@@ -142,7 +200,7 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         gg_prev = gg;
         double gd = 0.0;
         if ((rc = reduce_dot(ctx, gr, d, nv, &gd, s))) return rc;
-        if ((rc = project(g, pr, d, e, s))) return rc;
+        if ((rc = project(g, pr, mx, d, e, s))) return rc;
         double pe = 0.0, reg = 0.0;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &pe, s))) return rc;
         if (pr.lambda != 0.0) {
@@ -237,6 +295,8 @@ int run_loop(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params* params, co
     if (params->pair != TOMO_PAIR_RSVI && params->pair != TOMO_PAIR_SIDDONVI)
         return fail(TOMO_ERR_INVALID_ARG, "unknown pair");
     if (params->gradient_iteration_nb < 0) return fail(TOMO_ERR_INVALID_ARG, "negative iteration count");
+    if (params->storage != TOMO_F32 && params->storage != TOMO_F16 && params->storage != TOMO_BF16)
+        return fail(TOMO_ERR_INVALID_ARG, "storage must be TOMO_F32, TOMO_F16 or TOMO_BF16");
     hipStream_t s = reinterpret_cast<hipStream_t>(stream);
     if (!(params->flags & TOMO_FLAG_HOST_PTR)) return run(ctx, g, *params, sinogram, volume, stats, s);
     const size_t nv = g->nvox(), ns = g->nsino();

@@ -790,6 +790,8 @@ extern "C" int tomo_serve_submit(tomo_server* srv, const tomo_serve_request* req
         return fail(TOMO_ERR_INVALID_ARG, "SIRT relaxation must be positive");
     if (req->storage != TOMO_F32 && req->storage != TOMO_BF16)
         return fail(TOMO_ERR_UNSUPPORTED, "operator copies are FP32 or BF16");
+    if (gd.storage != TOMO_F32)
+        return fail(TOMO_ERR_INVALID_ARG, "served requests take tomo_serve_request.storage; gd.storage must be 0");
     auto q = std::make_shared<Request>();
     q->req = *req;
     q->host = (gd.flags & TOMO_FLAG_HOST_PTR) != 0;

@@ -122,6 +122,8 @@ int launch_mul(float* x, const float* w, size_t n, hipStream_t s);
 int launch_waxpy(float* x, const float* c, const float* b, int positive, size_t n, hipStream_t s);
 int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s);
 int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s);
+// max |x| over n FP32 elements.  Synchronous.
+int reduce_max_abs(const float* x, size_t n, float* out, hipStream_t s);
 int launch_convert(Buf in, MutBuf out, size_t n, hipStream_t s);
 
 // Ellipsoid phantoms.  mode 0: Shepp-Logan form (normalised coordinates,

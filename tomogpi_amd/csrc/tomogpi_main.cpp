@@ -13,7 +13,8 @@
 //   mem_CPU, SiddonVI, CPU/C   reference C-path Siddon (compat form, 256^3) + VI
 //   mem_CPU, SiddonVI, other   clean Siddon + VI
 // --precision=half stores volumes and sinograms in FP16 (scaled per array)
-// with FP32 accumulation; its files stay FP32, as the reference's half
+// with FP32 accumulation (the loops: every operator reads an FP16 copy of
+// its FP32 input, tomo_gd_params.storage); its files stay FP32, as the reference's half
 // containers convert on load and save (C/Image3D_CPU.cu:659-735,
 // C/Sinogram3D_CPU.cu:556-606).  --precision=double runs the FP32 operators
 // and reads and writes FP64 files, the layout of the reference's double
@@ -298,6 +299,7 @@ int main(int argc, char** argv) {
         gp.lambda = lambda;
         gp.positivity = pos;
         gp.alpha_min_init = -0.0347711;
+        gp.storage = half ? TOMO_F16 : TOMO_F32;  // the half loop (C/Iter3D_half.cu:545-700): FP16 operator inputs
         std::vector<float> weight;
         if (grad == "Weighted") {
             // doWeightedGradient: weights from <workdir>/weightedSinogram.s,

@@ -73,7 +73,7 @@ class GdParams(C.Structure):
     _fields_ = [
         ("pair", C.c_int32), ("flags", C.c_int32), ("gradient_iteration_nb", C.c_int32),
         ("optimal_step_iteration_nb", C.c_int32), ("lam", C.c_double), ("positivity", C.c_int32),
-        ("alpha_min_init", C.c_double),
+        ("alpha_min_init", C.c_double), ("storage", C.c_int32),
     ]
 
 
@@ -365,11 +365,12 @@ class Geometry:
         return vol
 
     def gd_run(self, sino, vol, pair=PAIR_RSVI, flags=0, iters=10, kopt=10, lam=0.0, positivity=0,
-               alpha_min_init=-0.0347711, stream=None, conjugate=False, weight=None):
+               alpha_min_init=-0.0347711, stream=None, conjugate=False, weight=None, storage=F32):
         """Simple-gradient loop (tomo_gd_run), conjugate gradient (tomo_cg_run),
-        or with a weight sinogram the weighted gradient (tomo_wgd_run)."""
+        or with a weight sinogram the weighted gradient (tomo_wgd_run);
+        storage F16 / BF16 gives the mixed-precision form."""
         p = GdParams(pair=pair, flags=flags, gradient_iteration_nb=iters, optimal_step_iteration_nb=kopt, lam=lam,
-                     positivity=positivity, alpha_min_init=alpha_min_init)
+                     positivity=positivity, alpha_min_init=alpha_min_init, storage=storage)
         stats = (GdIterStat * max(iters, 1))()
         ps, hs = _ptr(sino)
         pv, hv = _ptr(vol)
