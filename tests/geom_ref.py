@@ -85,6 +85,48 @@ def cylinder_slab_chord(S, D, R, zh):
     return max(0.0, l1 - l0)
 
 
+def siddon_merge_segments(g, S, D, raw=False):
+    """The segments of siddon_merge: (length mm, voxel index (i, j, k) or None
+    outside the grid) for every pair of consecutive distinct crossings.  With
+    raw=True also the sorted crossing list before duplicates are merged."""
+    dd = derived(g)
+    dl = dd["delta"]
+    L = np.linalg.norm(D - S)
+    d = (D - S) / L
+    R = 0.5 * g.nx * dl
+    zh = 0.5 * g.nz * dl
+    a = d[0] ** 2 + d[1] ** 2
+    b = S[0] * d[0] + S[1] * d[1]
+    c = S[0] ** 2 + S[1] ** 2 - R * R
+    disc = b * b - a * c
+    if a == 0 or disc <= 0:
+        return ([], np.zeros(0)) if raw else []
+    l0, l1 = (-b - np.sqrt(disc)) / a, (-b + np.sqrt(disc)) / a
+    if d[2] != 0:
+        ta, tb = (-zh - S[2]) / d[2], (zh - S[2]) / d[2]
+        l0, l1 = max(l0, min(ta, tb)), min(l1, max(ta, tb))
+    if l1 <= l0:
+        return ([], np.zeros(0)) if raw else []
+    cen = np.array([dd["cx"], dd["cy"], dd["cz"]])
+    nn = [g.nx, g.ny, g.nz]
+    cuts = [l0, l1]
+    for ax in range(3):
+        if d[ax] == 0:
+            continue
+        planes = (np.arange(nn[ax] + 1) - 0.5 - cen[ax]) * dl
+        lam = (planes - S[ax]) / d[ax]
+        cuts.extend(lam[(lam > l0) & (lam < l1)].tolist())
+    all_cuts = np.sort(np.array(cuts))
+    cuts = np.unique(all_cuts)
+    out = []
+    for lo, hi in zip(cuts[:-1], cuts[1:]):
+        mid = S + 0.5 * (lo + hi) * d
+        idx = np.floor(mid / dl + cen + 0.5).astype(int)
+        inside = all(0 <= idx[a_] < nn[a_] for a_ in range(3))
+        out.append((hi - lo, tuple(int(v) for v in idx) if inside else None))
+    return (out, all_cuts) if raw else out
+
+
 def siddon_merge(g, vol, S, D):
     """Siddon by merging every plane crossing (Jacobs et al. 1998 form):
     sort all crossings of x, y, z voxel planes inside the FOV, give each

@@ -272,6 +272,37 @@ def test_siddon_matches_merge_all_crossings():
                 assert out[p, n, m] == pytest.approx(ref, rel=1e-10, abs=1e-10)
 
 
+@pytest.mark.parametrize("n,nphi,nu,nv,start", [(10, 5, 13, 11, (1.0, 7.0)), (13, 4, 16, 15, (2.0, 9.0)),
+                                                 (16, 3, 19, 17, (1.0, 11.0))])
+def test_siddon_segment_count_from_merged_crossings(n, nphi, nu, nv, start):
+    """N_seg (og_siddon_count, the numerator of the Siddon roofline: chords
+    of positive FP32 length) against an independent merge of every plane
+    crossing (geom_ref.siddon_merge_segments), ray by ray.  Segments longer
+    than tau = 1e-4 voxel must each be one chord; crossings of different
+    planes that lie within tau of each other (corner passages, where FP32
+    rounding may or may not leave a sliver) may add at most one chord each.
+    So N_robust <= N_seg <= N_robust + N_ambiguous, with N_ambiguous a small
+    fraction: a dropped last chord, a counted zero chord or a missed axis
+    would move N_seg by about one per ray."""
+    g = small_geom(n, nphi, nu=nu, nv=nv, num_start=start[0], den_start=start[1])
+    o = Oracle(g)
+    dl = gr.derived(g)["delta"]
+    tau = 1e-4 * dl
+    robust = ambiguous = rays = 0
+    for p in range(g.nphi):
+        for v in range(g.nv):
+            for m in range(g.nu):
+                S, D = gr.source_detector(g, p, m, v)
+                segs, cuts = gr.siddon_merge_segments(g, S, D, raw=True)
+                robust += sum(1 for length, idx in segs if idx is not None and length > tau)
+                ambiguous += int((np.diff(cuts) <= tau).sum())
+                rays += bool(segs)
+    n_seg = o.siddon_count()
+    print(f"N_seg {n_seg}, robust {robust}, ambiguous {ambiguous}, rays {rays}")
+    assert rays > 100 and ambiguous < rays / 4
+    assert robust <= n_seg <= robust + ambiguous
+
+
 def test_siddon_compat_mirror_identity():
     """Reference C Siddon at (phi, s, t) == clean Siddon at (phi+pi, -s, t) on a
     ball that vanishes near the FOV boundary, where the compat quirks act."""
