@@ -590,6 +590,12 @@ def main():
     def rs():
         G.project(vol, sino, lib.PROJ_RS, stream=stream)
 
+    def rs_staged():
+        G.project(vol, sino, lib.PROJ_RS, lib.FLAG_STAGED, stream=stream)
+
+    def siddon_staged():
+        G.project(vol, sino, lib.PROJ_SIDDON, lib.FLAG_STAGED, stream=stream)
+
     if a.calibration_copy:
         copy = vol + 0.0  # an element-wise kernel: reads and writes the volume once
         torch.cuda.synchronize()
@@ -613,7 +619,13 @@ def main():
     sid_ms = sum(x.elapsed_time(y) for x, y in sid_ev) / a.steps
     vi_ms = sum(x.elapsed_time(y) for x, y in vi_ev) / a.steps
     rs_ms, _ = timed(rs, max(2, min(a.steps, 3)), stream)
-    print(f"[bench] siddon {sid_ms:.2f} ms, vi {vi_ms:.2f} ms, rs {rs_ms:.2f} ms", file=sys.stderr, flush=True)
+    rs_staged()
+    rs_staged_ms, _ = timed(rs_staged, max(2, min(a.steps, 3)), stream)
+    siddon_staged()
+    sid_staged_ms, _ = timed(siddon_staged, max(2, min(a.steps, 3)), stream)
+    siddon()  # the timed form's sinogram again (VI input of later legs)
+    print(f"[bench] siddon {sid_ms:.2f} ms, vi {vi_ms:.2f} ms, rs {rs_ms:.2f} ms (LDS-staged: siddon "
+          f"{sid_staged_ms:.2f} ms, rs {rs_staged_ms:.2f} ms)", file=sys.stderr, flush=True)
 
     upd = float(n) ** 3 * nphi
     nsino = float(G.info["nu"]) * G.info["nv"] * nphi
@@ -630,8 +642,11 @@ def main():
     measured = load_traffic(n, nphi)
     valu = load_valu(n, nphi)
     ops = {}
-    for name, ms, alg, key in (("siddon", sid_ms, b_sid, "siddon_kernel"), ("vi", vi_ms, b_vi, "vi_staged_kernel"),
-                               ("rs", rs_ms, b_rs, "rs_kernel")):
+    for name, ms, alg, key in (("siddon", sid_ms, b_sid, "siddon_kernel"),
+                               ("vi", vi_ms, b_vi, "vi_staged_kernel"),
+                               ("rs", rs_ms, b_rs, "rs_kernel"),
+                               ("siddon_staged", sid_staged_ms, b_sid, "siddon_staged_kernel"),
+                               ("rs_staged", rs_staged_ms, b_rs, "rs_staged_kernel")):
         rec = {"ms": round(ms, 3), "gups": round(upd / ms / 1e6, 2), "alg_bytes": alg,
                "effective_gather_frac": round(alg / (ms * 1e-3) / HBM_PEAK, 4)}
         if key in measured:
@@ -643,9 +658,17 @@ def main():
         ops[name] = rec
     ops["siddon"]["n_seg"] = n_seg
     ops["rs"]["n_samp"] = n_samp
+    ops["rs"]["form"] = "per-lane gathers from the z-pair copies (rs_kernel)"
+    ops["rs_staged"]["form"] = "taps staged in LDS per window of up to 4 planes (TOMO_FLAG_STAGED, rs_staged_kernel)"
+    ops["siddon"]["form"] = "per-lane gathers from the z-pair copies (siddon_kernel)"
+    ops["siddon_staged"]["form"] = ("quads staged in LDS per window of up to 4 slabs (TOMO_FLAG_STAGED, "
+                                    "siddon_staged_kernel)")
+    ops["siddon_staged"]["n_seg"] = n_seg
+    ops["rs_staged"]["n_samp"] = n_samp
     # compulsory traffic: the volume and the sinogram once each
     comp = float(n) ** 3 * 4 + nsino * 4
-    for name, ms in (("siddon", sid_ms), ("vi", vi_ms), ("rs", rs_ms)):
+    for name, ms in (("siddon", sid_ms), ("vi", vi_ms), ("rs", rs_ms), ("siddon_staged", sid_staged_ms),
+                     ("rs_staged", rs_staged_ms)):
         ops[name]["compulsory_frac"] = round(comp / (ms * 1e-3) / HBM_PEAK, 4)
     # The roofline line is the step's dominant kernel, the Siddon projector: its algorithmic bytes (every positive
     # chord counted as one 4-byte read, plus the sinogram write) over its HIP-event time.

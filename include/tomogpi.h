@@ -86,6 +86,12 @@ extern "C" {
                                         also where 32-bit buffer offsets
                                         apply; same results.  Lets that form
                                         be checked at small sizes.           */
+#define TOMO_FLAG_STAGED        0x20 /* RS, clean Siddon on FP32 volumes:
+                                        the workgroup stages the voxels its
+                                        rays cross, a window of slabs at a
+                                        time, in LDS and reads its taps there
+                                        instead of gathering them per lane
+                                        from the z-pair copies; same results */
 
 typedef struct tomo_ctx  tomo_ctx;
 typedef struct tomo_geom tomo_geom;

@@ -136,7 +136,8 @@ def test_cli_iterative_reconstruction(tmp_path):
         cols = [float(line[20 * c:20 * c + 20]) for c in range(8)]
         assert cols[0] == k and cols[4] == 0.01 and cols[6] == 0.0 and cols[7] == 0.0
         assert cols[2] == pytest.approx(st["residual_sq"], rel=1e-5)       # 6 significant digits
-        assert cols[3] == pytest.approx(st["j_reg"], rel=1e-5) and cols[3] > 0
+        assert cols[3] == pytest.approx(st["j_reg"], rel=1e-5)
+        assert (cols[3] > 0) == (k > 1)   # sum (L x)^2 before the update: 0 for the zero start
         assert cols[1] == pytest.approx(cols[2] + 0.01 * cols[3], rel=1e-5)
         assert cols[5] == pytest.approx(abs(st["alpha"]), rel=1e-5)
     # the timings.log line, appended in the working directory (C/Iter3D_Algo_Gradient.cu:207-210)

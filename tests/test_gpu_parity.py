@@ -91,24 +91,31 @@ def test_vi_fdk_bit_exact(setup):
     assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
 
 
+@pytest.mark.parametrize("staged", [False, True])
 @pytest.mark.parametrize("cube", [False, True])
-def test_rs_bit_exact(setup, cube):
+def test_rs_bit_exact(setup, cube, staged):
+    """RS with per-lane gathers from the z-pair copies (the default) and with
+    LDS-staged taps (TOMO_FLAG_STAGED), both bit-exact."""
     g, o, G = setup
     vol = ellipsoids(g.nx, g.ny, g.nz, count=12, seed=3, amin=2.0, amax=12.0)
     ref = o.rs(vol, cube=cube)
     out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
-    G.project(dev(vol), out, lib.PROJ_RS, lib.FLAG_RS_FOV_CUBE if cube else 0)
+    flags = (lib.FLAG_RS_FOV_CUBE if cube else 0) | (lib.FLAG_STAGED if staged else 0)
+    G.project(dev(vol), out, lib.PROJ_RS, flags)
     got = out.cpu().numpy()
     assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
 
 
-def test_siddon_bit_exact(setup):
+@pytest.mark.parametrize("staged", [False, True])
+def test_siddon_bit_exact(setup, staged):
+    """Clean Siddon with per-lane gathers from the z-pair copies (the
+    default) and with LDS-staged quads (TOMO_FLAG_STAGED)."""
     g, o, G = setup
     rng = np.random.default_rng(5)
     vol = rng.random(g.vol_shape, dtype=np.float32)
     ref = o.siddon(vol)
     out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
-    G.project(dev(vol), out, lib.PROJ_SIDDON)
+    G.project(dev(vol), out, lib.PROJ_SIDDON, lib.FLAG_STAGED if staged else 0)
     got = out.cpu().numpy()
     assert np.array_equal(got, ref), (rel_l2(got, ref), rel_linf(got, ref))
 

@@ -84,14 +84,17 @@ int run_operator(tomo_ctx* ctx, const tomo_geom* g, int flags, const void* in, i
     return TOMO_OK;
 }
 
-// Projector modes: bit 0 = cube FOV (RS) / compat (Siddon), bit 1 = index reads.
-constexpr int kModeIndexReads = 2;
+// Projector modes: bit 0 = cube FOV (RS) / compat (Siddon), bit 1 = index
+// reads, bit 2 = LDS-staged taps.
+constexpr int kModeIndexReads = 2, kModeStaged = 4;
 int proj_mode(int flags, int fov_or_compat) {
-    return ((flags & fov_or_compat) ? 1 : 0) | ((flags & TOMO_FLAG_INDEX_READS) ? kModeIndexReads : 0);
+    return ((flags & fov_or_compat) ? 1 : 0) | ((flags & TOMO_FLAG_INDEX_READS) ? kModeIndexReads : 0) |
+           ((flags & TOMO_FLAG_STAGED) ? kModeStaged : 0);
 }
 ProjExtra extra_of(int mode) {
     ProjExtra x;
     x.index_reads = (mode & kModeIndexReads) != 0;
+    x.staged = (mode & kModeStaged) != 0;
     return x;
 }
 int do_rs(const tomo_geom* g, int mode, Buf in, MutBuf out, hipStream_t s) {

@@ -17,6 +17,7 @@
 // samples and the sum are FP32.  The output is overwritten.
 #include "zpair.hpp"
 #include "batch.hpp"
+#include "stage_box.hpp"
 
 namespace tomo {
 namespace {
@@ -343,6 +344,276 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const v
     rs_tile<TV, TS, BUF>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged RS (FP32 volumes).
+//
+// A workgroup is 32 x 8 detector cells of one view.  When all its rays share
+// the dominant in-plane axis A (|d_A| >= the other two) and its direction,
+// it walks A in windows: for the window of samples with floor(A) in
+// [wlo, whi] it copies the voxels those samples interpolate (planes
+// wlo .. whi + 1, the window's box of z rows and B columns, stage_box.hpp)
+// into LDS, and every lane takes its samples of the window with the eight
+// taps read from there (four ds_read2_b32 per sample).  The staging is double
+// buffered: the copy of window i + 1 is in flight while the workgroup samples
+// window i, and one barrier per window separates them.  A window holds up to
+// kRsWmax sample planes, fewer (2, then 1) when its box is too tall for a
+// buffer.  The box is not clipped to the grid: a sample outside the grid (a
+// cylinder wider than a y extent smaller than x, or a ray whose clipped exit
+// lies behind its entry) reads zero taps there and adds +0, as the range
+// test of the per-ray loop makes it.  Samples, taps, arithmetic and the order
+// of the sum are those of the per-ray loop, so results equal rs_kernel's bit
+// for bit.  A window whose box exceeds a buffer even at one plane, and a
+// workgroup whose rays do not share A and its direction, read their taps
+// from the volume directly.
+constexpr int kRsWmax = 4;  // sample planes per window at most
+
+// The eight taps straight from the volume: the range and border rules and the
+// arithmetic of trilinear_quads.
+__device__ __forceinline__ float trilinear_direct(const float* __restrict__ f, int nx, int ny, int nz, float x,
+                                                  float y, float z) {
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    if (!(xf >= -1.0f && xf <= (float)nx && yf >= -1.0f && yf <= (float)ny && zf >= -1.0f && zf <= (float)nz))
+        return 0.0f;
+    const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
+    const float fx = x - xf, fy = y - yf, fz = z - zf;
+    const size_t sy = (size_t)nx, sz = (size_t)nx * ny;
+    auto t = [&](int X, int Y, int Z) {
+        const bool ok = (unsigned)X < (unsigned)nx && (unsigned)Y < (unsigned)ny && (unsigned)Z < (unsigned)nz;
+        return ok ? f[(size_t)X + sy * (size_t)Y + sz * (size_t)Z] : 0.0f;
+    };
+    const float t000 = t(ix, iy, iz), t100 = t(ix + 1, iy, iz), t010 = t(ix, iy + 1, iz);
+    const float t110 = t(ix + 1, iy + 1, iz), t001 = t(ix, iy, iz + 1), t101 = t(ix + 1, iy, iz + 1);
+    const float t011 = t(ix, iy + 1, iz + 1), t111 = t(ix + 1, iy + 1, iz + 1);
+    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
+    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
+    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
+    return fmaf(fz, c1 - c0, c0);
+}
+
+// A window: samples with floor(A) in [wlo, whi]; staged planes wlo .. whi + 1,
+// nzr z rows from zlo, columns from blo.  All fields are wave-uniform.
+struct RsWindow {
+    int wlo, whi, blo, zlo, nzr;
+    bool fits;
+};
+
+// The window that starts at sample plane `from` in the travel direction: the
+// largest of kRsWmax, 2, 1 planes whose box fits a buffer.
+__device__ __forceinline__ RsWindow rs_plan(const WgCorners& wc, int from, bool inc) {
+    RsWindow w{};
+    for (int W = kRsWmax;; W >>= 1) {
+        const int wlo = inc ? from : from - W + 1, whi = wlo + W - 1;
+        const StageBox b = stage_box(wc, (float)wlo, (float)(whi + 1), 0.0f, 0, 1, kStageRows);
+        const int nzr = b.zhi - b.zlo + 1;
+        w = RsWindow{wlo, whi, b.blo, b.zlo, nzr, b.fits && (W + 1) * nzr <= kStageRows};
+        if (w.fits || W == 1) break;
+    }
+    return w;
+}
+
+#ifndef RS_STAGED_UNROLL
+#define RS_STAGED_UNROLL 4
+#endif
+
+// The window walk of one workgroup along A (0: x, 1: y); inc: A increasing.
+// src: rows along B (A = 0: the y-fastest copy, A = 1: the volume).  first,
+// last: the workgroup's first and last sample planes.  wc: the corner rays
+// (every wave plans the same windows).
+template <int A>
+__device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r, int lr, bool inc, int first, int last,
+                                                const float* __restrict__ src, const float* __restrict__ vol,
+                                                const WgCorners& wc, float* __restrict__ stage, int lane, int wv) {
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    float acc = 0.0f;
+    int s = 0;
+    RsWindow cur = rs_plan(wc, first, inc);
+    if (cur.fits) stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, cur.wlo, cur.whi - cur.wlo + 2, cur.zlo, cur.nzr, cur.blo, stage, lane, wv);
+    for (int b = 0;; b ^= 1) {
+        // this wave's copies of window `cur` have landed; every wave is done with the other buffer
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const int from = inc ? cur.whi + 1 : cur.wlo - 1;
+        const bool more = inc ? from <= last : from >= last;
+        RsWindow nxt = cur;
+        if (more) {
+            nxt = rs_plan(wc, from, inc);
+            if (nxt.fits)
+                stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, nxt.wlo, nxt.whi - nxt.wlo + 2, nxt.zlo, nxt.nzr, nxt.blo,
+                           stage + (b ^ 1) * kStageBufF, lane, wv);
+        }
+        const float* buf = stage + b * kStageBufF;
+        const float wlof = (float)cur.wlo, whif = (float)cur.whi;
+        // offset of tap (a, z, b) in the buffer, ((a - wlo) nzr + z - zlo) pitch + b - blo, as one exact FP32
+        // expression of small integers
+        const int pstride = cur.nzr * kStagePitchP;
+        const float ps = (float)pstride;
+        const float c0 = -((float)cur.wlo * ps + (float)(cur.zlo * kStagePitchP + cur.blo));
+        for (;;) {
+            float px[RS_STAGED_UNROLL], py[RS_STAGED_UNROLL], pz[RS_STAGED_UNROLL];
+            bool act[RS_STAGED_UNROLL];
+#pragma unroll
+            for (int j = 0; j < RS_STAGED_UNROLL; ++j) {
+                const float sf = (float)(s + j);
+                px[j] = fmaf(sf, r.dx, r.ex);
+                py[j] = fmaf(sf, r.dy, r.ey);
+                pz[j] = fmaf(sf, r.dz, r.ez);
+                const float fa = floorf(A == 0 ? px[j] : py[j]);
+                act[j] = (s + j < lr) & (fa >= wlof) & (fa <= whif);
+            }
+            if (__ballot(act[0]) == 0ull) break;
+            float v[RS_STAGED_UNROLL];
+            if (cur.fits) {
+                float fx[RS_STAGED_UNROLL], fy[RS_STAGED_UNROLL], fz[RS_STAGED_UNROLL];
+                int off[RS_STAGED_UNROLL];
+#pragma unroll
+                for (int j = 0; j < RS_STAGED_UNROLL; ++j) {
+                    const float xf = floorf(px[j]), yf = floorf(py[j]), zf = floorf(pz[j]);
+                    fx[j] = px[j] - xf;
+                    fy[j] = py[j] - yf;
+                    fz[j] = pz[j] - zf;
+                    const float af = A == 0 ? xf : yf, bf = A == 0 ? yf : xf;
+                    const int o = (int)fmaf(af, ps, fmaf(zf, (float)kStagePitchP, bf + c0));
+                    off[j] = act[j] ? o : 0;
+                }
+                float u[RS_STAGED_UNROLL][4], w[RS_STAGED_UNROLL][4];
+#pragma unroll
+                for (int j = 0; j < RS_STAGED_UNROLL; ++j) {
+                    const float* q0 = buf + off[j];
+                    const float* q1 = q0 + pstride;
+                    // (b, z), (b, z+1), (b+1, z), (b+1, z+1) on planes ia, ia + 1
+                    u[j][0] = q0[0];
+                    u[j][1] = q0[kStagePitchP];
+                    u[j][2] = q0[1];
+                    u[j][3] = q0[kStagePitchP + 1];
+                    w[j][0] = q1[0];
+                    w[j][1] = q1[kStagePitchP];
+                    w[j][2] = q1[1];
+                    w[j][3] = q1[kStagePitchP + 1];
+                }
+#pragma unroll
+                for (int j = 0; j < RS_STAGED_UNROLL; ++j) {
+                    float t000, t100, t010, t110, t001, t101, t011, t111;
+                    if (A == 0) {
+                        t000 = u[j][0]; t001 = u[j][1]; t010 = u[j][2]; t011 = u[j][3];
+                        t100 = w[j][0]; t101 = w[j][1]; t110 = w[j][2]; t111 = w[j][3];
+                    } else {
+                        t000 = u[j][0]; t001 = u[j][1]; t100 = u[j][2]; t101 = u[j][3];
+                        t010 = w[j][0]; t011 = w[j][1]; t110 = w[j][2]; t111 = w[j][3];
+                    }
+                    const float c00 = fmaf(fx[j], t100 - t000, t000), c10 = fmaf(fx[j], t110 - t010, t010);
+                    const float c01 = fmaf(fx[j], t101 - t001, t001), c11 = fmaf(fx[j], t111 - t011, t011);
+                    const float e0 = fmaf(fy[j], c10 - c00, c00), e1 = fmaf(fy[j], c11 - c01, c01);
+                    const float val = fmaf(fz[j], e1 - e0, e0);
+                    v[j] = act[j] ? val : 0.0f;
+                }
+            } else {
+#pragma unroll
+                for (int j = 0; j < RS_STAGED_UNROLL; ++j)
+                    v[j] = act[j] ? trilinear_direct(vol, nx, ny, nz, px[j], py[j], pz[j]) : 0.0f;
+            }
+            int adv = 0;
+#pragma unroll
+            for (int j = 0; j < RS_STAGED_UNROLL; ++j) {
+                acc += v[j];
+                adv += act[j] ? 1 : 0;
+            }
+            s += adv;
+        }
+        if (!more) break;
+        cur = nxt;
+    }
+    return acc;
+}
+
+// Forward RS with LDS-staged taps (FP32 volume `vol`, `volT` its y-fastest
+// copy).  Grid: x = column group, y = view, z = row group.
+template <typename TS>
+__global__ __launch_bounds__(BX* BY, 4) void rs_staged_kernel(DevGeom g, int cube, const float* __restrict__ vol,
+                                                             const float* __restrict__ volT, TS* __restrict__ sino,
+                                                             float sino_scale) {
+    __shared__ float stage[2 * kStageBufF];
+    __shared__ int wsh[4][3];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int bx = blockIdx.x, p = blockIdx.y, bz = blockIdx.z;
+    const int m = bx * BX + threadIdx.x;
+    const int n = bz * BY + threadIdx.y;
+    const bool inside = m < g.nu && n < g.nv;
+    RsRay r{0, 0, 0, 0, 0, 0, 0};
+    if (inside) r = rs_setup(g, cube, p, m, n);
+    const bool ok = r.lr > 0;
+    int code = -1;
+    if (ok) {
+        const float ax = fabsf(r.dx), ay = fabsf(r.dy), az = fabsf(r.dz);
+        if (ax > 0.0f && ax >= ay && ax >= az) code = r.dx > 0.0f ? 0 : 1;
+        else if (ay > 0.0f && ay >= ax && ay >= az) code = r.dy > 0.0f ? 2 : 3;
+    }
+    // The workgroup's mode: 0..3 (axis x / y, increasing / decreasing), 4: no
+    // ray, -1: the rays differ.
+    const unsigned long long okmask = __ballot(ok);
+    int wmode = 4;
+    if (okmask != 0ull) {
+        wmode = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (__ballot(ok && code == c) == okmask) wmode = c;
+    }
+    if (lane == 0) wsh[wv][0] = wmode;
+    __syncthreads();
+    int mode = 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int mk = wsh[k][0];
+        if (mk != 4) mode = (mode == 4 || mode == mk) ? mk : -1;
+    }
+    mode = __builtin_amdgcn_readfirstlane(mode);
+    float acc = 0.0f;
+    if (mode == -1) {
+        for (int s = 0; s < r.lr; ++s) {
+            const float sf = (float)s;
+            acc += trilinear_direct(vol, g.nx, g.ny, g.nz, fmaf(sf, r.dx, r.ex), fmaf(sf, r.dy, r.ey),
+                                    fmaf(sf, r.dz, r.ez));
+        }
+    } else if (mode != 4) {
+        const int A = mode < 2 ? 0 : 1;
+        const bool inc = (mode & 1) == 0;
+        // the workgroup's first and last sample planes along the travel direction
+        int first = inc ? INT_MAX : INT_MIN, last = inc ? INT_MIN : INT_MAX;
+        if (ok) {
+            const float e = A == 0 ? r.ex : r.ey, d = A == 0 ? r.dx : r.dy;
+            first = (int)floorf(e);
+            last = (int)floorf(fmaf((float)(r.lr - 1), d, e));
+        }
+        first = inc ? wave_min(first) : wave_max(first);
+        last = inc ? wave_max(last) : wave_min(last);
+        __syncthreads();  // every wave has read wsh[*][0]
+        if (lane == 0) {
+            wsh[wv][1] = first;
+            wsh[wv][2] = last;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            first = inc ? min(first, wsh[k][1]) : max(first, wsh[k][1]);
+            last = inc ? max(last, wsh[k][2]) : min(last, wsh[k][2]);
+        }
+        first = __builtin_amdgcn_readfirstlane(first);
+        last = __builtin_amdgcn_readfirstlane(last);
+        const int mlo = bx * BX, mhi = min(bx * BX + BX - 1, g.nu - 1);
+        const int nlo = bz * BY, nhi = min(bz * BY + BY - 1, g.nv - 1);
+        const WgCorners wc = wg_corners(g, A, p, mlo, mhi, nlo, nhi);
+        if (A == 0)
+            acc = rs_staged_walk<0>(g, r, ok ? r.lr : 0, inc, first, last, volT, vol, wc, stage, lane, wv);
+        else
+            acc = rs_staged_walk<1>(g, r, ok ? r.lr : 0, inc, first, last, vol, vol, wc, stage, lane, wv);
+    }
+    if (inside) {
+        const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
+        Storage<TS>::store(sino, o, r.lr > 0 ? acc * g.delta : 0.0f, sino_scale);
+    }
+}
+
 // Shared launch (FP32 or BF16 (kItemBF16) z-pair copies, FP32 sinograms);
 // item flags bit 0 selects the cube FOV.
 // 8 waves per SIMD (52 VGPRs, no spills; unbounded, its four sampler
@@ -417,6 +688,23 @@ __global__ __launch_bounds__(BX* BY) void rs_count_kernel(DevGeom g, int cube, u
 int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s, const ProjExtra& x) {
     const DevGeom& d = g->dev;
     const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
+    if (x.staged && vol.dtype == TOMO_F32 && !x.index_reads) {
+        // LDS-staged taps; the y-fastest copy of the volume for the x slabs
+        float* vt = nullptr;
+        TOMO_HIP(hipMallocAsync((void**)&vt, g->nvox() * sizeof(float), s));
+        hipError_t le = launch_xy_transpose(vol.p, vt, 4, d.nx, d.ny, d.nz, s);
+        if (le == hipSuccess) {
+            TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
+                hipLaunchKernelGGL((rs_staged_kernel<TS>), grid, dim3(BX, BY), 0, s, d, cube, (const float*)vol.p,
+                                   (const float*)vt, (TS*)sino.p, sino.scale);
+            });
+            le = hipGetLastError();
+        }
+        const hipError_t fe = hipFreeAsync(vt, s);
+        TOMO_HIP(le);
+        TOMO_HIP(fe);
+        return TOMO_OK;
+    }
     const float inv = 1.0f / vol.scale;
     const uint32_t es = vol.dtype == TOMO_F32 ? 4u : 2u;
     ZPairs zp;

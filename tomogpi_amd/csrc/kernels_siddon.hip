@@ -18,6 +18,7 @@
 // % 256 with unsigned wrap, no z clip.  Defined for 256^3 volumes only.
 #include "zpair.hpp"
 #include "batch.hpp"
+#include "stage_box.hpp"
 
 namespace tomo {
 namespace {
@@ -450,6 +451,295 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
     if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), acc, sino_scale);
 }
 
+// ---------------------------------------------------------------------------
+// LDS-staged clean Siddon (FP32 volumes).
+//
+// The workgroup (32 x 8 detector cells of one view) walks the slabs of A in
+// lockstep when all its rays share A and its direction: at iteration k every
+// live lane is in slab I + k sA, I the workgroup's first slab.  Iterations go
+// in windows of up to kSidWmax slabs [wlo, whi] (fewer when the box is too
+// tall for a buffer); for each window the workgroup copies the voxels its
+// rays can touch there into LDS (stage_box.hpp: corner rays at the window's
+// boundary planes wlo - 1/2 and whi + 1/2, the voxel round(B) and its
+// neighbours, likewise in z), and each slab's quad -- the current
+// voxel, its B neighbour, its z neighbour and the diagonal one -- is two
+// ds_read2_b32 there instead of one 16-byte gather from a z-pair copy.  The
+// per-slab arithmetic is sid_walk_quads', so every lane takes the segments
+// of the per-ray traversal in its order: results are bit-identical to it.
+// A wave whose rays go up and down in z copies its share of rows and keeps
+// the barriers, then steps per lane; a window whose box exceeds the buffer
+// reads the quads from the volume; a workgroup whose rays do not share A and
+// its direction steps per lane on the volume.
+constexpr int kSidWmax = 4;  // slabs per window at most
+
+// Per-lane general traversal on the volume (x fastest), as sid_step.
+template <typename RD>
+__device__ __forceinline__ float sid_general(const SidRay& r, const RD& rd, bool ok, int nx, int ny, int nz) {
+    SidState s;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s.tn[a] = r.first[a];
+        s.cnt[a] = 0.0f;
+    }
+    const unsigned sy = (unsigned)nx, sz = (unsigned)nx * (unsigned)ny;
+    s.lin = ok ? (unsigned)r.idx[0] + sy * (unsigned)r.idx[1] + sz * (unsigned)r.idx[2] : 0u;
+    s.dlin[0] = (unsigned)r.step[0];
+    s.dlin[1] = (unsigned)r.step[1] * sy;
+    s.dlin[2] = (unsigned)r.step[2] * sz;
+    s.t = 0.0f;
+    s.acc = 0.0f;
+    s.live = ok;
+    int guard = nx + ny + nz + 4;
+    while (s.live && guard-- > 0) sid_step(s, r, rd);
+    return s.acc;
+}
+
+// The quad of corner (iA, bc, zc) straight from the volume, taps off the
+// grid 0: (b, z), (b, z+1), (b+1, z), (b+1, z+1).
+template <int A>
+__device__ __forceinline__ float4 sid_quad_direct(const float* __restrict__ vol, int nx, int ny, int nz, int iA, int bc,
+                                                  int zc) {
+    auto at = [&](int b, int z) {
+        const int x = A == 0 ? iA : b, y = A == 0 ? b : iA;
+        const bool ok = (unsigned)x < (unsigned)nx && (unsigned)y < (unsigned)ny && (unsigned)z < (unsigned)nz;
+        return ok ? vol[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * (size_t)z)] : 0.0f;
+    };
+    return make_float4(at(bc, zc), at(bc, zc + 1), at(bc + 1, zc), at(bc + 1, zc + 1));
+}
+
+// A window of slabs [wlo, whi] staged as planes wlo .. whi, nzr z rows from
+// zlo, columns from blo.  All fields are wave-uniform.
+struct SidWindow {
+    int wlo, whi, blo, zlo, nzr;
+    bool fits;
+};
+
+// The window that starts at slab `from` in the walk's direction: the largest
+// of kSidWmax, 2, 1 slabs whose box fits a buffer.
+__device__ __forceinline__ SidWindow sid_plan(const WgCorners& wc, int from, int sAw) {
+    SidWindow w{};
+    for (int W = kSidWmax;; W >>= 1) {
+        const int wlo = sAw > 0 ? from : from - W + 1, whi = wlo + W - 1;
+        const StageBox b = stage_box(wc, (float)wlo - 0.5f, (float)whi + 0.5f, 0.5f, 1, 1, kStageRows);
+        const int nzr = b.zhi - b.zlo + 1;
+        w = SidWindow{wlo, whi, b.blo, b.zlo, nzr, b.fits && W * nzr <= kStageRows};
+        if (w.fits || W == 1) break;
+    }
+    return w;
+}
+
+// walks: the wave takes part in the lockstep walk (otherwise it only copies
+// its rows and keeps the barriers).  I: the workgroup's first slab, sAw its
+// direction along A, trips its iteration count: at iteration k every live
+// lane is in slab I + k sAw.  The staging is double buffered: the copy of
+// window i + 1 is in flight while the workgroup walks window i, one barrier
+// per window.
+template <int A, bool ZNEG, int BNEG>
+__device__ __forceinline__ float sid_walk_staged(const DevGeom& g, const SidRay& r, bool live, bool walks, int I,
+                                                 int sAw, int trips, const float* __restrict__ src,
+                                                 const float* __restrict__ vol, const WgCorners& wc,
+                                                 float* __restrict__ stage, int lane, int wv) {
+    constexpr int B = 1 - A;
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    const int sB = r.step[B];
+    const bool bneg = BNEG == 2 ? sB < 0 : BNEG == 1;
+    float tA = r.first[A], tB = r.first[B], tz = r.first[2];
+    float cB = 0.0f, cz = 0.0f, cA = 0.0f;
+    float t = 0.0f, acc = 0.0f;
+    const int lag = live ? abs(r.idx[A] - I) : 0;
+    const int b0 = bneg ? r.idx[B] - 1 : r.idx[B], z0 = ZNEG ? r.idx[2] - 1 : r.idx[2];
+    // the quad corner's (z, B) part of its offset in a staged plane, z kStagePitchP + B; the plane
+    // and the window's origin are wave-uniform
+    int ez = live ? z0 * kStagePitchP + b0 : 0;
+    const int dB = sB, dZ = r.step[2] * kStagePitchP;
+    auto body = [&](const float4 v) {
+        // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
+        const float V0 = bneg ? (ZNEG ? v.w : v.z) : (ZNEG ? v.y : v.x);
+        const float Vb = bneg ? (ZNEG ? v.y : v.x) : (ZNEG ? v.w : v.z);
+        const float Vz = bneg ? (ZNEG ? v.z : v.w) : (ZNEG ? v.x : v.y);
+        const float Vbz = bneg ? (ZNEG ? v.x : v.y) : (ZNEG ? v.z : v.w);
+        const bool stepB = tB <= tA, stepZ = tz <= tA;
+        const float V1 = tB < tz ? Vb : Vz;
+        const float V2 = stepB ? (stepZ ? Vbz : Vb) : (stepZ ? Vz : V0);
+        const float c3 = min_nn(tA, r.tmax);
+        const float c1 = min_nn(min_nn(tB, tz), c3), c2 = min_nn(max_nn(tB, tz), c3);
+        acc = fmaf(c1 - t, V0, acc);
+        acc = fmaf(c2 - c1, V1, acc);
+        acc = fmaf(c3 - c2, V2, acc);
+        t = c3;
+        ez += (stepB ? dB : 0) + (stepZ ? dZ : 0);
+        cB += stepB ? 1.0f : 0.0f;
+        cz += stepZ ? 1.0f : 0.0f;
+        tB = fmaf(cB, r.inc[B], r.first[B]);
+        tz = fmaf(cz, r.inc[2], r.first[2]);
+        cA += 1.0f;
+        tA = fmaf(cA, r.inc[A], r.first[A]);
+        live = t < r.tmax;
+    };
+    const int last = I + sAw * (trips - 1);  // the slab of the last iteration
+    SidWindow cur = sid_plan(wc, I, sAw);
+    if (cur.fits) stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, cur.wlo, cur.whi - cur.wlo + 1, cur.zlo, cur.nzr, cur.blo, stage, lane, wv);
+    for (int b = 0;; b ^= 1) {
+        // this wave's copies of window `cur` have landed; every wave is done with the other buffer
+        __builtin_amdgcn_s_waitcnt(0);
+        __syncthreads();
+        const int from = sAw > 0 ? cur.whi + 1 : cur.wlo - 1;
+        const bool more = sAw > 0 ? from <= last : from >= last;
+        SidWindow nxt = cur;
+        if (more) {
+            nxt = sid_plan(wc, from, sAw);
+            if (nxt.fits)
+                stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, nxt.wlo, nxt.whi - nxt.wlo + 1, nxt.zlo, nxt.nzr, nxt.blo,
+                           stage + (b ^ 1) * kStageBufF, lane, wv);
+        }
+        if (walks) {
+            // iterations of the window's slabs, in the walk's order
+            const int k0 = sAw > 0 ? cur.wlo - I : I - cur.whi;
+            const int k1 = min(trips, k0 + (cur.whi - cur.wlo + 1));
+            if (cur.fits) {
+                const int ps = cur.nzr * kStagePitchP;
+                const float* buf = stage + b * kStageBufF - (cur.zlo * kStagePitchP + cur.blo);
+                for (int outer = k0; outer < k1; ++outer) {
+                    if (live && outer >= lag) {
+                        const int slab = I + sAw * outer;
+                        const float* q = buf + (slab - cur.wlo) * ps + ez;
+                        body(make_float4(q[0], q[kStagePitchP], q[1], q[kStagePitchP + 1]));
+                    }
+                }
+            } else {
+                for (int outer = k0; outer < k1; ++outer) {
+                    if (live && outer >= lag) {
+                        const int iA = I + sAw * outer;
+                        const int bc = b0 + sB * (int)cB, zc = z0 + r.step[2] * (int)cz;
+                        body(sid_quad_direct<A>(vol, nx, ny, nz, iA, bc, zc));
+                    }
+                }
+            }
+        }
+        if (!more) break;
+        cur = nxt;
+    }
+    return acc;
+}
+
+template <int A, bool ZNEG>
+__device__ __forceinline__ float sid_walk_staged_b(const DevGeom& g, const SidRay& r, bool live, bool walks, int I,
+                                                   int sAw, int trips, const float* __restrict__ src,
+                                                   const float* __restrict__ vol, const WgCorners& wc,
+                                                   float* __restrict__ stage, int lane, int wv) {
+    const bool pos_all = __ballot(live && r.step[1 - A] < 0) == 0ull;
+    const bool neg_all = __ballot(live && r.step[1 - A] >= 0) == 0ull;
+    if (pos_all) return sid_walk_staged<A, ZNEG, 0>(g, r, live, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+    if (neg_all) return sid_walk_staged<A, ZNEG, 1>(g, r, live, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+    return sid_walk_staged<A, ZNEG, 2>(g, r, live, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+}
+
+// Forward clean Siddon with LDS-staged quads (FP32 volume `vol`, `volT` its
+// y-fastest copy).  Grid: x = column group, y = view, z = row group.
+template <typename TS>
+__global__ __launch_bounds__(BX* BY, 4) void siddon_staged_kernel(DevGeom g, const float* __restrict__ vol,
+                                                                 const float* __restrict__ volT, unsigned vol_bytes,
+                                                                 TS* __restrict__ sino, float sino_scale) {
+    __shared__ float stage[2 * kStageBufF];
+    __shared__ int wsh[4][3];
+    const int tid = threadIdx.y * BX + threadIdx.x;
+    const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int bx = blockIdx.x, p = blockIdx.y, bz = blockIdx.z;
+    const int m = bx * BX + threadIdx.x;
+    const int n = bz * BY + threadIdx.y;
+    const bool inside = m < g.nu && n < g.nv;
+    SidRay r;
+    r.ok = false;
+    if (inside) r = sid_setup(g, p, m, n);
+    const bool ok = r.ok && r.tmax > 0.0f;
+    // slab code as in siddon_tile
+    int code = -1;
+    if (ok) {
+        const float big = 3.0e38f, margin = 1.0e-6f * r.tmax;
+        const float ix = r.step[0] ? r.inc[0] : big, iy = r.step[1] ? r.inc[1] : big;
+        const float iz = r.step[2] ? r.inc[2] : big;
+        if (r.step[0] != 0 && iy > ix + margin && iz > ix + margin && r.first[0] <= ix)
+            code = r.step[0] > 0 ? 0 : 1;
+        else if (r.step[1] != 0 && ix > iy + margin && iz > iy + margin && r.first[1] <= iy)
+            code = r.step[1] > 0 ? 2 : 3;
+    }
+    const unsigned long long okmask = __ballot(ok);
+    int wmode = 4;  // 0..3: the wave's common code; -1: codes differ; 4: no ray
+    if (okmask != 0ull) {
+        wmode = -1;
+#pragma unroll
+        for (int c = 0; c < 4; ++c)
+            if (__ballot(ok && code == c) == okmask) wmode = c;
+    }
+    if (lane == 0) wsh[wv][0] = wmode;
+    __syncthreads();
+    int mode = 4;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        const int mk = wsh[k][0];
+        if (mk != 4) mode = (mode == 4 || mode == mk) ? mk : -1;
+    }
+    mode = __builtin_amdgcn_readfirstlane(mode);
+    const int nx = g.nx, ny = g.ny, nz = g.nz;
+    float acc = 0.0f;
+    if (mode == -1) {
+        if (vol_bytes != 0u)
+            acc = sid_general(r, make_reader<float, true>(vol, vol_bytes, 1.0f), ok, nx, ny, nz);
+        else
+            acc = sid_general(r, make_reader<float, false>(vol, 0u, 1.0f), ok, nx, ny, nz);
+    } else if (mode != 4) {
+        const int A = mode < 2 ? 0 : 1;
+        const int sAw = (mode & 1) ? -1 : 1;
+        const int nA = A == 0 ? nx : ny;
+        // the workgroup's first slab, then its iteration count (as sid_walk_quads, with the workgroup's I)
+        const int iA = ok ? r.idx[A] : 0;
+        int I = sAw > 0 ? wave_min(ok ? iA : 0x7fffffff) : wave_max(ok ? iA : -0x7fffffff);
+        __syncthreads();  // every wave has read wsh[*][0]
+        if (lane == 0) wsh[wv][1] = I;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) I = sAw > 0 ? min(I, wsh[k][1]) : max(I, wsh[k][1]);
+        I = __builtin_amdgcn_readfirstlane(I);
+        int span = 0;
+        if (ok) {
+            const int lag = abs(iA - I);
+            const float kf = ceilf((r.tmax - r.first[A]) * __builtin_amdgcn_rcpf(r.inc[A]));
+            span = lag + (int)fminf(fmaxf(kf, 0.0f), 1.0e6f) + 3;
+        }
+        span = wave_max(span);
+        if (lane == 0) wsh[wv][2] = span;
+        __syncthreads();
+        int trips = 0;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) trips = max(trips, wsh[k][2]);
+        trips = __builtin_amdgcn_readfirstlane(min(trips, nA + 2));
+        // one z direction per wave, or none (then the wave steps per lane after the windows)
+        const bool zneg_all = __ballot(ok && r.step[2] > 0) == 0ull;
+        const bool zpos_all = __ballot(ok && r.step[2] < 0) == 0ull;
+        const bool walks = zneg_all || zpos_all;
+        const int mlo = bx * BX, mhi = min(bx * BX + BX - 1, g.nu - 1);
+        const int nlo = bz * BY, nhi = min(bz * BY + BY - 1, g.nv - 1);
+        const WgCorners wc = wg_corners(g, A, p, mlo, mhi, nlo, nhi);
+        const float* src = A == 0 ? volT : vol;
+        const bool zneg = zneg_all && !zpos_all;
+        if (A == 0) {
+            acc = zneg ? sid_walk_staged_b<0, true>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv)
+                       : sid_walk_staged_b<0, false>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+        } else {
+            acc = zneg ? sid_walk_staged_b<1, true>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv)
+                       : sid_walk_staged_b<1, false>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+        }
+        if (!walks) {
+            if (vol_bytes != 0u)
+                acc = sid_general(r, make_reader<float, true>(vol, vol_bytes, 1.0f), ok, nx, ny, nz);
+            else
+                acc = sid_general(r, make_reader<float, false>(vol, 0u, 1.0f), ok, nx, ny, nz);
+        }
+    }
+    if (inside) Storage<TS>::store(sino, (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p), acc, sino_scale);
+}
+
 template <typename TV, typename TS, bool BUF>
 __global__ __launch_bounds__(BX* BY) void siddon_kernel(DevGeom g, const TV* __restrict__ qx,
                                                       const TV* __restrict__ qy, unsigned qx_bytes,
@@ -609,6 +899,27 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
             });
         });
         TOMO_HIP(hipGetLastError());
+        return TOMO_OK;
+    }
+    if (x.staged && vol.dtype == TOMO_F32 && !x.index_reads) {
+        // LDS-staged quads; the y-fastest copy of the volume for the x slabs
+        if (g->nvox() >= (1ull << 32))
+            return fail(TOMO_ERR_UNSUPPORTED, "volume too large for the 32-bit traversal index of the Siddon projector");
+        float* vt = nullptr;
+        TOMO_HIP(hipMallocAsync((void**)&vt, g->nvox() * sizeof(float), s));
+        hipError_t le = launch_xy_transpose(vol.p, vt, 4, d.nx, d.ny, d.nz, s);
+        if (le == hipSuccess) {
+            const uint64_t vb = (uint64_t)g->nvox() * 4ull;
+            const unsigned vol_bytes = vb < (1ull << 32) ? (unsigned)vb : 0u;  // 0: index reads
+            TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
+                hipLaunchKernelGGL((siddon_staged_kernel<TS>), grid, dim3(BX, BY), 0, s, d, (const float*)vol.p,
+                                   (const float*)vt, vol_bytes, (TS*)sino.p, sino.scale);
+            });
+            le = hipGetLastError();
+        }
+        const hipError_t fe = hipFreeAsync(vt, s);
+        TOMO_HIP(le);
+        TOMO_HIP(fe);
         return TOMO_OK;
     }
     // z-pair copies of the volume for x slabs and y slabs (stream-ordered

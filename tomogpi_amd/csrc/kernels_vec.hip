@@ -7,6 +7,7 @@
 // as the reference host loops).  Squared norms are sum double(x)^2
 // (C/Image3D_CPU.cu:363-390; "L2Norm" without the square root), reduced in a
 // fixed order: per-block partials, then one block.
+#include <mutex>
 #include <type_traits>
 
 #include "zpair.hpp"
@@ -265,6 +266,26 @@ __global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, 
     zpair_y_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
+// out[y + ny (x + nx z)] = in[x + nx (y + ny z)]: the volume with y fastest,
+// one 32 x 32 (x, y) tile of one z plane per workgroup, through LDS.
+template <typename T>
+__global__ __launch_bounds__(256) void xy_transpose_kernel(const T* __restrict__ in, T* __restrict__ out, int nx,
+                                                         int ny) {
+    __shared__ T tile[32][33];
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
+    const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
+    const size_t plane = (size_t)nx * ny * blockIdx.z;
+    for (int r = ty; r < 32; r += 8) {
+        const int x = x0 + tx, y = y0 + r;
+        if (x < nx && y < ny) tile[r][tx] = in[plane + (size_t)y * nx + x];
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int y = y0 + tx, x = x0 + r;
+        if (x < nx && y < ny) out[plane + (size_t)x * ny + y] = tile[tx][r];
+    }
+}
+
 template <typename T>
 __global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
                                                     int nz) {
@@ -502,6 +523,17 @@ hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, 
     return hipGetLastError();
 }
 
+hipError_t launch_xy_transpose(const void* vol, void* out, size_t es, int nx, int ny, int nz, hipStream_t s) {
+    const dim3 grid(ceil_div(nx, 32), ceil_div(ny, 32), nz);
+    if (es == 4)
+        hipLaunchKernelGGL((xy_transpose_kernel<uint32_t>), grid, dim3(256), 0, s, (const uint32_t*)vol,
+                           (uint32_t*)out, nx, ny);
+    else
+        hipLaunchKernelGGL((xy_transpose_kernel<uint16_t>), grid, dim3(256), 0, s, (const uint16_t*)vol,
+                           (uint16_t*)out, nx, ny);
+    return hipGetLastError();
+}
+
 hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s) {
     hipLaunchKernelGGL(zpair_x_batch_kernel, dim3(kBatchGrid), dim3(256), 0, s, t);
     return hipGetLastError();
@@ -610,12 +642,8 @@ static int finish_reduce(double* part, int blocks, double* out, hipStream_t s) {
     return TOMO_OK;
 }
 
-int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s) {
-    (void)ctx;
-    double* part = nullptr;
-    TOMO_HIP(hipMallocAsync((void**)&part, sizeof(double) * (kReduceBlocks + 1), s));
-    unsigned blocks = grid_for(n);
-    if (blocks > (unsigned)kReduceBlocks) blocks = kReduceBlocks;
+// Per-block partials of sum double(x)^2 into part[0, blocks).
+static int launch_sq_partial(Buf x, size_t n, unsigned blocks, double* part, hipStream_t s) {
     const float inv = 1.0f / x.scale;
     switch (x.dtype) {
         case TOMO_F32:
@@ -630,6 +658,25 @@ int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s) {
             hipLaunchKernelGGL((sq_partial_kernel<__hip_bfloat16>), dim3(blocks), dim3(kBlock), 0, s,
                                (const __hip_bfloat16*)x.p, inv, n, part);
             break;
+    }
+    TOMO_HIP(hipGetLastError());
+    return TOMO_OK;
+}
+
+static unsigned reduce_blocks(size_t n) {
+    const unsigned b = grid_for(n);
+    return b > (unsigned)kReduceBlocks ? (unsigned)kReduceBlocks : b;
+}
+
+int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s) {
+    (void)ctx;
+    double* part = nullptr;
+    TOMO_HIP(hipMallocAsync((void**)&part, sizeof(double) * (kReduceBlocks + 1), s));
+    const unsigned blocks = reduce_blocks(n);
+    const int rc = launch_sq_partial(x, n, blocks, part, s);
+    if (rc) {
+        (void)hipFreeAsync(part, s);
+        return rc;
     }
     return finish_reduce(part, (int)blocks, out, s);
 }

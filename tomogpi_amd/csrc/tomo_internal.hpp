@@ -98,6 +98,7 @@ int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s);
 // stream-ordered pool for the call.
 struct ProjExtra {
     bool index_reads = false;
+    bool staged = false;  // RS, Siddon (FP32 volumes): LDS-staged taps instead of per-lane gathers
     void* qx = nullptr;
     void* qy = nullptr;
 };

@@ -357,6 +357,8 @@ int main(int argc, char** argv) {
                << "ms\n";
         }
         CHECK(write_file(res + "finalVolume.v", vol, nvox), "finalVolume.v");
+        // doMainIter's own save of the result (C/Iter3D_main.cu:410), skipped with --no_save
+        if (o.save) CHECK(write_file(res + "final_Volume.v", vol, nvox), "final_Volume.v");
         std::printf("results in %s\n", res.c_str());
     }
     const long ms = (long)std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count();

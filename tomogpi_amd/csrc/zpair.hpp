@@ -23,6 +23,11 @@ __device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int
 // Fills qx and qy from an x-fastest volume of es-byte elements (es 4 or 2).
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s);
 
+// out[y + ny (x + nx z)] = vol[x + nx (y + ny z)] for es-byte elements (es 4
+// or 2): the volume with y fastest, read by the LDS-staged RS projector for
+// slabs along x.
+hipError_t launch_xy_transpose(const void* vol, void* out, size_t es, int nx, int ny, int nz, hipStream_t s);
+
 // Both copies for the duration of one operator call, filled from the volume:
 // alloc() then release() on the same stream.  The copies come from the
 // stream-ordered pool, or are the caller's (ProjExtra::qx, qy), which

@@ -22,7 +22,7 @@ LIB_PATH = os.path.join(PKG, "libtomogpi_mi355x.so")
 TOMO_OK = 0
 F32, F16, BF16, F64 = 0, 1, 2, 3
 PROJ_RS, PROJ_SIDDON, BP_VI = 0, 1, 0
-FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR, FLAG_FDK, FLAG_INDEX_READS = 0x1, 0x2, 0x4, 0x8, 0x10
+FLAG_RS_FOV_CUBE, FLAG_SIDDON_COMPAT, FLAG_HOST_PTR, FLAG_FDK, FLAG_INDEX_READS, FLAG_STAGED = 0x1, 0x2, 0x4, 0x8, 0x10, 0x20
 PAIR_RSVI, PAIR_SIDDONVI = 0, 1
 SCHEDULE_DIRECT, SCHEDULE_RECURRENT = 0, 1
 ALGO_GD, ALGO_SIRT = 0, 1
