@@ -214,6 +214,18 @@ int tomo_axpy(tomo_ctx* ctx, float* y, const float* x, double lambda, int positi
  * Synchronous: the result is on the host when the call returns.             */
 int tomo_sq_norm(tomo_ctx* ctx, const void* x, int dtype, float scale, size_t n,
                  double* out, void* stream);
+/* tomo_sq_norm without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels, summation order and result as
+ * tomo_sq_norm; the partial sums live in a 16 KiB device array of the
+ * library (2048 partials and the final sum), one per GPU, held until the
+ * process ends.  Every n fits it (the partials are capped at 2048 blocks);
+ * TOMO_ERR_INVALID_ARG if the context's device has no array set up.  Calls
+ * that overlap on one GPU (other threads or streams) race on that array.
+ * Synchronous.                                                             */
+int tomo_sq_norm_noalloc(tomo_ctx* ctx, const void* x, int dtype, float scale, size_t n,
+                         double* out, void* stream);
 /* <a,b> in FP64 (adjoint tests).  Synchronous.                               */
 int tomo_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, void* stream);
 /* y = L x, L = 7-point Laplacian 6x - sum of 6 neighbours, zero padding

@@ -158,6 +158,8 @@ int tomo_ctx_create(int device, tomo_ctx** out) {
     TOMO_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(TOMO_ERR_INVALID_ARG, "no such device " + std::to_string(device));
     TOMO_HIP(hipSetDevice(device));
+    const int rc = noalloc_scratch_init(device);
+    if (rc) return rc;
     auto* c = new tomo_ctx();
     c->device = device;
     *out = c;
@@ -332,6 +334,14 @@ int tomo_sq_norm(tomo_ctx* ctx, const void* x, int dtype, float scale, size_t n,
     if (!x || !out) return fail(TOMO_ERR_INVALID_ARG, "null pointer");
     if ((rc = check_scale(dtype, scale))) return rc;
     return reduce_sq_norm(ctx, Buf{x, dtype, scale}, n, out, stream_of(ctx, stream));
+}
+int tomo_sq_norm_noalloc(tomo_ctx* ctx, const void* x, int dtype, float scale, size_t n, double* out,
+                         void* stream) {
+    int rc = bind(ctx);
+    if (rc) return rc;
+    if (!x || !out) return fail(TOMO_ERR_INVALID_ARG, "null pointer");
+    if ((rc = check_scale(dtype, scale))) return rc;
+    return reduce_sq_norm_noalloc(ctx->device, Buf{x, dtype, scale}, n, out, stream_of(ctx, stream));
 }
 int tomo_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, void* stream) {
     int rc = bind(ctx);

@@ -681,6 +681,62 @@ int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s) {
     return finish_reduce(part, (int)blocks, out, s);
 }
 
+// ---- allocation-free squared norm (tomo_sq_norm_noalloc) ------------------
+// The same kernels as reduce_sq_norm, with the partial sums and the final sum
+// in `scratch`, a file-scope device array of kScratchDoubles doubles (16 KiB)
+// that every GPU the library runs on holds until the process ends.  In host
+// code the array's name is not a device address: noalloc_scratch_init, called
+// once per device with that device current (tomo_ctx_create), looks the
+// address up with hipGetSymbolAddress and keeps it per device, and only that
+// pointer is ever passed to a kernel or a copy.  Every call on one GPU uses
+// that GPU's one array, so calls that overlap on a GPU (two streams, two
+// threads) race on it; nothing in it needs clearing, since every partial the
+// final sum reads is written by the same call.
+constexpr int kScratchDoubles = kReduceBlocks + 1;
+static __device__ __align__(256) double scratch[kScratchDoubles];
+constexpr int kMaxScratchDevices = 64;
+static std::mutex g_scratch_mu;
+static double* g_scratch_ptr[kMaxScratchDevices];
+
+int noalloc_scratch_init(int device) {
+    if (device < 0 || device >= kMaxScratchDevices)
+        return fail(TOMO_ERR_UNSUPPORTED, "device index beyond the allocation-free scratch table");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (g_scratch_ptr[device]) return TOMO_OK;
+    void* p = nullptr;
+    TOMO_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return fail(TOMO_ERR_HIP, "hipGetSymbolAddress returned no address for the reduction scratch");
+    g_scratch_ptr[device] = static_cast<double*>(p);
+    return TOMO_OK;
+}
+
+// The final sum copied to the host, the stream synchronised.
+static int copy_result(const double* d, double* out, hipStream_t s) {
+    double h = 0.0;
+    TOMO_HIP(hipMemcpyAsync(&h, d, sizeof(double), hipMemcpyDeviceToHost, s));
+    TOMO_HIP(hipStreamSynchronize(s));
+    *out = h;
+    return TOMO_OK;
+}
+
+int reduce_sq_norm_noalloc(int device, Buf x, size_t n, double* out, hipStream_t s) {
+    double* part = nullptr;
+    if (device >= 0 && device < kMaxScratchDevices) {
+        std::lock_guard<std::mutex> lk(g_scratch_mu);
+        part = g_scratch_ptr[device];
+    }
+    if (!part) return fail(TOMO_ERR_INVALID_ARG, "allocation-free scratch not set up on this device");
+    const unsigned blocks = reduce_blocks(n);
+    // blocks partials and the final sum; reduce_blocks keeps this within the array for every n.
+    if ((size_t)blocks + 1 > (size_t)kScratchDoubles)
+        return fail(TOMO_ERR_INVALID_ARG, "reduction needs more partial sums than the scratch holds");
+    int rc = launch_sq_partial(x, n, blocks, part, s);
+    if (rc) return rc;
+    hipLaunchKernelGGL(final_sum_kernel, dim3(1), dim3(kBlock), 0, s, part, (int)blocks, part + kReduceBlocks);
+    TOMO_HIP(hipGetLastError());
+    return copy_result(part + kReduceBlocks, out, s);
+}
+
 int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s) {
     (void)ctx;
     double* part = nullptr;

@@ -123,6 +123,11 @@ int launch_mul(float* x, const float* w, size_t n, hipStream_t s);
 int launch_waxpy(float* x, const float* c, const float* b, int positive, size_t n, hipStream_t s);
 int reduce_sq_norm(tomo_ctx* ctx, Buf x, size_t n, double* out, hipStream_t s);
 int reduce_dot(tomo_ctx* ctx, const float* a, const float* b, size_t n, double* out, hipStream_t s);
+// Allocation-free squared norm: the partial sums live in a device array of
+// the library, one per GPU (noalloc_scratch_init looks its address up once per
+// device, with that device current; tomo_ctx_create calls it).  Synchronous.
+int noalloc_scratch_init(int device);
+int reduce_sq_norm_noalloc(int device, Buf x, size_t n, double* out, hipStream_t s);
 // max |x| over n FP32 elements.  Synchronous.
 int reduce_max_abs(const float* x, size_t n, float* out, hipStream_t s);
 int launch_convert(Buf in, MutBuf out, size_t n, hipStream_t s);

@@ -35,7 +35,7 @@ EXPORTS = [
     "tomo_synchronize", "tomo_geom_create", "tomo_geom_desc_from_config", "tomo_geom_destroy",
     "tomo_geom_get_info", "tomo_geom_get_phi", "tomo_project", "tomo_backproject", "tomo_project_adjoint",
     "tomo_backproject_adjoint", "tomo_project_work", "tomo_convert", "tomo_diff", "tomo_scale", "tomo_axpy",
-    "tomo_sq_norm", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_wgd_run", "tomo_cg_run", "tomo_sirt_run", "tomo_phantom_shepp_logan",
+    "tomo_sq_norm", "tomo_sq_norm_noalloc", "tomo_dot", "tomo_laplacian", "tomo_gd_run", "tomo_wgd_run", "tomo_cg_run", "tomo_sirt_run", "tomo_phantom_shepp_logan",
     "tomo_phantom_ellipsoids", "tomo_file_read", "tomo_file_write", "tomo_serve_create", "tomo_serve_submit",
     "tomo_serve_poll", "tomo_serve_wait", "tomo_serve_iter_stats", "tomo_serve_get_stats", "tomo_serve_destroy",
 ]
@@ -144,6 +144,7 @@ def load() -> C.CDLL:
         "tomo_diff": ([vp, vp, vp, vp, sz, vp], i), "tomo_scale": ([vp, vp, f, sz, vp], i),
         "tomo_axpy": ([vp, vp, vp, d, i, sz, vp], i),
         "tomo_sq_norm": ([vp, vp, i, f, sz, C.POINTER(d), vp], i),
+        "tomo_sq_norm_noalloc": ([vp, vp, i, f, sz, C.POINTER(d), vp], i),
         "tomo_dot": ([vp, vp, vp, sz, C.POINTER(d), vp], i),
         "tomo_laplacian": ([vp, vp, vp, vp, vp], i),
         "tomo_gd_run": ([vp, vp, C.POINTER(GdParams), vp, vp, C.POINTER(GdIterStat), vp], i),
@@ -409,6 +410,15 @@ def sq_norm(ctx: Context, x, scale=1.0, stream=None) -> float:
     out = C.c_double()
     check(ctx.lib.tomo_sq_norm(ctx.h, C.c_void_p(_ptr(x)[0]), dtype_code(x), scale, x.numel(), C.byref(out),
                                C.c_void_p(_stream(stream))))
+    return out.value
+
+
+def sq_norm_noalloc(ctx: Context, x, scale=1.0, stream=None) -> float:
+    """tomo_sq_norm_noalloc: tomo_sq_norm with its partial sums in the
+    library's per-GPU scratch array (no device allocation)."""
+    out = C.c_double()
+    check(ctx.lib.tomo_sq_norm_noalloc(ctx.h, C.c_void_p(_ptr(x)[0]), dtype_code(x), scale, x.numel(),
+                                       C.byref(out), C.c_void_p(_stream(stream))))
     return out.value
 
 
