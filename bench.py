@@ -654,7 +654,7 @@ def main():
             rec["physical_hbm_frac"] = round(measured[key] / (ms * 1e-3) / HBM_PEAK, 4)
         if key in valu:
             rec["valu_issue_frac"] = valu[key]["issue_frac"]
-            rec["valu_busy"] = valu[key].get("valu_busy")
+            rec["texture_data_busy"] = valu[key].get("td_busy")
         ops[name] = rec
     ops["siddon"]["n_seg"] = n_seg
     ops["rs"]["n_samp"] = n_samp
