@@ -105,14 +105,17 @@ __device__ __forceinline__ StageBox stage_box(const WgCorners& w, float a0, floa
 // Copies planes [lo, lo + planes) x rows [zlo, zlo + nzr) x columns
 // [blo, blo + 63] into the buffer as rows (plane, z row) = pi nzr + zi,
 // kStagePitchP floats apart (planes * nzr <= kStageRows); wave wv of the
-// workgroup's 4 copies rows wv, wv + 4, ....  src (src_elems floats): rows
-// along B (A = 0: the y-fastest copy, element (y, x, z) at y + ny (x + nx z);
-// A = 1: the volume).  One buffer resource covers the window's rows from its
-// first row in the grid; a lane's byte offset is its row's plus its
-// column's, added with unsigned saturation, and a row or column off the grid
-// has an offset of 0xffffffff, past the resource's end, so it reads 0.
-// Every argument but lane is wave-uniform.  The caller waits for the copies
-// (s_waitcnt) and synchronises the workgroup before reading.
+// workgroup's 4 copies the rows zi = wv, wv + 4, ... of every plane.  src
+// (src_elems floats): rows along B (A = 0: the y-fastest copy, element
+// (y, x, z) at y + ny (x + nx z); A = 1: the volume).  One buffer resource
+// covers the window's rows from its first row in the grid; a lane's byte
+// offset is its row's plus its column's, added with unsigned saturation, and
+// a column off the grid has an offset of 0xffffffff, past the resource's end,
+// so it reads 0.  When every row of the window is in the grid (the usual
+// case) the row loop only steps two offsets; otherwise a row off the grid
+// also gets the offset 0xffffffff.  Every argument but lane is wave-uniform.
+// The caller waits for the copies (s_waitcnt) and synchronises the workgroup
+// before reading.
 __device__ __forceinline__ void stage_rows(const float* __restrict__ src, size_t src_elems, int nA, int nB, int nz,
                                            int lo, int planes, int zlo, int nzr, int blo, float* __restrict__ stage,
                                            int lane, int wv) {
@@ -123,26 +126,34 @@ __device__ __forceinline__ void stage_rows(const float* __restrict__ src, size_t
                                                   (uint32_t)(left < 0xfffffff0ull ? left : 0xfffffff0ull));
     const int col = blo + lane;
     const uint32_t coff = (unsigned)col < (unsigned)nB ? (uint32_t)col * 4u : 0xffffffffu;
-    // byte offset of row (zlo + zi, lo + pi) from the resource's base, stepped as rows advance
-    const int64_t rstep = (int64_t)nA * nB * 4;  // one z row
-    int pi = 0, zi = wv;
-    while (zi >= nzr && nzr > 0) {
-        zi -= nzr;
-        ++pi;
+    const int64_t rstep = (int64_t)nA * nB * 4;  // bytes per z row
+    const uint32_t step4 = (uint32_t)(4 * rstep);
+    const bool inside = zlo >= 0 && zlo + nzr <= nz && lo >= 0 && lo + planes <= nA;
+    if (inside) {
+        // rows (lo + pi, zlo + zi) at byte offset (zi nA + pi) nB 4 from the resource's base
+        for (int pi = 0; pi < planes; ++pi) {
+            uint32_t ro = (uint32_t)(((int64_t)wv * nA + pi) * nB * 4);
+            float* dst = stage + (pi * nzr + wv) * kStagePitchP;
+            for (int zi = wv; zi < nzr; zi += 4) {
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, (__attribute__((address_space(3))) void*)dst, 4,
+                                                         __builtin_elementwise_add_sat(ro, coff), 0, 0, 0);
+                ro += step4;
+                dst += 4 * kStagePitchP;
+            }
+        }
+        return;
     }
-    int64_t roff = ((int64_t)(zlo + zi - zc) * nA + (lo + pi - ac)) * (int64_t)nB * 4;
-    const int rows = planes * nzr;
-    for (int q = wv; q < rows; q += 4) {
-        const bool in = (unsigned)(lo + pi) < (unsigned)nA && (unsigned)(zlo + zi) < (unsigned)nz;
-        const uint32_t ro = in ? (uint32_t)roff : 0xffffffffu;
-        __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, (__attribute__((address_space(3))) void*)(stage + q * kStagePitchP),
-                                                 4, __builtin_elementwise_add_sat(ro, coff), 0, 0, 0);
-        zi += 4;
-        roff += 4 * rstep;
-        while (zi >= nzr) {
-            zi -= nzr;
-            ++pi;
-            roff += (int64_t)nB * 4 - (int64_t)nzr * rstep;
+    for (int pi = 0; pi < planes; ++pi) {
+        const int a = lo + pi;
+        int64_t roff = ((int64_t)(zlo + wv - zc) * nA + (a - ac)) * (int64_t)nB * 4;
+        float* dst = stage + (pi * nzr + wv) * kStagePitchP;
+        for (int zi = wv; zi < nzr; zi += 4) {
+            const bool in = (unsigned)a < (unsigned)nA && (unsigned)(zlo + zi) < (unsigned)nz;
+            const uint32_t ro = in ? (uint32_t)roff : 0xffffffffu;
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rr, (__attribute__((address_space(3))) void*)dst, 4,
+                                                     __builtin_elementwise_add_sat(ro, coff), 0, 0, 0);
+            roff += 4 * rstep;
+            dst += 4 * kStagePitchP;
         }
     }
 }
