@@ -356,8 +356,8 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const v
 // taps read from there (four ds_read2_b32 per sample).  The staging is double
 // buffered: the copy of window i + 1 is in flight while the workgroup samples
 // window i, and one barrier per window separates them.  A window holds up to
-// kRsWmax sample planes, fewer (2, then 1) when its box is too tall for a
-// buffer.  The box is not clipped to the grid: a sample outside the grid (a
+// kRsWmax sample planes, fewer (6, 4, 2, then 1) when its box is too large
+// for a buffer.  The box is not clipped to the grid: a sample outside the grid (a
 // cylinder wider than a y extent smaller than x, or a ray whose clipped exit
 // lies behind its entry) reads zero taps there and adds +0, as the range
 // test of the per-ray loop makes it.  Samples, taps, arithmetic and the order
@@ -365,7 +365,7 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const v
 // for bit.  A window whose box exceeds a buffer even at one plane, and a
 // workgroup whose rays do not share A and its direction, read their taps
 // from the volume directly.
-constexpr int kRsWmax = 4;  // sample planes per window at most
+constexpr int kRsWmax = 8;  // sample planes per window at most
 
 // The eight taps straight from the volume: the range and border rules and the
 // arithmetic of trilinear_quads.
@@ -390,23 +390,22 @@ __device__ __forceinline__ float trilinear_direct(const float* __restrict__ f, i
     return fmaf(fz, c1 - c0, c0);
 }
 
-// A window: samples with floor(A) in [wlo, whi]; staged planes wlo .. whi + 1,
-// nzr z rows from zlo, columns from blo.  All fields are wave-uniform.
+// A window: samples with floor(A) in [wlo, whi], staged planes wlo .. whi + 1
+// (s.lo = wlo).  All fields are wave-uniform.
 struct RsWindow {
-    int wlo, whi, blo, zlo, nzr;
-    bool fits;
+    int wlo, whi;
+    StageWin s;
 };
 
 // The window that starts at sample plane `from` in the travel direction: the
-// largest of kRsWmax, 2, 1 planes whose box fits a buffer.
-__device__ __forceinline__ RsWindow rs_plan(const WgCorners& wc, int from, bool inc) {
+// largest of kRsWmax, 6, 4, 2, 1 planes whose box fits a buffer.
+__device__ __forceinline__ RsWindow rs_plan(const WgCorners& wc, int from, bool inc, int max_nzr) {
     RsWindow w{};
-    for (int W = kRsWmax;; W >>= 1) {
+    for (int W = kRsWmax;; W = W > 4 ? W - 2 : W >> 1) {
         const int wlo = inc ? from : from - W + 1, whi = wlo + W - 1;
-        const StageBox b = stage_box(wc, (float)wlo, (float)(whi + 1), 0.0f, 0, 1, kStageRows);
-        const int nzr = b.zhi - b.zlo + 1;
-        w = RsWindow{wlo, whi, b.blo, b.zlo, nzr, b.fits && (W + 1) * nzr <= kStageRows};
-        if (w.fits || W == 1) break;
+        const StageBox b = stage_box(wc, (float)wlo, (float)(whi + 1), 0.0f, 0, 1);
+        w = RsWindow{wlo, whi, stage_window(b, wlo, W + 1, max_nzr)};
+        if (w.s.fits || W == 1) break;
     }
     return w;
 }
@@ -416,20 +415,28 @@ __device__ __forceinline__ RsWindow rs_plan(const WgCorners& wc, int from, bool 
 #endif
 
 // The window walk of one workgroup along A (0: x, 1: y); inc: A increasing.
-// src: rows along B (A = 0: the y-fastest copy, A = 1: the volume).  first,
-// last: the workgroup's first and last sample planes.  wc: the corner rays
-// (every wave plans the same windows).
+// src: rows of nBp floats along B (A = 0: the y-fastest copy, A = 1: the
+// volume or its x-padded copy).  first, last: the workgroup's first and last
+// sample planes.  wc: the corner rays (every wave plans the same windows).
 template <int A>
 __device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r, int lr, bool inc, int first, int last,
-                                                const float* __restrict__ src, const float* __restrict__ vol,
-                                                const WgCorners& wc, float* __restrict__ stage, int lane, int wv) {
+                                                const float* __restrict__ src, int nBp, const float* __restrict__ vol,
+                                                const WgCorners& wc, float* __restrict__ st0,
+                                                float* __restrict__ st1, int lane, int wv) {
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    const int nA = A == 0 ? nx : ny;
+    const size_t src_elems = (size_t)nBp * (size_t)nA * (size_t)nz;
     float acc = 0.0f;
     int s = 0;
-    RsWindow cur = rs_plan(wc, first, inc);
-    if (cur.fits) stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, cur.wlo, cur.whi - cur.wlo + 2, cur.zlo, cur.nzr, cur.blo, stage, lane, wv);
-    for (int b = 0;; b ^= 1) {
+    const int max_nzr = stage_max_rows(nA, nBp);
+    RsWindow cur = rs_plan(wc, first, inc, max_nzr);
+    if (cur.s.fits) stage_copy(src, src_elems, nA, nBp, nz, cur.s, st0, lane, wv);
+    // One window: window `cur` is in `buf`, the next one is copied into
+    // `other`; false after the last window.  The windows alternate between
+    // the two LDS arrays and each call names them directly, so the compiler
+    // knows that the copy in flight does not write what the reads read and
+    // does not wait for it before them.
+    auto window = [&](const float* __restrict__ buf, float* __restrict__ other) __attribute__((always_inline)) {
         // this wave's copies of window `cur` have landed; every wave is done with the other buffer
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -437,18 +444,17 @@ __device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r
         const bool more = inc ? from <= last : from >= last;
         RsWindow nxt = cur;
         if (more) {
-            nxt = rs_plan(wc, from, inc);
-            if (nxt.fits)
-                stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, nxt.wlo, nxt.whi - nxt.wlo + 2, nxt.zlo, nxt.nzr, nxt.blo,
-                           stage + (b ^ 1) * kStageBufF, lane, wv);
+            nxt = rs_plan(wc, from, inc, max_nzr);
+            if (nxt.s.fits) stage_copy(src, src_elems, nA, nBp, nz, nxt.s, other, lane, wv);
         }
-        const float* buf = stage + b * kStageBufF;
         const float wlof = (float)cur.wlo, whif = (float)cur.whi;
-        // offset of tap (a, z, b) in the buffer, ((a - wlo) nzr + z - zlo) pitch + b - blo, as one exact FP32
-        // expression of small integers
-        const int pstride = cur.nzr * kStagePitchP;
-        const float ps = (float)pstride;
-        const float c0 = -((float)cur.wlo * ps + (float)(cur.zlo * kStagePitchP + cur.blo));
+        // offset of tap (a, z, b) in the buffer, ((a - wlo) nzr + z - zlo) P + b - blo4 with P = 4 nch
+        // floats per row, as one exact FP32 expression of integers below 2^24 (the launcher admits
+        // grids with nA kSlotFloats < 2^24)
+        const int P = 4 * cur.s.nch;
+        const int pstride = cur.s.nzr * P;
+        const float ps = (float)pstride, pf = (float)P;
+        const float c0 = -((float)cur.wlo * ps + (float)(cur.s.zlo * P + cur.s.blo4));
         for (;;) {
             float px[RS_STAGED_UNROLL], py[RS_STAGED_UNROLL], pz[RS_STAGED_UNROLL];
             bool act[RS_STAGED_UNROLL];
@@ -463,7 +469,7 @@ __device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r
             }
             if (__ballot(act[0]) == 0ull) break;
             float v[RS_STAGED_UNROLL];
-            if (cur.fits) {
+            if (cur.s.fits) {
                 float fx[RS_STAGED_UNROLL], fy[RS_STAGED_UNROLL], fz[RS_STAGED_UNROLL];
                 int off[RS_STAGED_UNROLL];
 #pragma unroll
@@ -473,7 +479,7 @@ __device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r
                     fy[j] = py[j] - yf;
                     fz[j] = pz[j] - zf;
                     const float af = A == 0 ? xf : yf, bf = A == 0 ? yf : xf;
-                    const int o = (int)fmaf(af, ps, fmaf(zf, (float)kStagePitchP, bf + c0));
+                    const int o = (int)fmaf(af, ps, fmaf(zf, pf, bf + c0));
                     off[j] = act[j] ? o : 0;
                 }
                 float u[RS_STAGED_UNROLL][4], w[RS_STAGED_UNROLL][4];
@@ -483,13 +489,13 @@ __device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r
                     const float* q1 = q0 + pstride;
                     // (b, z), (b, z+1), (b+1, z), (b+1, z+1) on planes ia, ia + 1
                     u[j][0] = q0[0];
-                    u[j][1] = q0[kStagePitchP];
+                    u[j][1] = q0[P];
                     u[j][2] = q0[1];
-                    u[j][3] = q0[kStagePitchP + 1];
+                    u[j][3] = q0[P + 1];
                     w[j][0] = q1[0];
-                    w[j][1] = q1[kStagePitchP];
+                    w[j][1] = q1[P];
                     w[j][2] = q1[1];
-                    w[j][3] = q1[kStagePitchP + 1];
+                    w[j][3] = q1[P + 1];
                 }
 #pragma unroll
                 for (int j = 0; j < RS_STAGED_UNROLL; ++j) {
@@ -520,19 +526,25 @@ __device__ __forceinline__ float rs_staged_walk(const DevGeom& g, const RsRay& r
             }
             s += adv;
         }
-        if (!more) break;
         cur = nxt;
+        return more;
+    };
+    while (window(st0, st1) && window(st1, st0)) {
     }
     return acc;
 }
 
-// Forward RS with LDS-staged taps (FP32 volume `vol`, `volT` its y-fastest
-// copy).  Grid: x = column group, y = view, z = row group.
+// Forward RS with LDS-staged taps (FP32 volume `vol`).  volT: its y-fastest
+// copy with rows of nyp floats, volP: rows of nxp floats along x (the volume
+// itself when nxp = nx); nxp, nyp are nx, ny rounded up to multiples of four.
+// Grid: x = column group, y = view, z = row group.
 template <typename TS>
-__global__ __launch_bounds__(BX* BY, 4) void rs_staged_kernel(DevGeom g, int cube, const float* __restrict__ vol,
-                                                             const float* __restrict__ volT, TS* __restrict__ sino,
-                                                             float sino_scale) {
-    __shared__ float stage[2 * kStageBufF];
+__global__ __launch_bounds__(BX* BY, 5) void rs_staged_kernel(DevGeom g, int cube, const float* __restrict__ vol,
+                                                             const float* __restrict__ volT,
+                                                             const float* __restrict__ volP, int nxp, int nyp,
+                                                             TS* __restrict__ sino, float sino_scale) {
+    __shared__ float st0[kSlotFloats];
+    __shared__ float st1[kSlotFloats];
     __shared__ int wsh[4][3];
     const int tid = threadIdx.y * BX + threadIdx.x;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -604,9 +616,9 @@ __global__ __launch_bounds__(BX* BY, 4) void rs_staged_kernel(DevGeom g, int cub
         const int nlo = bz * BY, nhi = min(bz * BY + BY - 1, g.nv - 1);
         const WgCorners wc = wg_corners(g, A, p, mlo, mhi, nlo, nhi);
         if (A == 0)
-            acc = rs_staged_walk<0>(g, r, ok ? r.lr : 0, inc, first, last, volT, vol, wc, stage, lane, wv);
+            acc = rs_staged_walk<0>(g, r, ok ? r.lr : 0, inc, first, last, volT, nyp, vol, wc, st0, st1, lane, wv);
         else
-            acc = rs_staged_walk<1>(g, r, ok ? r.lr : 0, inc, first, last, vol, vol, wc, stage, lane, wv);
+            acc = rs_staged_walk<1>(g, r, ok ? r.lr : 0, inc, first, last, volP, nxp, vol, wc, st0, st1, lane, wv);
     }
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
@@ -688,19 +700,19 @@ __global__ __launch_bounds__(BX* BY) void rs_count_kernel(DevGeom g, int cube, u
 int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s, const ProjExtra& x) {
     const DevGeom& d = g->dev;
     const dim3 grid(ceil_div(d.nu, BX), d.nphi, ceil_div(d.nv, BY));
-    if (x.staged && vol.dtype == TOMO_F32 && !x.index_reads) {
-        // LDS-staged taps; the y-fastest copy of the volume for the x slabs
-        float* vt = nullptr;
-        TOMO_HIP(hipMallocAsync((void**)&vt, g->nvox() * sizeof(float), s));
-        hipError_t le = launch_xy_transpose(vol.p, vt, 4, d.nx, d.ny, d.nz, s);
+    // LDS-staged taps: FP32 volumes whose tap offsets stay exact in FP32
+    // ((max(nx, ny) + 16) kSlotFloats < 2^24)
+    if (x.staged && vol.dtype == TOMO_F32 && !x.index_reads && std::max(d.nx, d.ny) <= 4096) {
+        StageSources src;
+        hipError_t le = src.make((const float*)vol.p, d.nx, d.ny, d.nz, s);
         if (le == hipSuccess) {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 hipLaunchKernelGGL((rs_staged_kernel<TS>), grid, dim3(BX, BY), 0, s, d, cube, (const float*)vol.p,
-                                   (const float*)vt, (TS*)sino.p, sino.scale);
+                                   (const float*)src.vt, src.rows_x, src.nxp, src.nyp, (TS*)sino.p, sino.scale);
             });
             le = hipGetLastError();
         }
-        const hipError_t fe = hipFreeAsync(vt, s);
+        const hipError_t fe = src.release(s);
         TOMO_HIP(le);
         TOMO_HIP(fe);
         return TOMO_OK;

@@ -470,7 +470,7 @@ __device__ __forceinline__ void siddon_tile(const DevGeom& g, const TV* __restri
 // the barriers, then steps per lane; a window whose box exceeds the buffer
 // reads the quads from the volume; a workgroup whose rays do not share A and
 // its direction steps per lane on the volume.
-constexpr int kSidWmax = 4;  // slabs per window at most
+constexpr int kSidWmax = 8;  // slabs per window at most
 
 // Per-lane general traversal on the volume (x fastest), as sid_step.
 template <typename RD>
@@ -507,23 +507,22 @@ __device__ __forceinline__ float4 sid_quad_direct(const float* __restrict__ vol,
     return make_float4(at(bc, zc), at(bc, zc + 1), at(bc + 1, zc), at(bc + 1, zc + 1));
 }
 
-// A window of slabs [wlo, whi] staged as planes wlo .. whi, nzr z rows from
-// zlo, columns from blo.  All fields are wave-uniform.
+// A window of slabs [wlo, whi] staged as planes wlo .. whi (s.lo = wlo).
+// All fields are wave-uniform.
 struct SidWindow {
-    int wlo, whi, blo, zlo, nzr;
-    bool fits;
+    int wlo, whi;
+    StageWin s;
 };
 
 // The window that starts at slab `from` in the walk's direction: the largest
-// of kSidWmax, 2, 1 slabs whose box fits a buffer.
-__device__ __forceinline__ SidWindow sid_plan(const WgCorners& wc, int from, int sAw) {
+// of kSidWmax, 4, 2, 1 slabs whose box fits a buffer.
+__device__ __forceinline__ SidWindow sid_plan(const WgCorners& wc, int from, int sAw, int max_nzr) {
     SidWindow w{};
     for (int W = kSidWmax;; W >>= 1) {
         const int wlo = sAw > 0 ? from : from - W + 1, whi = wlo + W - 1;
-        const StageBox b = stage_box(wc, (float)wlo - 0.5f, (float)whi + 0.5f, 0.5f, 1, 1, kStageRows);
-        const int nzr = b.zhi - b.zlo + 1;
-        w = SidWindow{wlo, whi, b.blo, b.zlo, nzr, b.fits && W * nzr <= kStageRows};
-        if (w.fits || W == 1) break;
+        const StageBox b = stage_box(wc, (float)wlo - 0.5f, (float)whi + 0.5f, 0.5f, 1, 1);
+        w = SidWindow{wlo, whi, stage_window(b, wlo, W, max_nzr)};
+        if (w.s.fits || W == 1) break;
     }
     return w;
 }
@@ -536,12 +535,14 @@ __device__ __forceinline__ SidWindow sid_plan(const WgCorners& wc, int from, int
 // per window.
 template <int A, bool ZNEG, int BNEG>
 __device__ __forceinline__ float sid_walk_staged(const DevGeom& g, const SidRay& r, bool live, bool walks, int I,
-                                                 int sAw, int trips, const float* __restrict__ src,
+                                                 int sAw, int trips, const float* __restrict__ src, int nBp,
                                                  const float* __restrict__ vol, const WgCorners& wc,
-                                                 float* __restrict__ stage, int lane, int wv) {
+                                                 float* __restrict__ st0, float* __restrict__ st1, int lane, int wv) {
     constexpr int B = 1 - A;
     const int nx = g.nx, ny = g.ny, nz = g.nz;
-    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    const int nA = A == 0 ? nx : ny;
+    const size_t src_elems = (size_t)nBp * (size_t)nA * (size_t)nz;
+    const int max_nzr = stage_max_rows(nA, nBp);
     const int sB = r.step[B];
     const bool bneg = BNEG == 2 ? sB < 0 : BNEG == 1;
     float tA = r.first[A], tB = r.first[B], tz = r.first[2];
@@ -549,10 +550,11 @@ __device__ __forceinline__ float sid_walk_staged(const DevGeom& g, const SidRay&
     float t = 0.0f, acc = 0.0f;
     const int lag = live ? abs(r.idx[A] - I) : 0;
     const int b0 = bneg ? r.idx[B] - 1 : r.idx[B], z0 = ZNEG ? r.idx[2] - 1 : r.idx[2];
-    // the quad corner's (z, B) part of its offset in a staged plane, z kStagePitchP + B; the plane
-    // and the window's origin are wave-uniform
-    int ez = live ? z0 * kStagePitchP + b0 : 0;
-    const int dB = sB, dZ = r.step[2] * kStagePitchP;
+    // the quad corner's (z, B) part of its offset in the window's buffer, (z - zlo) P + B - blo4
+    // with P floats per staged row (set per window); the plane's part is wave-uniform
+    int ez = 0;
+    const int dB = sB;
+    int dZ = 0;
     auto body = [&](const float4 v) {
         // v = (b, z), (b, z+1), (b+1, z), (b+1, z+1) of the block's corner
         const float V0 = bneg ? (ZNEG ? v.w : v.z) : (ZNEG ? v.y : v.x);
@@ -578,9 +580,14 @@ __device__ __forceinline__ float sid_walk_staged(const DevGeom& g, const SidRay&
         live = t < r.tmax;
     };
     const int last = I + sAw * (trips - 1);  // the slab of the last iteration
-    SidWindow cur = sid_plan(wc, I, sAw);
-    if (cur.fits) stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, cur.wlo, cur.whi - cur.wlo + 1, cur.zlo, cur.nzr, cur.blo, stage, lane, wv);
-    for (int b = 0;; b ^= 1) {
+    SidWindow cur = sid_plan(wc, I, sAw, max_nzr);
+    if (cur.s.fits) stage_copy(src, src_elems, nA, nBp, nz, cur.s, st0, lane, wv);
+    // One window: window `cur` is in `stage`, the next one is copied into
+    // `other`; false after the last window.  The windows alternate between
+    // the two LDS arrays and each call names them directly, so the compiler
+    // knows that the copy in flight does not write what the reads read and
+    // does not wait for it before them.
+    auto window = [&](const float* __restrict__ stage, float* __restrict__ other) __attribute__((always_inline)) {
         // this wave's copies of window `cur` have landed; every wave is done with the other buffer
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
@@ -588,23 +595,24 @@ __device__ __forceinline__ float sid_walk_staged(const DevGeom& g, const SidRay&
         const bool more = sAw > 0 ? from <= last : from >= last;
         SidWindow nxt = cur;
         if (more) {
-            nxt = sid_plan(wc, from, sAw);
-            if (nxt.fits)
-                stage_rows(src, (size_t)nx * ny * nz, nA, nB, nz, nxt.wlo, nxt.whi - nxt.wlo + 1, nxt.zlo, nxt.nzr, nxt.blo,
-                           stage + (b ^ 1) * kStageBufF, lane, wv);
+            nxt = sid_plan(wc, from, sAw, max_nzr);
+            if (nxt.s.fits) stage_copy(src, src_elems, nA, nBp, nz, nxt.s, other, lane, wv);
         }
         if (walks) {
             // iterations of the window's slabs, in the walk's order
             const int k0 = sAw > 0 ? cur.wlo - I : I - cur.whi;
             const int k1 = min(trips, k0 + (cur.whi - cur.wlo + 1));
-            if (cur.fits) {
-                const int ps = cur.nzr * kStagePitchP;
-                const float* buf = stage + b * kStageBufF - (cur.zlo * kStagePitchP + cur.blo);
+            if (cur.s.fits) {
+                const int P = 4 * cur.s.nch, ps = cur.s.nzr * P;
+                dZ = r.step[2] * P;
+                // the current quad corner, from the crossings taken so far
+                const int bc = b0 + sB * (int)cB, zc = z0 + r.step[2] * (int)cz;
+                ez = (zc - cur.s.zlo) * P + (bc - cur.s.blo4);
                 for (int outer = k0; outer < k1; ++outer) {
                     if (live && outer >= lag) {
                         const int slab = I + sAw * outer;
-                        const float* q = buf + (slab - cur.wlo) * ps + ez;
-                        body(make_float4(q[0], q[kStagePitchP], q[1], q[kStagePitchP + 1]));
+                        const float* q = stage + (slab - cur.wlo) * ps + ez;
+                        body(make_float4(q[0], q[P], q[1], q[P + 1]));
                     }
                 }
             } else {
@@ -617,31 +625,39 @@ __device__ __forceinline__ float sid_walk_staged(const DevGeom& g, const SidRay&
                 }
             }
         }
-        if (!more) break;
         cur = nxt;
+        return more;
+    };
+    while (window(st0, st1) && window(st1, st0)) {
     }
     return acc;
 }
 
 template <int A, bool ZNEG>
 __device__ __forceinline__ float sid_walk_staged_b(const DevGeom& g, const SidRay& r, bool live, bool walks, int I,
-                                                   int sAw, int trips, const float* __restrict__ src,
+                                                   int sAw, int trips, const float* __restrict__ src, int nBp,
                                                    const float* __restrict__ vol, const WgCorners& wc,
-                                                   float* __restrict__ stage, int lane, int wv) {
+                                                   float* __restrict__ st0, float* __restrict__ st1, int lane,
+                                                   int wv) {
     const bool pos_all = __ballot(live && r.step[1 - A] < 0) == 0ull;
     const bool neg_all = __ballot(live && r.step[1 - A] >= 0) == 0ull;
-    if (pos_all) return sid_walk_staged<A, ZNEG, 0>(g, r, live, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
-    if (neg_all) return sid_walk_staged<A, ZNEG, 1>(g, r, live, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
-    return sid_walk_staged<A, ZNEG, 2>(g, r, live, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+    if (pos_all) return sid_walk_staged<A, ZNEG, 0>(g, r, live, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv);
+    if (neg_all) return sid_walk_staged<A, ZNEG, 1>(g, r, live, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv);
+    return sid_walk_staged<A, ZNEG, 2>(g, r, live, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv);
 }
 
-// Forward clean Siddon with LDS-staged quads (FP32 volume `vol`, `volT` its
-// y-fastest copy).  Grid: x = column group, y = view, z = row group.
+// Forward clean Siddon with LDS-staged quads (FP32 volume `vol`).  volT: its
+// y-fastest copy with rows of nyp floats, volP: rows of nxp floats along x
+// (the volume itself when nxp = nx); nxp, nyp are nx, ny rounded up to
+// multiples of four.  Grid: x = column group, y = view, z = row group.
 template <typename TS>
-__global__ __launch_bounds__(BX* BY, 4) void siddon_staged_kernel(DevGeom g, const float* __restrict__ vol,
-                                                                 const float* __restrict__ volT, unsigned vol_bytes,
-                                                                 TS* __restrict__ sino, float sino_scale) {
-    __shared__ float stage[2 * kStageBufF];
+__global__ __launch_bounds__(BX* BY, 5) void siddon_staged_kernel(DevGeom g, const float* __restrict__ vol,
+                                                                 const float* __restrict__ volT,
+                                                                 const float* __restrict__ volP, int nxp, int nyp,
+                                                                 unsigned vol_bytes, TS* __restrict__ sino,
+                                                                 float sino_scale) {
+    __shared__ float st0[kSlotFloats];
+    __shared__ float st1[kSlotFloats];
     __shared__ int wsh[4][3];
     const int tid = threadIdx.y * BX + threadIdx.x;
     const int lane = tid & 63, wv = __builtin_amdgcn_readfirstlane(tid >> 6);
@@ -721,14 +737,15 @@ __global__ __launch_bounds__(BX* BY, 4) void siddon_staged_kernel(DevGeom g, con
         const int mlo = bx * BX, mhi = min(bx * BX + BX - 1, g.nu - 1);
         const int nlo = bz * BY, nhi = min(bz * BY + BY - 1, g.nv - 1);
         const WgCorners wc = wg_corners(g, A, p, mlo, mhi, nlo, nhi);
-        const float* src = A == 0 ? volT : vol;
+        const float* src = A == 0 ? volT : volP;
+        const int nBp = A == 0 ? nyp : nxp;
         const bool zneg = zneg_all && !zpos_all;
         if (A == 0) {
-            acc = zneg ? sid_walk_staged_b<0, true>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv)
-                       : sid_walk_staged_b<0, false>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+            acc = zneg ? sid_walk_staged_b<0, true>(g, r, ok, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv)
+                       : sid_walk_staged_b<0, false>(g, r, ok, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv);
         } else {
-            acc = zneg ? sid_walk_staged_b<1, true>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv)
-                       : sid_walk_staged_b<1, false>(g, r, ok, walks, I, sAw, trips, src, vol, wc, stage, lane, wv);
+            acc = zneg ? sid_walk_staged_b<1, true>(g, r, ok, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv)
+                       : sid_walk_staged_b<1, false>(g, r, ok, walks, I, sAw, trips, src, nBp, vol, wc, st0, st1, lane, wv);
         }
         if (!walks) {
             if (vol_bytes != 0u)
@@ -902,22 +919,23 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
         return TOMO_OK;
     }
     if (x.staged && vol.dtype == TOMO_F32 && !x.index_reads) {
-        // LDS-staged quads; the y-fastest copy of the volume for the x slabs
+        // LDS-staged quads, from copies of the volume with rows along y (x
+        // slabs) and, when nx is not a multiple of four, along x (y slabs)
         if (g->nvox() >= (1ull << 32))
             return fail(TOMO_ERR_UNSUPPORTED, "volume too large for the 32-bit traversal index of the Siddon projector");
-        float* vt = nullptr;
-        TOMO_HIP(hipMallocAsync((void**)&vt, g->nvox() * sizeof(float), s));
-        hipError_t le = launch_xy_transpose(vol.p, vt, 4, d.nx, d.ny, d.nz, s);
+        StageSources src;
+        hipError_t le = src.make((const float*)vol.p, d.nx, d.ny, d.nz, s);
         if (le == hipSuccess) {
             const uint64_t vb = (uint64_t)g->nvox() * 4ull;
             const unsigned vol_bytes = vb < (1ull << 32) ? (unsigned)vb : 0u;  // 0: index reads
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 hipLaunchKernelGGL((siddon_staged_kernel<TS>), grid, dim3(BX, BY), 0, s, d, (const float*)vol.p,
-                                   (const float*)vt, vol_bytes, (TS*)sino.p, sino.scale);
+                                   (const float*)src.vt, src.rows_x, src.nxp, src.nyp, vol_bytes, (TS*)sino.p,
+                                   sino.scale);
             });
             le = hipGetLastError();
         }
-        const hipError_t fe = hipFreeAsync(vt, s);
+        const hipError_t fe = src.release(s);
         TOMO_HIP(le);
         TOMO_HIP(fe);
         return TOMO_OK;

@@ -266,23 +266,33 @@ __global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, 
     zpair_y_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
-// out[y + ny (x + nx z)] = in[x + nx (y + ny z)]: the volume with y fastest,
-// one 32 x 32 (x, y) tile of one z plane per workgroup, through LDS.
-template <typename T>
-__global__ __launch_bounds__(256) void xy_transpose_kernel(const T* __restrict__ in, T* __restrict__ out, int nx,
-                                                         int ny) {
-    __shared__ T tile[32][33];
+// out[y + nyp (x + nx z)] = in[x + nx (y + ny z)], 0 for ny <= y < nyp: the
+// volume with y fastest and zero-padded rows, one 32 x 32 (x, y) tile of one
+// z plane per workgroup, through LDS.
+__global__ __launch_bounds__(256) void xy_transpose_kernel(const float* __restrict__ in, float* __restrict__ out,
+                                                         int nx, int ny, int nyp) {
+    __shared__ float tile[32][33];
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;
     const int x0 = blockIdx.x * 32, y0 = blockIdx.y * 32;
-    const size_t plane = (size_t)nx * ny * blockIdx.z;
+    const size_t plane_in = (size_t)nx * ny * blockIdx.z, plane_out = (size_t)nx * nyp * blockIdx.z;
     for (int r = ty; r < 32; r += 8) {
         const int x = x0 + tx, y = y0 + r;
-        if (x < nx && y < ny) tile[r][tx] = in[plane + (size_t)y * nx + x];
+        tile[r][tx] = (x < nx && y < ny) ? in[plane_in + (size_t)y * nx + x] : 0.0f;
     }
     __syncthreads();
     for (int r = ty; r < 32; r += 8) {
         const int y = y0 + tx, x = x0 + r;
-        if (x < nx && y < ny) out[plane + (size_t)x * ny + y] = tile[tx][r];
+        if (x < nx && y < nyp) out[plane_out + (size_t)x * nyp + y] = tile[tx][r];
+    }
+}
+
+// out[x + nxp r] = in[x + nx r], 0 for nx <= x < nxp.
+__global__ __launch_bounds__(256) void row_pad_kernel(const float* __restrict__ in, float* __restrict__ out, int nx,
+                                                    int nxp, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256 + threadIdx.x; i < n; i += (size_t)gridDim.x * 256) {
+        const size_t r = i / (size_t)nxp;
+        const int x = (int)(i - r * (size_t)nxp);
+        out[i] = x < nx ? in[r * (size_t)nx + x] : 0.0f;
     }
 }
 
@@ -523,14 +533,15 @@ hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, 
     return hipGetLastError();
 }
 
-hipError_t launch_xy_transpose(const void* vol, void* out, size_t es, int nx, int ny, int nz, hipStream_t s) {
-    const dim3 grid(ceil_div(nx, 32), ceil_div(ny, 32), nz);
-    if (es == 4)
-        hipLaunchKernelGGL((xy_transpose_kernel<uint32_t>), grid, dim3(256), 0, s, (const uint32_t*)vol,
-                           (uint32_t*)out, nx, ny);
-    else
-        hipLaunchKernelGGL((xy_transpose_kernel<uint16_t>), grid, dim3(256), 0, s, (const uint16_t*)vol,
-                           (uint16_t*)out, nx, ny);
+hipError_t launch_xy_transpose(const float* vol, float* out, int nx, int ny, int nyp, int nz, hipStream_t s) {
+    const dim3 grid(ceil_div(nx, 32), ceil_div(nyp, 32), nz);
+    hipLaunchKernelGGL(xy_transpose_kernel, grid, dim3(256), 0, s, vol, out, nx, ny, nyp);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_pad(const float* vol, float* out, int nx, int nxp, size_t rows, hipStream_t s) {
+    const size_t n = rows * (size_t)nxp;
+    hipLaunchKernelGGL(row_pad_kernel, dim3(grid_for(n)), dim3(256), 0, s, vol, out, nx, nxp, n);
     return hipGetLastError();
 }
 

@@ -23,10 +23,46 @@ __device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int
 // Fills qx and qy from an x-fastest volume of es-byte elements (es 4 or 2).
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s);
 
-// out[y + ny (x + nx z)] = vol[x + nx (y + ny z)] for es-byte elements (es 4
-// or 2): the volume with y fastest, read by the LDS-staged RS projector for
-// slabs along x.
-hipError_t launch_xy_transpose(const void* vol, void* out, size_t es, int nx, int ny, int nz, hipStream_t s);
+// out[y + nyp (x + nx z)] = vol[x + nx (y + ny z)] for y < ny and 0 for
+// ny <= y < nyp (nyp >= ny), FP32: the volume with y fastest and rows padded
+// with zeros, read by the LDS-staged projectors for slabs along x.
+hipError_t launch_xy_transpose(const float* vol, float* out, int nx, int ny, int nyp, int nz, hipStream_t s);
+
+// out[x + nxp r] = vol[x + nx r] for x < nx and 0 for nx <= x < nxp, r over
+// the ny nz rows, FP32: rows along x padded with zeros.
+hipError_t launch_row_pad(const float* vol, float* out, int nx, int nxp, size_t rows, hipStream_t s);
+
+// The rows read by the LDS-staged projectors for the duration of one call:
+// along y (vt, rows of nyp floats) for slabs along x, along x (the volume
+// itself, or vp when nx is not a multiple of four) for slabs along y.  nxp,
+// nyp: nx, ny rounded up to multiples of four.  make() then release() on the
+// same stream; the copies come from the stream-ordered pool.
+struct StageSources {
+    float* vt = nullptr;
+    float* vp = nullptr;
+    const float* rows_x = nullptr;
+    int nxp = 0, nyp = 0;
+    hipError_t make(const float* vol, int nx, int ny, int nz, hipStream_t s) {
+        nxp = (nx + 3) & ~3;
+        nyp = (ny + 3) & ~3;
+        hipError_t e = hipMallocAsync((void**)&vt, (size_t)nyp * nx * nz * sizeof(float), s);
+        if (e == hipSuccess) e = launch_xy_transpose(vol, vt, nx, ny, nyp, nz, s);
+        rows_x = vol;
+        if (e == hipSuccess && nxp != nx) {
+            e = hipMallocAsync((void**)&vp, (size_t)nxp * ny * nz * sizeof(float), s);
+            if (e == hipSuccess) e = launch_row_pad(vol, vp, nx, nxp, (size_t)ny * nz, s);
+            rows_x = vp;
+        }
+        return e;
+    }
+    hipError_t release(hipStream_t s) {
+        hipError_t e = hipSuccess, e2 = hipSuccess;
+        if (vp) e = hipFreeAsync(vp, s);
+        if (vt) e2 = hipFreeAsync(vt, s);
+        vp = vt = nullptr;
+        return e != hipSuccess ? e : e2;
+    }
+};
 
 // Both copies for the duration of one operator call, filled from the volume:
 // alloc() then release() on the same stream.  The copies come from the
