@@ -64,26 +64,6 @@ __device__ __forceinline__ ColumnView column_view(const DevGeom& g, const double
     return r;
 }
 
-// The same terms in FP32 (u and A only), for the FP16 u-pair form without
-// the FDK weight: its u-weights are rounded to multiples of 1/2048 pixel,
-// two orders of magnitude above the FP32 error of u (below 1e-4 pixel at
-// 1024^3) and of v.  The reciprocal is v_rcp_f32 (1 ulp).
-__device__ __forceinline__ ColumnView column_view_f32(const DevGeom& g, const double* c, float xp, float yp) {
-    const float w = fmaf((float)c[2], xp, fmaf((float)c[3], yp, (float)g.fod));
-    const float inv = __builtin_amdgcn_rcpf(w);
-    const float un = fmaf((float)c[4], xp, (float)c[5] * yp);
-    ColumnView r;
-    r.u = fmaf(un, inv, g.cu);
-    r.A = (float)c[6] * inv;
-    r.num = 0.0f;
-    r.base = 1.0f;
-    if (!(__builtin_isfinite(r.u) && __builtin_isfinite(r.A))) {
-        r.u = -8.0f;
-        r.A = 0.0f;
-    }
-    return r;
-}
-
 // ---------------------------------------------------------------------------
 // LDS-staged VI for FP32 sinograms.
 //
@@ -100,10 +80,8 @@ __device__ __forceinline__ ColumnView column_view_f32(const DevGeom& g, const do
 // the row extremes are the floors at the four (z', A) corners, so the box is
 // exact.  The rows are not clamped as in vi_tile: a row off the detector
 // holds zeros in the box, as the clamped cell it stands for.  View p is
-// staged in buffer p & 1 (two LDS arrays), so one barrier per view separates
-// the copy from the reads, and the copy of view p + 1 is in flight while the
-// workgroup reads view p.  A view whose box exceeds the buffer reads the taps
-// directly.
+// staged in buffer p & 1, so one barrier per view separates the copy from
+// the reads.  A view whose box exceeds the buffer reads the taps directly.
 constexpr int SX = kViStageX, SY = kViStageY;  // voxel columns per workgroup; a wave is 8 x 8 columns
 // Floats per staged row.  A wave's 64 lanes read taps (u, r) at word
 // r * pitch + u; lanes spread over about 20 columns and a few rows, so two
@@ -266,20 +244,13 @@ __device__ __forceinline__ void pair_sample4(const uint32_t (&p0)[4], const uint
         : "v"(p0[0]), "v"(p0[1]), "v"(p0[2]), "v"(p0[3]), "v"(p1[0]), "v"(p1[1]), "v"(p1[2]), "v"(p1[3]), "v"(w));
 }
 
-// floor(v) as an int in one instruction (v finite).
-__device__ __forceinline__ int floor_to_int(float v) {
-    int r;
-    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(v));
-    return r;
-}
-
 // PAIR: `sino` holds u-pair words (vi_pair_kernel; nu + 1 per row) instead of
 // FP32 taps; the box copy and the tap offsets are the same, one word per row.
 template <typename TV, bool FDK, int ZS, bool PAIR = false>
 __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __restrict__ sino,
                                                const ViBox* __restrict__ boxes, TV* __restrict__ vol,
                                                float vol_scale, float post, int bx, int by, int bz,
-                                               float* __restrict__ st0, float* __restrict__ st1) {
+                                               float* __restrict__ stage) {
     using SS = StageShape<ZS>;
     constexpr int NT = SX * SY;
     constexpr int NW = NT / 64;                             // waves per workgroup
@@ -330,12 +301,12 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     // bytes for a row off the detector), so a tap off the detector reads past
     // the resource's end, which writes 0; the lane offsets are the same for
     // every row.
-    auto copy_box = [&](const StagePlan& s, int p, float* buf) __attribute__((always_inline)) {
+    auto copy_box = [&](const StagePlan& s, int p, int b) {
         if (!s.fits) return;
         const float* view = sino + (size_t)p * plane;
         // past the row's end unless 0 <= u < nu (FP32) or the pair index u + 1 <= nu
         const uint32_t loff = (uint32_t)(s.u0 + (PAIR ? 1 : 0) + lane) * 4u;
-        float* dst = buf + wv * kStagePitch;
+        float* dst = stage + b * SS::taps + wv * kStagePitch;
 #pragma unroll
         for (int q = 0; q < kSlots; ++q) {
             if (wv + q * NW < s.rows) {
@@ -352,28 +323,22 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
     const uint32_t view_bytes = (uint32_t)(plane * sizeof(float));
     auto box_of = [&](int p) { return ViBox{bt[p].u0, bt[p].u1, bt[p].amin, bt[p].amax}; };
     StagePlan nxt = stage_plan<ZS>(box_of(0), zp0, cv, lo_amax, hi_amax);
-    // View p is staged in st0 for even p and st1 for odd p.  `stage` (the
-    // buffer read) and `other` (the one the next copy goes to) are always the
-    // two arrays themselves, never a pointer chosen at run time: the compiler
-    // then knows that the copy in flight does not write what the reads read
-    // and does not wait for it before them.
-    auto view = [&](int p, const float* __restrict__ stage, float* __restrict__ other)
-                    __attribute__((always_inline)) {
+    copy_box(nxt, 0, 0);
+    for (int p = 0; p < g.nphi; ++p) {
         // this wave's copies of view p have landed; every wave is done with view p - 1
         __builtin_amdgcn_s_waitcnt(0);
         __syncthreads();
         const StagePlan now = nxt;
         if (p + 1 < g.nphi) {
             nxt = stage_plan<ZS>(box_of(p + 1), zp0, cv, lo_amax, hi_amax);
-            copy_box(nxt, p + 1, other);
+            copy_box(nxt, p + 1, (p + 1) & 1);
         }
         if (active) {
             const ConstView vt = (ConstView)(g.view + (size_t)p * kViewStride);
             double c[kViewStride];
 #pragma unroll
             for (int e = 0; e < kViewStride; ++e) c[e] = vt[e];
-            const ColumnView cvw = (PAIR && !FDK) ? column_view_f32(g, c, (float)xp, (float)yp)
-                                                  : column_view<FDK>(g, c, xp, yp);
+            const ColumnView cvw = column_view<FDK>(g, c, xp, yp);
             const float uf = floorf(cvw.u);
             const float fu = cvw.u - uf;
             const float uc = __builtin_amdgcn_fmed3f(uf, -2.0f, fnu);
@@ -381,43 +346,27 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
             if (now.fits) {
                 // byte offset in the buffer of tap (uc, vf): ((vf - v0) pitch + uc - u0) 4,
                 // one exact fma of small integers
-                const float colb = (float)(((int)uc - now.u0 - now.v0 * kStagePitch) * 4);
+                const float colb =
+                    (float)(((int)uc - now.u0 - now.v0 * kStagePitch + (p & 1) * SS::taps) * 4);
                 if constexpr (PAIR) {
-                    static_assert(ZS % 8 == 0, "u-pair samples go four voxels at a time, two groups in flight");
-                    // Group of four voxels from k4: their u-pair words (rows v, v + 1) and
-                    // fractions.  The next group's words are read before this group's dot
-                    // products, so the LDS latency overlaps them.  fv = fract(v) and the row
-                    // from an integer multiply-add (v - floor(v) rounds to 1 for v just below
-                    // 0 where fract stays below 1; the u-pair form is not bit-exact anyway).
-                    const int colb_i = ((int)uc - now.u0 - now.v0 * kStagePitch) * 4;
-                    auto fetch = [&](int k4, uint32_t (&p0)[4], uint32_t (&p1)[4], float (&fv)[4]) {
+                    static_assert(ZS % 4 == 0, "u-pair samples go four voxels at a time");
+#pragma unroll
+                    for (int k4 = 0; k4 < ZS; k4 += 4) {
+                        uint32_t p0[4], p1[4];
+                        float fv[4], r0[4], dr[4];
 #pragma unroll
                         for (int t = 0; t < 4; ++t) {
                             const float v = fmaf(zp0 + (float)(k4 + t), cvw.A, cv);
-                            const int iv = floor_to_int(v);
-                            fv[t] = __builtin_amdgcn_fractf(v);
-                            const uint32_t off = (uint32_t)(__mul24(iv, kStagePitch * 4) + colb_i);
+                            const float vf = floorf(v);
+                            const uint32_t off = (uint32_t)fmaf(vf, (float)(kStagePitch * 4), colb);
                             const uint32_t* qw = (const uint32_t*)((const char*)stage + off);
                             p0[t] = qw[0];
                             p1[t] = qw[kStagePitch];
+                            fv[t] = v - vf;
                         }
-                    };
-                    auto sample = [&](int k4, const uint32_t (&p0)[4], const uint32_t (&p1)[4],
-                                      const float (&fv)[4]) {
-                        float r0[4], dr[4];
                         pair_sample4(p0, p1, pw, r0, dr);
 #pragma unroll
                         for (int t = 0; t < 4; ++t) add(cvw, k4 + t, fmaf(fv[t], dr[t], r0[t]));
-                    };
-                    uint32_t a0[4], a1[4], b0[4], b1[4];
-                    float fa[4], fb[4];
-                    fetch(0, a0, a1, fa);
-#pragma unroll
-                    for (int k4 = 0; k4 < ZS; k4 += 8) {
-                        fetch(k4 + 4, b0, b1, fb);
-                        sample(k4, a0, a1, fa);
-                        if (k4 + 8 < ZS) fetch(k4 + 8, a0, a1, fa);
-                        sample(k4 + 4, b0, b1, fb);
                     }
                 } else {
 #pragma unroll
@@ -459,11 +408,6 @@ __device__ __forceinline__ void vi_staged_tile(const DevGeom& g, const float* __
                 }
             }
         }
-    };
-    copy_box(nxt, 0, st0);
-    for (int p = 0; p < g.nphi; p += 2) {
-        view(p, st0, st1);
-        if (p + 1 < g.nphi) view(p + 1, st1, st0);
     }
     if (!active) return;
     const size_t sxy = (size_t)g.nx * g.ny;
@@ -480,10 +424,9 @@ template <typename TV, bool FDK, int ZS, bool PAIR>
 __global__ __launch_bounds__(SX* SY, FDK ? 6 : 7) void vi_staged_kernel(DevGeom g, const float* __restrict__ sino,
                                                          const ViBox* __restrict__ boxes, TV* __restrict__ vol,
                                                          float vol_scale, float post) {
-    __shared__ float st0[StageShape<ZS>::taps];
-    __shared__ float st1[StageShape<ZS>::taps];
+    __shared__ float stage[2 * StageShape<ZS>::taps];
     vi_staged_tile<TV, FDK, ZS, PAIR>(g, sino, boxes, vol, vol_scale, post, blockIdx.x, blockIdx.y, blockIdx.z,
-                                      st0, st1);
+                                      stage);
 }
 
 // u-pair words of an FP16 sinogram, one workgroup per detector row: word j in
@@ -505,15 +448,14 @@ __global__ __launch_bounds__(256) void vi_pair_kernel(const uint16_t* __restrict
 // ceil(ny/SY), ceil(nz/16)) on the sinogram `in` with the problem's box table
 // `aux` (launch_vi_boxes); the stored value is (delta * sum) * a.
 __global__ __launch_bounds__(SX* SY, 7) void vi_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
-    __shared__ float st0[StageShape<kViTileZ>::taps];
-    __shared__ float st1[StageShape<kViTileZ>::taps];
+    __shared__ float stage[2 * StageShape<kViTileZ>::taps];
     TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
         const uint32_t bx = local % it.gx, rest = local / it.gx;
         const int by = (int)(rest % it.gy), bz = (int)(rest / it.gy);
         vi_staged_tile<float, false, kViTileZ>(it.g, (const float*)it.in, (const ViBox*)it.aux, (float*)it.out, 1.0f,
-                                               it.a, (int)bx, by, bz, st0, st1);
+                                               it.a, (int)bx, by, bz, stage);
     }
 }
 
