@@ -659,10 +659,11 @@ def main():
     ops["siddon"]["n_seg"] = n_seg
     ops["rs"]["n_samp"] = n_samp
     ops["rs"]["form"] = "per-lane gathers from the z-pair copies (rs_kernel)"
-    ops["rs_staged"]["form"] = "taps staged in LDS per window of up to 4 planes (TOMO_FLAG_STAGED, rs_staged_kernel)"
+    ops["rs_staged"]["form"] = ("taps staged in LDS per window of up to 8 sample planes, copied as 16-byte "
+                                "chunks (TOMO_FLAG_STAGED, rs_staged_kernel)")
     ops["siddon"]["form"] = "per-lane gathers from the z-pair copies (siddon_kernel)"
-    ops["siddon_staged"]["form"] = ("quads staged in LDS per window of up to 4 slabs (TOMO_FLAG_STAGED, "
-                                    "siddon_staged_kernel)")
+    ops["siddon_staged"]["form"] = ("quads staged in LDS per window of up to 8 slabs, copied as 16-byte chunks "
+                                    "(TOMO_FLAG_STAGED, siddon_staged_kernel)")
     ops["siddon_staged"]["n_seg"] = n_seg
     ops["rs_staged"]["n_samp"] = n_samp
     # compulsory traffic: the volume and the sinogram once each
