@@ -59,7 +59,9 @@ def parse():
                     help="views of the FP64 oracle check of the mixed forms at 512^3 / 512 views "
                          "(0 skips the mixed legs)")
     ap.add_argument("--c2-oracle-views", type=int, default=32,
-                    help="views of the oracle checks of the 720-view workload at 512^3 (a quarter at 1024^3)")
+                    help="views of the oracle checks of the 720-view workload at edges up to 512")
+    ap.add_argument("--c2-oracle-views-large", type=int, default=8,
+                    help="views of the oracle checks of the 720-view workload at edges above 512")
     ap.add_argument("--c2-sizes", default="512,1024",
                     help="edges of the mixed-precision workload (720 views, 768x512 detector); empty skips it")
     ap.add_argument("--batch-problems", type=int, default=64,
@@ -329,13 +331,23 @@ def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views, check_fp32=False
     o = Oracle(gsub)
     vol_host = vol.cpu().numpy()
     t0 = time.perf_counter()
-    ref = {"siddon": o.siddon(vol_host.astype(np.float64), f64=True),
-           "rs": o.rs(vol_host.astype(np.float64), f64=True)}
-    sino_sub = o.siddon(vol_host)  # FP32 sinogram subset, the backprojection input
-    ref["vi"] = o.vi(sino_sub.astype(np.float64), f64=True)
+
+    def oracle_step(name, fn):
+        t = time.perf_counter()
+        r = fn()
+        print(f"[mixed] n={n}: oracle {name} on {oracle_views} views: {time.perf_counter() - t:.1f} s",
+              file=sys.stderr, flush=True)
+        return r
+
+    ref = {"siddon": oracle_step("siddon f64", lambda: o.siddon(vol_host.astype(np.float64), f64=True)),
+           "rs": oracle_step("rs f64", lambda: o.rs(vol_host.astype(np.float64), f64=True))}
+    # FP32 sinogram subset, the backprojection input
+    sino_sub = oracle_step("siddon f32", lambda: o.siddon(vol_host))
+    ref["vi"] = oracle_step("vi f64", lambda: o.vi(sino_sub.astype(np.float64), f64=True))
     ref32 = {}
     if check_fp32:
-        ref32 = {"siddon": sino_sub, "rs": o.rs(vol_host), "vi": o.vi(sino_sub)}
+        ref32 = {"siddon": sino_sub, "rs": oracle_step("rs f32", lambda: o.rs(vol_host)),
+                 "vi": oracle_step("vi f32", lambda: o.vi(sino_sub))}
     del vol_host
     print(f"[mixed] n={n}: oracle on {oracle_views} views: {time.perf_counter() - t0:.1f} s", file=sys.stderr,
           flush=True)
@@ -412,7 +424,7 @@ def mixed_leg(ctx, G, desc, vol, fp32_ms, stream, oracle_views, check_fp32=False
     return out
 
 
-def c2_leg(ctx, sizes, stream, oracle_views):
+def c2_leg(ctx, sizes, stream, oracle_views, oracle_views_large):
     """The mixed-precision workload at each edge n: 720 views over 360 deg,
     768 x 512 detector with pixels of 2 n / 512 mm (full vertical coverage),
     FOD / FDD 500 / 1000 mm scaled by n / 256, 1 mm voxels; volume: 64
@@ -439,7 +451,7 @@ def c2_leg(ctx, sizes, stream, oracle_views):
               flush=True)
         del sino, back
         torch.cuda.empty_cache()
-        res = mixed_leg(ctx, G, desc, vol, fp32, stream, oracle_views if n <= 512 else max(1, oracle_views // 4),
+        res = mixed_leg(ctx, G, desc, vol, fp32, stream, oracle_views if n <= 512 else max(1, oracle_views_large),
                         check_fp32=True)
         res["fp32"] = {k: {"ms": round(v, 3), "gups": round(upd / v / 1e6, 2)} for k, v in fp32.items()}
         out[str(n)] = res
@@ -705,7 +717,8 @@ def main():
     torch.cuda.empty_cache()
     mixed_c2 = None
     if a.c2_sizes.strip() and a.mixed_oracle_views > 0:
-        mixed_c2 = c2_leg(ctx, [int(x) for x in a.c2_sizes.split(",")], stream, a.c2_oracle_views)
+        mixed_c2 = c2_leg(ctx, [int(x) for x in a.c2_sizes.split(",")], stream, a.c2_oracle_views,
+                          a.c2_oracle_views_large)
     if a.serving_requests > 0:
         names = {"recurrent": lib.SCHEDULE_RECURRENT, "direct": lib.SCHEDULE_DIRECT}
         scheds = [names[x] for x in a.serving_schedules.split(",") if x]
