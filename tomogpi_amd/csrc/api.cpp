@@ -160,6 +160,19 @@ int tomo_ctx_create(int device, tomo_ctx** out) {
     TOMO_HIP(hipSetDevice(device));
     const int rc = noalloc_scratch_init(device);
     if (rc) return rc;
+    // Keep up to a quarter of the device's memory in its stream-ordered pool
+    // between operator calls.  The projectors' per-call copies and the loops'
+    // work buffers then come back from the pool; with the pool's default
+    // threshold (0) every synchronisation handed them back to the driver and
+    // the next allocation mapped them again, which left the device idle for
+    // up to seconds per reconstruction, by an amount that varied between runs
+    // of the same one.  tomo_ctx_destroy trims what is unused.
+    hipMemPool_t pool = nullptr;
+    size_t free_b = 0, total_b = 0;
+    TOMO_HIP(hipDeviceGetDefaultMemPool(&pool, device));
+    TOMO_HIP(hipMemGetInfo(&free_b, &total_b));
+    uint64_t keep = (uint64_t)(total_b / 4);
+    TOMO_HIP(hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &keep));
     auto* c = new tomo_ctx();
     c->device = device;
     *out = c;
@@ -168,6 +181,9 @@ int tomo_ctx_create(int device, tomo_ctx** out) {
 
 int tomo_ctx_destroy(tomo_ctx* ctx) {
     if (!ctx) return TOMO_OK;
+    hipMemPool_t pool = nullptr;
+    if (hipSetDevice(ctx->device) == hipSuccess && hipDeviceGetDefaultMemPool(&pool, ctx->device) == hipSuccess)
+        (void)hipMemPoolTrimTo(pool, 0);
     delete ctx;
     return TOMO_OK;
 }

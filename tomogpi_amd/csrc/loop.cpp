@@ -29,11 +29,53 @@ struct DevBuf {
     }
 };
 
-int project(const tomo_geom* g, const tomo_gd_params& pr, const float* vol, float* sino, hipStream_t s) {
+// A loop's work buffer from the stream-ordered pool of the loop's stream,
+// released on that stream (after the work queued on it), so allocating and
+// releasing it neither waits for the device nor stalls other streams.
+struct PoolBuf {
+    void* p = nullptr;
+    hipStream_t s = nullptr;
+    hipError_t alloc(size_t bytes, hipStream_t st) {
+        s = st;
+        return hipMallocAsync(&p, bytes, st);
+    }
+    ~PoolBuf() {
+        if (p) (void)hipFreeAsync(p, s);
+    }
+};
+
+// The projector's two z-pair copies, allocated once for a loop's run and
+// passed to each of its projections (which otherwise take them from the
+// stream-ordered pool per call: two volume-sized allocations per projection,
+// whose pool allocations stalled the host for up to seconds now and then
+// between the loop's synchronisations).  The compat Siddon reads the volume
+// itself and takes none; a Siddon volume beyond its 32-bit traversal index
+// takes none either, so that the projector reports it as unsupported.
+struct LoopCopies {
+    PoolBuf qx, qy;
+    ProjExtra extra;
+    int alloc(const tomo_geom* g, const tomo_gd_params& pr, int storage, hipStream_t s) {
+        const int nx = g->info.nx, ny = g->info.ny, nz = g->info.nz;
+        if (pr.pair == TOMO_PAIR_SIDDONVI &&
+            ((pr.flags & TOMO_FLAG_SIDDON_COMPAT) || zpair_elems_x(nx, ny, nz) >= (1ull << 32) ||
+             zpair_elems_y(nx, ny, nz) >= (1ull << 32)))
+            return TOMO_OK;
+        const size_t es = storage == TOMO_F32 ? 4 : 2;
+        TOMO_HIP(qx.alloc(zpair_elems_x(nx, ny, nz) * es, s));
+        TOMO_HIP(qy.alloc(zpair_elems_y(nx, ny, nz) * es, s));
+        extra.qx = qx.p;
+        extra.qy = qy.p;
+        return TOMO_OK;
+    }
+};
+
+int project(const tomo_geom* g, const tomo_gd_params& pr, const float* vol, float* sino, hipStream_t s,
+            const ProjExtra& x) {
     Buf in{vol, TOMO_F32, 1.0f};
     MutBuf out{sino, TOMO_F32, 1.0f};
-    if (pr.pair == TOMO_PAIR_SIDDONVI) return launch_siddon(g, (pr.flags & TOMO_FLAG_SIDDON_COMPAT) ? 1 : 0, in, out, s);
-    return launch_rs(g, (pr.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, in, out, s);
+    if (pr.pair == TOMO_PAIR_SIDDONVI)
+        return launch_siddon(g, (pr.flags & TOMO_FLAG_SIDDON_COMPAT) ? 1 : 0, in, out, s, x);
+    return launch_rs(g, (pr.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, in, out, s, x);
 }
 
 // Operator inputs of the mixed-precision loops (tomo_gd_params.storage
@@ -44,10 +86,10 @@ int project(const tomo_geom* g, const tomo_gd_params& pr, const float* vol, floa
 // reused by the next operator of the same stream, after this one read it.
 struct Mixed {
     int dtype = TOMO_F32;
-    DevBuf tmp;
-    int init(int storage, size_t n) {
+    PoolBuf tmp;
+    int init(int storage, size_t n, hipStream_t s) {
         dtype = storage;
-        if (dtype != TOMO_F32) TOMO_HIP(hipMalloc(&tmp.p, n * 2));
+        if (dtype != TOMO_F32) TOMO_HIP(tmp.alloc(n * 2, s));
         return TOMO_OK;
     }
     int stage(const float* x, size_t n, Buf* in, hipStream_t s) {
@@ -71,13 +113,15 @@ struct Mixed {
     }
 };
 
-int project(const tomo_geom* g, const tomo_gd_params& pr, Mixed& mx, const float* vol, float* sino, hipStream_t s) {
+int project(const tomo_geom* g, const tomo_gd_params& pr, Mixed& mx, const float* vol, float* sino, hipStream_t s,
+            const ProjExtra& x) {
     Buf in{};
     int rc = mx.stage(vol, g->nvox(), &in, s);
     if (rc) return rc;
     const MutBuf out{sino, TOMO_F32, 1.0f};
-    if (pr.pair == TOMO_PAIR_SIDDONVI) return launch_siddon(g, (pr.flags & TOMO_FLAG_SIDDON_COMPAT) ? 1 : 0, in, out, s);
-    return launch_rs(g, (pr.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, in, out, s);
+    if (pr.pair == TOMO_PAIR_SIDDONVI)
+        return launch_siddon(g, (pr.flags & TOMO_FLAG_SIDDON_COMPAT) ? 1 : 0, in, out, s, x);
+    return launch_rs(g, (pr.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0, in, out, s, x);
 }
 
 int backproject(const tomo_geom* g, Mixed& mx, const float* sino, float* vol, hipStream_t s) {
@@ -92,22 +136,24 @@ int backproject(const tomo_geom* g, Mixed& mx, const float* sino, float* vol, hi
 int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const float* sino, const float* weight,
            float* vol, tomo_gd_iter_stat* stats, hipStream_t s) {
     const size_t nv = g->nvox(), ns = g->nsino();
-    DevBuf est, dj, t1, t2;
-    TOMO_HIP(hipMalloc(&est.p, ns * sizeof(float)));
-    TOMO_HIP(hipMalloc(&dj.p, nv * sizeof(float)));
+    PoolBuf est, dj, t1, t2;
+    TOMO_HIP(est.alloc(ns * sizeof(float), s));
+    TOMO_HIP(dj.alloc(nv * sizeof(float), s));
     if (pr.lambda != 0.0) {
-        TOMO_HIP(hipMalloc(&t1.p, nv * sizeof(float)));
-        TOMO_HIP(hipMalloc(&t2.p, nv * sizeof(float)));
+        TOMO_HIP(t1.alloc(nv * sizeof(float), s));
+        TOMO_HIP(t2.alloc(nv * sizeof(float), s));
     }
     Mixed mx;
-    int rc = mx.init(pr.storage, std::max(nv, ns));
+    int rc = mx.init(pr.storage, std::max(nv, ns), s);
     if (rc) return rc;
+    LoopCopies cp;
+    if ((rc = cp.alloc(g, pr, pr.storage, s))) return rc;
     float* e = (float*)est.p;
     float* d = (float*)dj.p;
     double alpha_min = pr.alpha_min_init;
     for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
         tomo_gd_iter_stat st{0, 0, 0, 0, 0, 0};
-        if ((rc = project(g, pr, mx, vol, e, s))) return rc;
+        if ((rc = project(g, pr, mx, vol, e, s, cp.extra))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
         if (weight && (rc = launch_mul(e, weight, ns, s))) return rc;  // weighted form: r = W (y - P x)
@@ -131,7 +177,7 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         }
         double alpha;
         if (k < pr.optimal_step_iteration_nb) {
-            if ((rc = project(g, pr, mx, d, e, s))) return rc;
+            if ((rc = project(g, pr, mx, d, e, s, cp.extra))) return rc;
             if ((rc = reduce_sq_norm(ctx, Buf{d, TOMO_F32, 1.0f}, nv, &st.norm_dj, s))) return rc;
             double pe = 0.0;
             if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &pe, s))) return rc;
@@ -159,12 +205,12 @@ int gd_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
 int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const float* sino, float* vol,
            tomo_gd_iter_stat* stats, hipStream_t s) {
     const size_t nv = g->nvox(), ns = g->nsino();
-    DevBuf est, gb, db, t1, t2;
-    TOMO_HIP(hipMalloc(&est.p, ns * sizeof(float)));
-    TOMO_HIP(hipMalloc(&gb.p, nv * sizeof(float)));
-    TOMO_HIP(hipMalloc(&db.p, nv * sizeof(float)));
-    TOMO_HIP(hipMalloc(&t1.p, nv * sizeof(float)));
-    if (pr.lambda != 0.0) TOMO_HIP(hipMalloc(&t2.p, nv * sizeof(float)));
+    PoolBuf est, gb, db, t1, t2;
+    TOMO_HIP(est.alloc(ns * sizeof(float), s));
+    TOMO_HIP(gb.alloc(nv * sizeof(float), s));
+    TOMO_HIP(db.alloc(nv * sizeof(float), s));
+    TOMO_HIP(t1.alloc(nv * sizeof(float), s));
+    if (pr.lambda != 0.0) TOMO_HIP(t2.alloc(nv * sizeof(float), s));
     float* e = (float*)est.p;
     float* gr = (float*)gb.p;
     float* d = (float*)db.p;
@@ -172,12 +218,14 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
     float* b = (float*)t2.p;
     const int nx = g->info.nx, ny = g->info.ny, nz = g->info.nz;
     Mixed mx;
-    int rc = mx.init(pr.storage, std::max(nv, ns));
+    int rc = mx.init(pr.storage, std::max(nv, ns), s);
     if (rc) return rc;
+    LoopCopies cp;
+    if ((rc = cp.alloc(g, pr, pr.storage, s))) return rc;
     double gg_prev = 0.0;
     for (int k = 1; k <= pr.gradient_iteration_nb; ++k) {
         tomo_gd_iter_stat st{0, 0, 0, 0, 0, 0};
-        if ((rc = project(g, pr, mx, vol, e, s))) return rc;
+        if ((rc = project(g, pr, mx, vol, e, s, cp.extra))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &st.residual_sq, s))) return rc;
         if ((rc = backproject(g, mx, e, gr, s))) return rc;
@@ -200,7 +248,7 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
         gg_prev = gg;
         double gd = 0.0;
         if ((rc = reduce_dot(ctx, gr, d, nv, &gd, s))) return rc;
-        if ((rc = project(g, pr, mx, d, e, s))) return rc;
+        if ((rc = project(g, pr, mx, d, e, s, cp.extra))) return rc;
         double pe = 0.0, reg = 0.0;
         if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &pe, s))) return rc;
         if (pr.lambda != 0.0) {
@@ -225,14 +273,16 @@ int cg_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_gd_params& pr, const fl
 int sirt_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_sirt_params& pr, const float* sino, float* vol,
              double* residual_sq, hipStream_t s) {
     const size_t nv = g->nvox(), ns = g->nsino();
-    DevBuf rw, cw, est, b;
-    TOMO_HIP(hipMalloc(&rw.p, ns * sizeof(float)));
-    TOMO_HIP(hipMalloc(&est.p, ns * sizeof(float)));
-    TOMO_HIP(hipMalloc(&cw.p, nv * sizeof(float)));
-    TOMO_HIP(hipMalloc(&b.p, nv * sizeof(float)));
+    PoolBuf rw, cw, est, b;
+    TOMO_HIP(rw.alloc(ns * sizeof(float), s));
+    TOMO_HIP(est.alloc(ns * sizeof(float), s));
+    TOMO_HIP(cw.alloc(nv * sizeof(float), s));
+    TOMO_HIP(b.alloc(nv * sizeof(float), s));
     tomo_gd_params gp{};
     gp.pair = pr.pair;
     gp.flags = pr.flags;
+    LoopCopies cp;
+    if (const int arc = cp.alloc(g, gp, TOMO_F32, s)) return arc;
     float* R = (float*)rw.p;
     float* Cv = (float*)cw.p;
     float* e = (float*)est.p;
@@ -240,14 +290,14 @@ int sirt_run(tomo_ctx* ctx, const tomo_geom* g, const tomo_sirt_params& pr, cons
     int rc;
     // weights from the operators applied to ones
     if ((rc = launch_fill(bv, 1.0f, nv, s))) return rc;
-    if ((rc = project(g, gp, bv, R, s))) return rc;
+    if ((rc = project(g, gp, bv, R, s, cp.extra))) return rc;
     if ((rc = launch_recip_weight(R, 1.0f, R, ns, s))) return rc;
     if ((rc = launch_fill(e, 1.0f, ns, s))) return rc;
     if ((rc = launch_vi(g, Buf{e, TOMO_F32, 1.0f}, MutBuf{Cv, TOMO_F32, 1.0f}, s))) return rc;
     if ((rc = launch_recip_weight(Cv, (float)pr.relaxation, Cv, nv, s))) return rc;
     if ((rc = launch_fill(vol, 0.0f, nv, s))) return rc;
     for (int k = 0; k < pr.iterations; ++k) {
-        if ((rc = project(g, gp, vol, e, s))) return rc;
+        if ((rc = project(g, gp, vol, e, s, cp.extra))) return rc;
         if ((rc = launch_diff(sino, e, e, ns, s))) return rc;
         if (residual_sq) {
             if ((rc = reduce_sq_norm(ctx, Buf{e, TOMO_F32, 1.0f}, ns, &residual_sq[k], s))) return rc;
