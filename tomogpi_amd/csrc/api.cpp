@@ -158,8 +158,9 @@ int tomo_ctx_create(int device, tomo_ctx** out) {
     TOMO_HIP(hipGetDeviceCount(&n));
     if (device < 0 || device >= n) return fail(TOMO_ERR_INVALID_ARG, "no such device " + std::to_string(device));
     TOMO_HIP(hipSetDevice(device));
-    const int rc = noalloc_scratch_init(device);
+    int rc = noalloc_scratch_init(device);
     if (rc) return rc;
+    if ((rc = batch_grids_init(device))) return rc;
     // Keep up to a quarter of the device's memory in its stream-ordered pool
     // between operator calls.  The projectors' per-call copies and the loops'
     // work buffers then come back from the pool; with the pool's default

@@ -10,9 +10,18 @@
 // element-wise operators then take tiles w + kBatchGrid, w + 2 kBatchGrid, ...
 // (equal-cost tiles); the projectors and the backprojector, whose tiles differ
 // in cost with the problem's views and the rays' lengths, claim their next
-// tile from the table's counter instead (one vector atomic per tile, on a
-// cache line of its own, broadcast through LDS), which balances them over the
-// chip as hardware dispatch would.  Because the grid and the table address
+// tile from a counter of the table instead (one vector atomic per tile, on a
+// cache line apart from the items, broadcast through LDS), which balances them
+// as hardware dispatch would.  The dispatcher deals a launch's workgroups to
+// the kBatchXcds XCDs round robin, so workgroup w runs on XCD w mod kBatchXcds;
+// the claims keep that mapping for tiles too: the workgroups of XCD x claim
+// from counter x the tiles t = x (mod kBatchXcds), in order.  A single-problem
+// launch puts the same tiles on the same XCD (tile t is its workgroup t, up to
+// a rotation by the problem's first tile), so a detector column group meets
+// the same XCD's L2 at every view, as it does there; one shared counter sent
+// each tile to whichever XCD had a workgroup free, and the projectors' shared
+// launches ran about a fifth slower than their single-problem launches on the
+// same problems.  Because the grid and the table address
 // are fixed, a captured hipGraph of shared launches stays valid while the
 // table contents change between replays; the host's copy of the table before
 // each launch sequence resets the counter, so a table serves one launch per
@@ -24,7 +33,8 @@
 namespace tomo {
 
 constexpr int kBatchMax = 64;
-constexpr int kBatchGrid = 2048;
+constexpr int kBatchXcds = 8;     // XCDs of the chip: claim counters per table
+constexpr int kBatchGrid = 2048;  // a multiple of kBatchXcds
 constexpr int kVecTile = 4096;  // elements per tile of the element-wise ops
 // VI workgroup (single problems and shared launches): kViStageX x kViStageY
 // voxel columns of kViTileZ voxels, the taps staged in LDS.  Its transpose
@@ -68,10 +78,11 @@ struct BatchTable {
     uint32_t total;
     uint32_t pad[2];
     BatchItem item[kBatchMax];
-    // Tiles claimed beyond the first kBatchGrid (0 when the host writes the
-    // table); its own cache line keeps the atomics off the items.
-    alignas(128) uint32_t next;
+    // Tiles claimed beyond the first kBatchGrid, per XCD (0 when the host
+    // writes the table); their own cache line keeps the atomics off the items.
+    alignas(128) uint32_t next[kBatchXcds];
 };
+static_assert(kBatchGrid % kBatchXcds == 0, "every XCD starts with the same number of workgroups");
 
 // Item flag bit of the projector launches: the operator copies (z-pairs) are
 // BF16, converted from the FP32 state by the copy launches.  RS keeps bit 0
@@ -94,15 +105,21 @@ enum GdSlot {
 // Static order: tiles blockIdx.x, blockIdx.x + gridDim.x, ... (`tile` in the body).
 #define TOMO_BATCH_STRIDED(t) for (uint32_t tile = blockIdx.x; tile < (t)->total; tile += gridDim.x)
 
-// The workgroup's next claimed tile (every thread of the workgroup calls it).
+// The workgroup's next claimed tile (every thread of the workgroup calls it):
+// XCD x's k-th claim is tile x + kBatchXcds (gridDim.x / kBatchXcds + k), so
+// the residue class x is covered once, after the first tiles blockIdx.x.
 __device__ __forceinline__ uint32_t batch_claim(uint32_t* next, uint32_t* slot) {
     __syncthreads();  // the previous tile's readers of *slot are done
-    if (threadIdx.x == 0 && threadIdx.y == 0) *slot = gridDim.x + atomicAdd(next, 1u);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        const uint32_t x = blockIdx.x % kBatchXcds;
+        *slot = x + kBatchXcds * (gridDim.x / kBatchXcds + atomicAdd(next + x, 1u));
+    }
     __syncthreads();
     return *slot;
 }
 
-// Dynamic order: tile blockIdx.x, then tiles claimed from `next`.
+// Dynamic order: tile blockIdx.x, then tiles claimed from the counters
+// `next` (kBatchXcds of them).
 #define TOMO_BATCH_CLAIMED(t, next)                                                         \
     __shared__ uint32_t tomo_batch_slot;                                                    \
     for (uint32_t tile = blockIdx.x; tile < (t)->total; tile = ::tomo::batch_claim((next), &tomo_batch_slot))
@@ -125,6 +142,11 @@ hipError_t launch_batch_zpair_x(const BatchTable* t, hipStream_t s);   // in vol
 hipError_t launch_batch_zpair_y(const BatchTable* t, hipStream_t s);   // in volume -> out qy (FP32/BF16)
 hipError_t launch_batch_siddon(BatchTable* t, hipStream_t s);          // in qx, in2 qy -> out sinogram
 hipError_t launch_batch_rs(BatchTable* t, hipStream_t s);              // in qx, in2 qy -> out; flags bit0 cube
+// Grids of the shared launches whose workgroups take more registers than
+// kBatchGrid resident workgroups allow (RS): the number the device holds at
+// once, a multiple of kBatchXcds.  Called at context creation, before any
+// launch is captured.
+int batch_grids_init(int device);
 hipError_t launch_batch_vi(BatchTable* t, hipStream_t s);              // in sinogram, aux boxes -> out volume * a
 int launch_vi_boxes(const DevGeom& d, void* boxes, hipStream_t s);     // vi_box_count(d) ViBox of the geometry
 hipError_t launch_batch_diff_sq(const BatchTable* t, hipStream_t s);   // out = in - in2; aux partials of out^2

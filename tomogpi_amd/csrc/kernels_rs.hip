@@ -15,6 +15,8 @@
 // The ray setup is FP64 (the reference parametrises by x, which is singular
 // for rays parallel to y; the arc-length form here is well conditioned), the
 // samples and the sum are FP32.  The output is overwritten.
+#include <atomic>
+
 #include "zpair.hpp"
 #include "batch.hpp"
 #include "stage_box.hpp"
@@ -627,10 +629,14 @@ __global__ __launch_bounds__(BX* BY, 5) void rs_staged_kernel(DevGeom g, int cub
 }
 
 // Shared launch (FP32 or BF16 (kItemBF16) z-pair copies, FP32 sinograms);
-// item flags bit 0 selects the cube FOV.
-// 8 waves per SIMD (52 VGPRs, no spills; unbounded, its four sampler
-// variants take 86 and 5 waves).
-__global__ __launch_bounds__(BX* BY, 8) void rs_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
+// item flags bit 0 selects the cube FOV.  No minimum-waves bound: under one
+// (8 or 6 waves per SIMD, 52 or 74 VGPRs) the compiler waits for each
+// sample's two gathers before it issues the next sample's, where unbounded
+// (86 VGPRs, 5 waves per SIMD) it issues the eight gathers of four samples
+// together, as rs_kernel does; the serialised form took 27% longer per
+// 512^3 projection.  Its grid is what the chip holds at once
+// (batch_grids_init), so every workgroup is resident from the start.
+__global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __restrict__ t, uint32_t* __restrict__ next) {
     TOMO_BATCH_CLAIMED(t, next) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         const uint32_t local = tile - it.begin;
@@ -743,8 +749,22 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s,
     return TOMO_OK;
 }
 
+namespace {
+std::atomic<int> g_rs_batch_grid{kBatchGrid};
+}
+
+int batch_grids_init(int device) {
+    int per_cu = 0, cus = 0;
+    TOMO_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, rs_batch_kernel, BX * BY, 0));
+    TOMO_HIP(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device));
+    int grid = per_cu * cus;
+    grid -= grid % kBatchXcds;
+    g_rs_batch_grid.store(std::min(std::max(grid, kBatchXcds), kBatchGrid));
+    return TOMO_OK;
+}
+
 hipError_t launch_batch_rs(BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(rs_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t, &t->next);
+    hipLaunchKernelGGL(rs_batch_kernel, dim3(g_rs_batch_grid.load()), dim3(BX, BY), 0, s, t, t->next);
     return hipGetLastError();
 }
 

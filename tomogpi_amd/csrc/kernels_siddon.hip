@@ -971,7 +971,7 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
 }
 
 hipError_t launch_batch_siddon(BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(siddon_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t, &t->next);
+    hipLaunchKernelGGL(siddon_batch_kernel, dim3(kBatchGrid), dim3(BX, BY), 0, s, t, t->next);
     return hipGetLastError();
 }
 

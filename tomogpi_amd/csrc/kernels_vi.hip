@@ -601,7 +601,7 @@ int launch_vi_boxes(const DevGeom& d, void* boxes, hipStream_t s) {
 }
 
 hipError_t launch_batch_vi(BatchTable* t, hipStream_t s) {
-    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(SX * SY), 0, s, t, &t->next);
+    hipLaunchKernelGGL(vi_batch_kernel, dim3(kBatchGrid), dim3(SX * SY), 0, s, t, t->next);
     return hipGetLastError();
 }
 

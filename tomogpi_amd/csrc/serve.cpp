@@ -241,7 +241,7 @@ struct TableBuilder {
     void reset() {
         t->count = 0;
         t->total = 0;
-        t->next = 0;
+        for (uint32_t& c : t->next) c = 0;
     }
     BatchItem& add(const DevGeom& g, uint32_t gx, uint32_t gy, uint32_t gz) {
         BatchItem& it = t->item[t->count++];
