@@ -209,6 +209,17 @@ __device__ __forceinline__ TO copy_cvt(TI v) {
     else return __bfloat16_as_ushort(__float2bfloat16(v));
 }
 
+// One z-pair (the two elements at an even index o) as one aligned store: the
+// copies start on 256-byte boundaries, so out + o is aligned to the pair.
+template <typename T>
+__device__ __forceinline__ void store_pair(T* __restrict__ out, size_t o, T a, T b) {
+    typedef T T2 __attribute__((ext_vector_type(2)));
+    T2 v;
+    v.x = a;
+    v.y = b;
+    *reinterpret_cast<T2*>(out + o) = v;
+}
+
 // z-pair copy for y slabs (A = y, B = x): B is also the input's fastest axis,
 // a direct pass.  Grid (ceil((nx+2)/256), ny, nz+1).
 template <typename TI, typename T>
@@ -225,9 +236,7 @@ __device__ __forceinline__ void zpair_y_tile(const TI* __restrict__ in, T* __res
         if (x >= 0 && x < nx && z >= 0 && z < nz)
             v[c] = copy_cvt<T>(in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]);
     }
-    const size_t o = zpair_index(y, zz, bb, nz, nx);
-    out[o] = v[0];
-    out[o + 1] = v[1];
+    store_pair(out, zpair_index(y, zz, bb, nz, nx), v[0], v[1]);
 }
 
 // z-pair copy for x slabs (A = x, B = y): a 32x32 (x, B) tile transpose of
@@ -253,9 +262,7 @@ __device__ __forceinline__ void zpair_x_tile(const TI* __restrict__ in, T* __res
     for (int r = ty; r < 32; r += 8) {
         const int x = x0 + r, bb = b0 + tx;
         if (x < nx && bb <= ny + 1) {
-            const size_t o = zpair_index(x, zz, bb, nz, ny);
-            out[o] = tile[0][tx][r];
-            out[o + 1] = tile[1][tx][r];
+            store_pair(out, zpair_index(x, zz, bb, nz, ny), tile[0][tx][r], tile[1][tx][r]);
         }
     }
 }
@@ -342,17 +349,22 @@ __global__ __launch_bounds__(256) void zpair_x_batch_kernel(const BatchTable* __
     }
 }
 
+// A tile is a whole (y, z pair) row of nx + 2 pairs, in chunks of 256 (the
+// single-problem grid's tiles of one row): tiles of one chunk left half the
+// launch's workgroups nearly empty at 256^3 (258 pairs a row) and paid the
+// table lookup for each.
 __global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __restrict__ t) {
     TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
-        if (it.flags & kItemBF16)
-            zpair_y_tile<float, uint16_t>((const float*)it.in, (uint16_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by,
-                                          bz);
-        else
-            zpair_y_tile<uint32_t, uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx,
-                                             by, bz);
+        const int nx = it.g.nx, ny = it.g.ny, nz = it.g.nz;
+        for (int c = 0; c * 256 < nx + 2; ++c) {
+            if (it.flags & kItemBF16)
+                zpair_y_tile<float, uint16_t>((const float*)it.in, (uint16_t*)it.out, nx, ny, nz, c, by, bz);
+            else
+                zpair_y_tile<uint32_t, uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, nx, ny, nz, c, by, bz);
+        }
     }
 }
 

@@ -287,7 +287,7 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
             zx.in = vol;
             zx.out = q.qx;
             zx.flags = cflag;
-            BatchItem& zy = tb[opy].add(d, cdiv(d.nx + 2, 256), d.ny, d.nz + 1);
+            BatchItem& zy = tb[opy].add(d, 1, d.ny, d.nz + 1);  // whole rows
             zy.in = vol;
             zy.out = q.qy;
             zy.flags = cflag;
