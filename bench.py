@@ -229,8 +229,8 @@ def serving_leg(ctx, n_req, rates, schedules, seed=42, bf16_rate=None, direct_re
     torch.cuda.synchronize()
     gd = dict(iters=20, kopt=21, lam=0.01)
     # Sequential plain form: the faster of two timed runs per configuration
-    # (single runs of one configuration have taken up to 2.5x longer now and
-    # then on the same inputs, with identical results).
+    # (repeated runs now agree within 1%; the minimum keeps a one-off host
+    # stall out of the baseline).
     service_of, refs = {}, {}
     for i, r in enumerate(reqs):
         if r["key"] in service_of:
