@@ -1,7 +1,8 @@
 // batch.hpp - problem tables of the shared (batched) launches.
 //
 // A shared launch runs one operator for up to kBatchMax independent problems
-// of different sizes.  Its grid is fixed (kBatchGrid workgroups) and
+// of different sizes.  Its grid is fixed (kBatchGrid workgroups; RS: the
+// workgroups the device holds at once, batch_grids_init) and
 // persistent.  The tiles of problem q are [item[q].begin, item[q].begin +
 // item[q].tiles); a tile is one workgroup of the single-problem kernel's
 // grid, numbered x fastest, then y, then z, and runs the same code as there.
