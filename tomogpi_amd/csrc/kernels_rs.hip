@@ -98,6 +98,27 @@ __device__ __forceinline__ RsRay rs_setup(const DevGeom& g, int cube, int p, int
     return r;
 }
 
+// The trilinear blend of a sample's eight taps, given as the (B, z) quads of
+// its two A planes: u on plane floor(A), w on floor(A) + 1, each ordered
+// (b, z), (b, z+1), (b+1, z), (b+1, z+1).  Interpolated along x, then y, then
+// z with explicit fmaf.
+template <int A>
+__device__ __forceinline__ float trilinear_taps(float u0, float u1, float u2, float u3, float w0, float w1, float w2,
+                                                float w3, float fx, float fy, float fz) {
+    float t000, t100, t010, t110, t001, t101, t011, t111;
+    if (A == 0) {
+        t000 = u0; t001 = u1; t010 = u2; t011 = u3;
+        t100 = w0; t101 = w1; t110 = w2; t111 = w3;
+    } else {
+        t000 = u0; t001 = u1; t100 = u2; t101 = u3;
+        t010 = w0; t011 = w1; t110 = w2; t111 = w3;
+    }
+    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
+    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
+    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
+    return fmaf(fz, c1 - c0, c0);
+}
+
 // Trilinear interpolation of f at (x, y, z) (voxel units), taps outside the
 // grid 0, interpolated along x, then y, then z with explicit fmaf.  The eight
 // taps come from two z-pair quads of the slab axis A: the (B, z) block of the
@@ -122,18 +143,7 @@ __device__ __forceinline__ float trilinear_quads(const TV* __restrict__ q, int n
     // u, w = (b, z), (b, z+1), (b+1, z), (b+1, z+1) on planes ia, ia+1
     const float u0 = p0 ? u.x : 0.0f, u1 = p0 ? u.y : 0.0f, u2 = p0 ? u.z : 0.0f, u3 = p0 ? u.w : 0.0f;
     const float w0 = p1 ? w.x : 0.0f, w1 = p1 ? w.y : 0.0f, w2 = p1 ? w.z : 0.0f, w3 = p1 ? w.w : 0.0f;
-    float t000, t100, t010, t110, t001, t101, t011, t111;
-    if (A == 0) {
-        t000 = u0; t001 = u1; t010 = u2; t011 = u3;
-        t100 = w0; t101 = w1; t110 = w2; t111 = w3;
-    } else {
-        t000 = u0; t001 = u1; t100 = u2; t101 = u3;
-        t010 = w0; t011 = w1; t110 = w2; t111 = w3;
-    }
-    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
-    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
-    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
-    return fmaf(fz, c1 - c0, c0);
+    return trilinear_taps<A>(u0, u1, u2, u3, w0, w1, w2, w3, fx, fy, fz);
 }
 
 // Buffer-path sampler, for copies below 4 GiB whose A-plane stride is below
@@ -184,18 +194,59 @@ __device__ __forceinline__ float trilinear_quads_buf(const CopyBuf& q, int nx, i
         u0 = U::lo(u[0]); u1 = U::hi(u[0]); u2 = U::lo(u[1]); u3 = U::hi(u[1]);
         w0 = U::lo(w[0]); w1 = U::hi(w[0]); w2 = U::lo(w[1]); w3 = U::hi(w[1]);
     }
-    float t000, t100, t010, t110, t001, t101, t011, t111;
-    if (A == 0) {
-        t000 = u0; t001 = u1; t010 = u2; t011 = u3;
-        t100 = w0; t101 = w1; t110 = w2; t111 = w3;
-    } else {
-        t000 = u0; t001 = u1; t100 = u2; t101 = u3;
-        t010 = w0; t011 = w1; t110 = w2; t111 = w3;
-    }
-    const float c00 = fmaf(fx, t100 - t000, t000), c10 = fmaf(fx, t110 - t010, t010);
-    const float c01 = fmaf(fx, t101 - t001, t001), c11 = fmaf(fx, t111 - t011, t011);
-    const float c0 = fmaf(fy, c10 - c00, c00), c1 = fmaf(fy, c11 - c01, c01);
-    return fmaf(fz, c1 - c0, c0);
+    return trilinear_taps<A>(u0, u1, u2, u3, w0, w1, w2, w3, fx, fy, fz);
+}
+
+// A-pair sampler (16-bit volumes, copies below 4 GiB): the eight taps are two
+// neighbouring groups of the A-pair copy (zpair.hpp), one 16-byte load where
+// the z-pair copy takes two.  Its planes -1 and nA are zeros of the copy, so
+// only a block outside the grid (B or z corner = n, A corner = nA) or an
+// inactive sample reads from kOutside.  Same taps and arithmetic as above.
+struct ApairShape {
+    uint64_t bytes_x, bytes_y;
+    uint32_t sa_x, sz_x, sa_y, sz_y;
+    bool buf;
+};
+__host__ __device__ __forceinline__ ApairShape rs_apair_shape(int nx, int ny, int nz) {
+    ApairShape c;
+    c.sz_x = (uint32_t)(ny + 2) * 8u;
+    c.sz_y = (uint32_t)(nx + 2) * 8u;
+    c.bytes_x = apair_elems_x(nx, ny, nz) * 2u;
+    c.bytes_y = apair_elems_y(nx, ny, nz) * 2u;
+    const uint64_t sax = (uint64_t)c.sz_x * (uint64_t)(nz + 1), say = (uint64_t)c.sz_y * (uint64_t)(nz + 1);
+    c.sa_x = (uint32_t)sax;
+    c.sa_y = (uint32_t)say;
+    c.buf = c.bytes_x < kOutside && c.bytes_y < kOutside && sax < (1ull << 24) && say < (1ull << 24);
+    return c;
+}
+
+template <int A, typename TV>
+__device__ __forceinline__ float trilinear_apair_buf(const CopyBuf& q, int nx, int ny, int nz, float x, float y,
+                                                     float z, bool act) {
+    static_assert(sizeof(TV) == 2, "A-pair copies hold 16-bit volumes");
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    const float xc = __builtin_amdgcn_fmed3f(xf, -1.0f, (float)nx);
+    const float yc = __builtin_amdgcn_fmed3f(yf, -1.0f, (float)ny);
+    const float zc = __builtin_amdgcn_fmed3f(zf, -1.0f, (float)nz);
+    const bool in = act & (xc == xf) & (yc == yf) & (zc == zf);
+    const int ix = (int)xc, iy = (int)yc, iz = (int)zc;
+    const float fx = x - xf, fy = y - yf, fz = z - zf;
+    const int ia = A == 0 ? ix : iy, ib = A == 0 ? iy : ix;
+    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    const bool blk = in & (ib < nB) & (iz < nz) & (ia < nA);  // ia >= -1 from the clamp
+    const uint32_t bq = (uint32_t)(min(ib, nB - 1) + 1), zq = (uint32_t)(min(iz, nz - 1) + 1);
+    const uint32_t ap = (uint32_t)(min(ia, nA - 1) + 1);
+    // kOutside unless blk, as a mask: a select of the offset would let the
+    // compiler move its arithmetic into a branch, which splits the unrolled
+    // samples and serialises their loads
+    const uint32_t keep = 0u - (uint32_t)blk;
+    const uint32_t o = ((__umul24(ap, q.sa) + __umul24(zq, q.sz) + bq * 8u) & keep) | (kOutside & ~keep);
+    const auto d = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o, 0, 0);
+    using U = Unpack16<TV>;
+    // d[0], d[1] = (a, b, z), (a, b, z+1), (a+1, b, z), (a+1, b, z+1); d[2], d[3]: the same at b + 1
+    const float u0 = U::lo(d[0]), u1 = U::hi(d[0]), w0 = U::lo(d[1]), w1 = U::hi(d[1]);
+    const float u2 = U::lo(d[2]), u3 = U::hi(d[2]), w2 = U::lo(d[3]), w3 = U::hi(d[3]);
+    return trilinear_taps<A>(u0, u1, u2, u3, w0, w1, w2, w3, fx, fy, fz);
 }
 
 __device__ __forceinline__ int wave_min(int v) {
@@ -216,14 +267,15 @@ __device__ __forceinline__ float wave_minf(float v) {
 // s = k - k0 for k0 <= k < k0 + lr.  Same positions (fma(s, d, entry)) and
 // the same order of the sum as the per-ray loop.
 // q is the z-pair copy of A (FP32 or 16-bit, as stored); BUF reads it
-// through qb (see the buffer-path samplers).  The buffer path
+// through qb (see the buffer-path samplers), AP (16-bit, BUF) the A-pair copy
+// of A through qb instead.  The buffer path
 // has no branch: RS_UNROLL samples per iteration, their loads in flight
 // together, an inactive sample (s outside [0, lr)) adding +0 in its place,
 // which leaves the sum unchanged (it starts at +0 and is never -0).
 #ifndef RS_UNROLL
 #define RS_UNROLL 4
 #endif
-template <int A, typename TV, bool BUF>
+template <int A, typename TV, bool BUF, bool AP>
 __device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restrict__ q, const CopyBuf& qb, int nx,
                                             int ny, int nz, int k0, int kend) {
     float acc = 0.0f;
@@ -236,7 +288,8 @@ __device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restri
                 const float sf = (float)s;
                 const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
                 const bool act = (s >= 0) & (s < r.lr);
-                v[j] = trilinear_quads_buf<A, TV>(qb, nx, ny, nz, x, y, z, act);
+                if constexpr (AP) v[j] = trilinear_apair_buf<A, TV>(qb, nx, ny, nz, x, y, z, act);
+                else v[j] = trilinear_quads_buf<A, TV>(qb, nx, ny, nz, x, y, z, act);
             }
 #pragma unroll
             for (int j = 0; j < RS_UNROLL; ++j) acc += v[j];
@@ -283,9 +336,10 @@ __host__ __device__ __forceinline__ CopyShape rs_copy_shape(int nx, int ny, int 
 // samples near the same A plane, one sample per lane per iteration.  Its 64
 // rays (a 32 x 2 block of detector cells) then read neighbouring quads of the
 // z-pair copy of A, two loads per sample (16 bytes each for FP32 storage,
-// 8 for 16-bit).  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
+// 8 for 16-bit), or (AP, 16-bit volumes) the A-pair copy of A, one 16-byte
+// load per sample.  Other waves sample from k = 0 on the y-slab copy.  Grid: x = column group, y = view, z = row group
 // (every view of a row band before the next band).
-template <typename TV, typename TS, bool BUF>
+template <typename TV, typename TS, bool BUF, bool AP = false>
 __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* __restrict__ qx,
                                         const void* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
                                         float sino_scale, int bx, int p, int bz) {
@@ -324,14 +378,21 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* 
     // storage scale applies once to it (FP32 storage has no scale).
     CopyBuf qb{};
     if constexpr (BUF) {
-        const CopyShape cs = rs_copy_shape(nx, ny, nz, sizeof(TV));
         const bool xs = mode >= 0 && mode < 2;
-        qb.r = buffer_rsrc(xs ? qx : qy, (uint32_t)(xs ? cs.bytes_x : cs.bytes_y));
-        qb.sa = xs ? cs.sa_x : cs.sa_y;
-        qb.sz = xs ? cs.sz_x : cs.sz_y;
+        if constexpr (AP) {
+            const ApairShape cs = rs_apair_shape(nx, ny, nz);
+            qb.r = buffer_rsrc(xs ? qx : qy, (uint32_t)(xs ? cs.bytes_x : cs.bytes_y));
+            qb.sa = xs ? cs.sa_x : cs.sa_y;
+            qb.sz = xs ? cs.sz_x : cs.sz_y;
+        } else {
+            const CopyShape cs = rs_copy_shape(nx, ny, nz, sizeof(TV));
+            qb.r = buffer_rsrc(xs ? qx : qy, (uint32_t)(xs ? cs.bytes_x : cs.bytes_y));
+            qb.sa = xs ? cs.sa_x : cs.sa_y;
+            qb.sz = xs ? cs.sz_x : cs.sz_y;
+        }
     }
-    if (mode >= 0 && mode < 2) acc = rs_samples<0, TV, BUF>(r, qx, qb, nx, ny, nz, k0, kend);
-    else acc = rs_samples<1, TV, BUF>(r, qy, qb, nx, ny, nz, k0, kend);
+    if (mode >= 0 && mode < 2) acc = rs_samples<0, TV, BUF, AP>(r, qx, qb, nx, ny, nz, k0, kend);
+    else acc = rs_samples<1, TV, BUF, AP>(r, qy, qb, nx, ny, nz, k0, kend);
     if (sizeof(TV) == 2) acc *= vol_inv;
     if (inside) {
         const size_t o = (size_t)m + (size_t)g.nu * ((size_t)n + (size_t)g.nv * p);
@@ -339,11 +400,31 @@ __device__ __forceinline__ void rs_tile(const DevGeom& g, int cube, const void* 
     }
 }
 
-template <typename TV, typename TS, bool BUF>
+template <typename TV, typename TS, bool BUF, bool AP>
 __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const void* __restrict__ qx,
                                                   const void* __restrict__ qy, float vol_inv, TS* __restrict__ sino,
                                                   float sino_scale) {
-    rs_tile<TV, TS, BUF>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
+    rs_tile<TV, TS, BUF, AP>(g, cube, qx, qy, vol_inv, sino, sino_scale, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+// The gather form's launch: A-pair copies (16-bit volumes) when ap, else the
+// z-pair copies through the buffer path (buf) or 64-bit indices.
+template <typename TV, typename TS>
+void launch_rs_gathers(bool ap, bool buf, dim3 grid, hipStream_t s, const DevGeom& d, int cube, const void* cx,
+                       const void* cy, float inv, TS* sino, float sino_scale) {
+    if constexpr (sizeof(TV) == 2) {
+        if (ap) {
+            hipLaunchKernelGGL((rs_kernel<TV, TS, true, true>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv, sino,
+                               sino_scale);
+            return;
+        }
+    }
+    if (buf)
+        hipLaunchKernelGGL((rs_kernel<TV, TS, true, false>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv, sino,
+                           sino_scale);
+    else
+        hipLaunchKernelGGL((rs_kernel<TV, TS, false, false>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv, sino,
+                           sino_scale);
 }
 
 // ---------------------------------------------------------------------------
@@ -725,20 +806,18 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s,
     }
     const float inv = 1.0f / vol.scale;
     const uint32_t es = vol.dtype == TOMO_F32 ? 4u : 2u;
+    // 16-bit volumes whose A-pair copies fit the buffer path take one load per
+    // sample from them (the taps are the same numbers)
+    const bool ap = es == 2 && !x.index_reads && rs_apair_shape(d.nx, d.ny, d.nz).buf;
     ZPairs zp;
-    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s, x);
+    hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s, x, ap);
     const void* cx = zp.qx;
     const void* cy = zp.qy;
     if (le == hipSuccess) {
         const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, es).buf && !x.index_reads;
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
-                if (buf)
-                    hipLaunchKernelGGL((rs_kernel<TV, TS, true>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv,
-                                       (TS*)sino.p, sino.scale);
-                else
-                    hipLaunchKernelGGL((rs_kernel<TV, TS, false>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv,
-                                       (TS*)sino.p, sino.scale);
+                launch_rs_gathers<TV, TS>(ap, buf, grid, s, d, cube, cx, cy, inv, (TS*)sino.p, sino.scale);
             });
         });
         le = hipGetLastError();

@@ -310,6 +310,63 @@ __global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, 
     zpair_x_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
 }
 
+// One A-pair group (four 16-bit elements at an index that is a multiple of
+// four) as one aligned 8-byte store.
+__device__ __forceinline__ void store_group(uint16_t* __restrict__ out, size_t o, uint16_t a, uint16_t b, uint16_t c,
+                                            uint16_t d) {
+    typedef uint16_t U4 __attribute__((ext_vector_type(4)));
+    U4 v;
+    v.x = a;
+    v.y = b;
+    v.z = c;
+    v.w = d;
+    *reinterpret_cast<U4*>(out + o) = v;
+}
+
+__device__ __forceinline__ uint16_t voxel16(const uint16_t* __restrict__ in, int x, int y, int z, int nx, int ny,
+                                            int nz) {
+    return (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
+               ? in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]
+               : (uint16_t)0;
+}
+
+// A-pair copy for x slabs (A = x, B = y): a tile of 32 plane pairs x 32 B
+// columns of one z pair through LDS (the input runs along x, the copy along
+// y).  Grid (ceil((nx+1)/32), ceil((ny+2)/32), nz+1).
+__global__ __launch_bounds__(256) void apair_x_kernel(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                                    int nx, int ny, int nz) {
+    __shared__ uint16_t tile[2][32][34];  // [z - (zz-1)][bb - b0][x - (ap0-1)]
+    const int ap0 = blockIdx.x * 32, b0 = blockIdx.y * 32, zz = blockIdx.z;
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    for (int r = ty; r < 32; r += 8) {
+        const int y = b0 + r - 1;
+        for (int c = tx; c < 34; c += 32) {
+            tile[0][r][c] = voxel16(in, ap0 - 1 + c, y, zz - 1, nx, ny, nz);
+            tile[1][r][c] = voxel16(in, ap0 - 1 + c, y, zz, nx, ny, nz);
+        }
+    }
+    __syncthreads();
+    for (int r = ty; r < 32; r += 8) {
+        const int ap = ap0 + r, bb = b0 + tx;
+        if (ap <= nx && bb <= ny + 1)
+            store_group(out, apair_index(ap, zz, bb, nz, ny), tile[0][tx][r], tile[1][tx][r], tile[0][tx][r + 1],
+                        tile[1][tx][r + 1]);
+    }
+}
+
+// A-pair copy for y slabs (A = y, B = x): B is the input's fastest axis, a
+// direct pass.  Grid (ceil((nx+2)/256), ny+1, nz+1): group (blockIdx.y,
+// blockIdx.z, bb).
+__global__ __launch_bounds__(256) void apair_y_kernel(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                                    int nx, int ny, int nz) {
+    const int bb = blockIdx.x * 256 + threadIdx.x;
+    if (bb > nx + 1) return;
+    const int x = bb - 1, ap = blockIdx.y, zz = blockIdx.z;
+    store_group(out, apair_index(ap, zz, bb, nz, nx), voxel16(in, x, ap - 1, zz - 1, nx, ny, nz),
+                voxel16(in, x, ap - 1, zz, nx, ny, nz), voxel16(in, x, ap, zz - 1, nx, ny, nz),
+                voxel16(in, x, ap, zz, nx, ny, nz));
+}
+
 // ---- shared launches of the gradient loop (FP32).  Element-wise tiles
 // cover kVecTile consecutive elements, 16 per thread at stride 256; FP64
 // partial sums are per tile, in that fixed order, so a problem's sums do not
@@ -527,6 +584,14 @@ unsigned grid_for(size_t n) {
 }
 
 }  // namespace
+
+hipError_t launch_apair(const void* vol, void* qx, void* qy, int nx, int ny, int nz, hipStream_t s) {
+    const dim3 gx(ceil_div((size_t)nx + 1, 32), ceil_div((size_t)ny + 2, 32), nz + 1);
+    const dim3 gy(ceil_div((size_t)nx + 2, 256), ny + 1, nz + 1);
+    hipLaunchKernelGGL(apair_x_kernel, gx, dim3(256), 0, s, (const uint16_t*)vol, (uint16_t*)qx, nx, ny, nz);
+    hipLaunchKernelGGL(apair_y_kernel, gy, dim3(256), 0, s, (const uint16_t*)vol, (uint16_t*)qy, nx, ny, nz);
+    return hipGetLastError();
+}
 
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s) {
     const dim3 gx(ceil_div(nx, 32), ceil_div((size_t)ny + 2, 32), nz + 1);

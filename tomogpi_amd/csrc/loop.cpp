@@ -44,7 +44,8 @@ struct PoolBuf {
     }
 };
 
-// The projector's two z-pair copies, allocated once for a loop's run and
+// The projector's two copies (z-pair, or A-pair for RS on 16-bit volumes:
+// RS loops size them for the larger), allocated once for a loop's run and
 // passed to each of its projections (which otherwise take them from the
 // stream-ordered pool per call: two volume-sized allocations per projection,
 // whose pool allocations stalled the host for up to seconds now and then
@@ -61,8 +62,9 @@ struct LoopCopies {
              zpair_elems_y(nx, ny, nz) >= (1ull << 32)))
             return TOMO_OK;
         const size_t es = storage == TOMO_F32 ? 4 : 2;
-        TOMO_HIP(qx.alloc(zpair_elems_x(nx, ny, nz) * es, s));
-        TOMO_HIP(qy.alloc(zpair_elems_y(nx, ny, nz) * es, s));
+        const bool rs = pr.pair != TOMO_PAIR_SIDDONVI;
+        TOMO_HIP(qx.alloc(rs ? proj_copy_bytes_x(nx, ny, nz, es) : zpair_elems_x(nx, ny, nz) * es, s));
+        TOMO_HIP(qy.alloc(rs ? proj_copy_bytes_y(nx, ny, nz, es) : zpair_elems_y(nx, ny, nz) * es, s));
         extra.qx = qx.p;
         extra.qy = qy.p;
         return TOMO_OK;

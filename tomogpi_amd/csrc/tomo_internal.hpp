@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstdint>
 #include <cstddef>
 #include <string>
@@ -69,6 +70,21 @@ namespace tomo {
 // slabs (qy): nA * (nz+1) * (nB+2) pairs.
 __host__ __device__ inline size_t zpair_elems_x(int nx, int ny, int nz) { return (size_t)nx * (nz + 1) * (ny + 2) * 2; }
 __host__ __device__ inline size_t zpair_elems_y(int nx, int ny, int nz) { return (size_t)ny * (nz + 1) * (nx + 2) * 2; }
+// Element counts of the A-pair copies (16-bit volumes, RS): (nA+1) * (nz+1)
+// * (nB+2) groups of four elements (zpair.hpp).
+__host__ __device__ inline size_t apair_elems_x(int nx, int ny, int nz) { return (size_t)(nx + 1) * (nz + 1) * (ny + 2) * 4; }
+__host__ __device__ inline size_t apair_elems_y(int nx, int ny, int nz) { return (size_t)(ny + 1) * (nz + 1) * (nx + 2) * 4; }
+// Bytes of the largest copy any projector form reads for slabs along x (y)
+// from a volume of es-byte elements: what a caller that hands copies to every
+// projection (ProjExtra) allocates.
+inline size_t proj_copy_bytes_x(int nx, int ny, int nz, size_t es) {
+    const size_t z = zpair_elems_x(nx, ny, nz) * es;
+    return es == 2 ? std::max(z, apair_elems_x(nx, ny, nz) * es) : z;
+}
+inline size_t proj_copy_bytes_y(int nx, int ny, int nz, size_t es) {
+    const size_t z = zpair_elems_y(nx, ny, nz) * es;
+    return es == 2 ? std::max(z, apair_elems_y(nx, ny, nz) * es) : z;
+}
 
 // Derives the geometry exactly as the reference constructors do.
 int derive_geometry(const tomo_geom_desc& d, tomo_geom* g);
@@ -92,10 +108,10 @@ int launch_vi(const tomo_geom* g, Buf sino, MutBuf vol, hipStream_t s, int fdk =
 int launch_vi_adjoint(const tomo_geom* g, Buf vol, MutBuf sino, hipStream_t s);
 // Projector options beyond the FOV / compat mode.  index_reads: read the
 // z-pair copies through 64-bit element indices even where 32-bit buffer
-// offsets apply (the form every copy of 4 GiB or more takes).  qx, qy: z-pair
-// copies in memory the caller provides (zpair_elems_x / _y elements of the
-// volume's element size each); null: the launcher allocates them from the
-// stream-ordered pool for the call.
+// offsets apply (the form every copy of 4 GiB or more takes).  qx, qy: the
+// projector's copies in memory the caller provides (proj_copy_bytes_x / _y
+// bytes each, for the volume's element size); null: the launcher allocates
+// them from the stream-ordered pool for the call.
 struct ProjExtra {
     bool index_reads = false;
     bool staged = false;  // RS, Siddon (FP32 volumes): LDS-staged taps instead of per-lane gathers

@@ -10,6 +10,14 @@
 // for any corner b in [-1, nB-1], z in [-1, nz-1], with the voxels outside
 // the grid already 0.  qx (A = x) and qy (A = y) each hold
 // nA * (nz+1) * (nB+2) * 2 elements, about twice the volume.
+//
+// The A-pair copies of a 16-bit volume (read by the RS projector, whose
+// samples interpolate between two A planes) add the plane pair: group
+// (ap, zz, bb), ap in [0, nA], holds four elements,
+//   f(ap-1, bb-1, zz-1), f(ap-1, bb-1, zz), f(ap, bb-1, zz-1), f(ap, bb-1, zz)
+// (planes -1 and nA are 0), B varying fastest; two neighbouring groups are the
+// eight taps of a sample with lower corner (ap-1, bb-1, zz-1), one 16-byte
+// load.  Each copy holds (nA+1) * (nz+1) * (nB+2) * 4 elements.
 #pragma once
 
 #include "device_common.hpp"
@@ -20,8 +28,16 @@ __device__ __forceinline__ size_t zpair_index(int a, int zz, int bb, int nz, int
     return (((size_t)a * (nz + 1) + zz) * (nB + 2) + bb) * 2;
 }
 
+// First element of A-pair group (ap, zz, bb).
+__device__ __forceinline__ size_t apair_index(int ap, int zz, int bb, int nz, int nB) {
+    return (((size_t)ap * (nz + 1) + zz) * (nB + 2) + bb) * 4;
+}
+
 // Fills qx and qy from an x-fastest volume of es-byte elements (es 4 or 2).
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s);
+
+// Fills the A-pair copies qx and qy from an x-fastest 16-bit volume.
+hipError_t launch_apair(const void* vol, void* qx, void* qy, int nx, int ny, int nz, hipStream_t s);
 
 // out[y + nyp (x + nx z)] = vol[x + nx (y + ny z)] for y < ny and 0 for
 // ny <= y < nyp (nyp >= ny), FP32: the volume with y fastest and rows padded
@@ -66,24 +82,28 @@ struct StageSources {
 
 // Both copies for the duration of one operator call, filled from the volume:
 // alloc() then release() on the same stream.  The copies come from the
-// stream-ordered pool, or are the caller's (ProjExtra::qx, qy), which
-// release() leaves alone.
+// stream-ordered pool, or are the caller's (ProjExtra::qx, qy, sized for the
+// largest form), which release() leaves alone.  apair: the A-pair form
+// (16-bit volumes) instead of the z-pair form.
 struct ZPairs {
     void* qx = nullptr;
     void* qy = nullptr;
     bool owned = false;
     hipError_t alloc(const void* vol, size_t es, int nx, int ny, int nz, hipStream_t s,
-                     const ProjExtra& x = ProjExtra{}) {
+                     const ProjExtra& x = ProjExtra{}, bool apair = false) {
         hipError_t e = hipSuccess;
         if (x.qx && x.qy) {
             qx = x.qx;
             qy = x.qy;
         } else {
             owned = true;
-            e = hipMallocAsync(&qx, zpair_elems_x(nx, ny, nz) * es, s);
-            if (e == hipSuccess) e = hipMallocAsync(&qy, zpair_elems_y(nx, ny, nz) * es, s);
+            const size_t bx = (apair ? apair_elems_x(nx, ny, nz) : zpair_elems_x(nx, ny, nz)) * es;
+            const size_t by = (apair ? apair_elems_y(nx, ny, nz) : zpair_elems_y(nx, ny, nz)) * es;
+            e = hipMallocAsync(&qx, bx, s);
+            if (e == hipSuccess) e = hipMallocAsync(&qy, by, s);
         }
-        if (e == hipSuccess) e = launch_zpair(vol, qx, qy, es, nx, ny, nz, s);
+        if (e == hipSuccess)
+            e = apair ? launch_apair(vol, qx, qy, nx, ny, nz, s) : launch_zpair(vol, qx, qy, es, nx, ny, nz, s);
         return e;
     }
     hipError_t release(hipStream_t s) {
