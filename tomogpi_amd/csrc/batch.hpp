@@ -19,10 +19,9 @@
 // from counter x the tiles t = x (mod kBatchXcds), in order.  A single-problem
 // launch puts the same tiles on the same XCD (tile t is its workgroup t, up to
 // a rotation by the problem's first tile), so a detector column group meets
-// the same XCD's L2 at every view, as it does there; one shared counter sent
-// each tile to whichever XCD had a workgroup free, and the projectors' shared
-// launches ran about a fifth slower than their single-problem launches on the
-// same problems.  Because the grid and the table address
+// the same XCD's L2 at every view, as it does there (one shared counter, which
+// sends a tile to whichever XCD has a workgroup free, measured the same
+// time).  Because the grid and the table address
 // are fixed, a captured hipGraph of shared launches stays valid while the
 // table contents change between replays; the host's copy of the table before
 // each launch sequence resets the counter, so a table serves one launch per
@@ -89,6 +88,9 @@ static_assert(kBatchGrid % kBatchXcds == 0, "every XCD starts with the same numb
 // BF16, converted from the FP32 state by the copy launches.  RS keeps bit 0
 // for the cube FOV.
 constexpr int32_t kItemBF16 = 0x10;
+// Item flag bit of the BF16 copy and RS launches: the copies are A-pair
+// copies (zpair.hpp), read with one load per RS sample.
+constexpr int32_t kItemApair = 0x20;
 
 // Per-problem scalars of the gradient loop, FP64, in `slot`.
 enum GdSlot {

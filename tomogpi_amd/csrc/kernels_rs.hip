@@ -214,10 +214,9 @@ __host__ __device__ __forceinline__ ApairShape rs_apair_shape(int nx, int ny, in
     c.sz_y = (uint32_t)(nx + 2) * 8u;
     c.bytes_x = apair_elems_x(nx, ny, nz) * 2u;
     c.bytes_y = apair_elems_y(nx, ny, nz) * 2u;
-    const uint64_t sax = (uint64_t)c.sz_x * (uint64_t)(nz + 1), say = (uint64_t)c.sz_y * (uint64_t)(nz + 1);
-    c.sa_x = (uint32_t)sax;
-    c.sa_y = (uint32_t)say;
-    c.buf = c.bytes_x < kOutside && c.bytes_y < kOutside && sax < (1ull << 24) && say < (1ull << 24);
+    c.sa_x = c.sz_x * (uint32_t)(nz + 1);
+    c.sa_y = c.sz_y * (uint32_t)(nz + 1);
+    c.buf = apair_buffer_path(nx, ny, nz);
     return c;
 }
 
@@ -717,8 +716,9 @@ __global__ __launch_bounds__(BX* BY, 5) void rs_staged_kernel(DevGeom g, int cub
     }
 }
 
-// Shared launch (FP32 or BF16 (kItemBF16) z-pair copies, FP32 sinograms);
-// item flags bit 0 selects the cube FOV.  No minimum-waves bound: under one
+// Shared launch (FP32 or BF16 (kItemBF16) z-pair copies, or BF16 A-pair
+// copies (kItemApair), FP32 sinograms); item flags bit 0 selects the cube
+// FOV.  No minimum-waves bound: under one
 // (8 or 6 waves per SIMD, 52 or 74 VGPRs) the compiler waits for each
 // sample's two gathers before it issues the next sample's, where unbounded
 // (86 VGPRs, 5 waves per SIMD) it issues the eight gathers of four samples
@@ -734,7 +734,9 @@ __global__ __launch_bounds__(BX* BY) void rs_batch_kernel(const BatchTable* __re
         const bool bf = (it.flags & kItemBF16) != 0;
         const bool buf = rs_copy_shape(it.g.nx, it.g.ny, it.g.nz, bf ? 2u : 4u).buf;
         using BF = __hip_bfloat16;
-        if (bf && buf)
+        if (it.flags & kItemApair)
+            rs_tile<BF, float, true, true>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
+        else if (bf && buf)
             rs_tile<BF, float, true>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);
         else if (bf)
             rs_tile<BF, float, false>(it.g, cube, it.in, it.in2, 1.0f, (float*)it.out, 1.0f, (int)bx, by, bz);

@@ -323,20 +323,23 @@ __device__ __forceinline__ void store_group(uint16_t* __restrict__ out, size_t o
     *reinterpret_cast<U4*>(out + o) = v;
 }
 
-__device__ __forceinline__ uint16_t voxel16(const uint16_t* __restrict__ in, int x, int y, int z, int nx, int ny,
-                                            int nz) {
+// A voxel of an x-fastest volume as a 16-bit copy element (0 outside the
+// grid): a bit copy of a 16-bit volume, or an FP32 voxel rounded to BF16.
+template <typename TI>
+__device__ __forceinline__ uint16_t voxel16(const TI* __restrict__ in, int x, int y, int z, int nx, int ny, int nz) {
     return (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-               ? in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)]
+               ? copy_cvt<uint16_t>(in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)])
                : (uint16_t)0;
 }
 
 // A-pair copy for x slabs (A = x, B = y): a tile of 32 plane pairs x 32 B
 // columns of one z pair through LDS (the input runs along x, the copy along
 // y).  Grid (ceil((nx+1)/32), ceil((ny+2)/32), nz+1).
-__global__ __launch_bounds__(256) void apair_x_kernel(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
-                                                    int nx, int ny, int nz) {
-    __shared__ uint16_t tile[2][32][34];  // [z - (zz-1)][bb - b0][x - (ap0-1)]
-    const int ap0 = blockIdx.x * 32, b0 = blockIdx.y * 32, zz = blockIdx.z;
+template <typename TI>
+__device__ __forceinline__ void apair_x_tile(const TI* __restrict__ in, uint16_t* __restrict__ out, int nx, int ny,
+                                             int nz, int bx, int by, int zz, uint16_t (*tile)[32][34]) {
+    // tile[z - (zz-1)][bb - b0][x - (ap0-1)]
+    const int ap0 = bx * 32, b0 = by * 32;
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
     for (int r = ty; r < 32; r += 8) {
         const int y = b0 + r - 1;
@@ -355,16 +358,28 @@ __global__ __launch_bounds__(256) void apair_x_kernel(const uint16_t* __restrict
 }
 
 // A-pair copy for y slabs (A = y, B = x): B is the input's fastest axis, a
-// direct pass.  Grid (ceil((nx+2)/256), ny+1, nz+1): group (blockIdx.y,
-// blockIdx.z, bb).
-__global__ __launch_bounds__(256) void apair_y_kernel(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
-                                                    int nx, int ny, int nz) {
-    const int bb = blockIdx.x * 256 + threadIdx.x;
+// direct pass.  Grid (ceil((nx+2)/256), ny+1, nz+1): group (ap, zz, bb) with
+// bb = 256 bx + thread.
+template <typename TI>
+__device__ __forceinline__ void apair_y_tile(const TI* __restrict__ in, uint16_t* __restrict__ out, int nx, int ny,
+                                             int nz, int bx, int ap, int zz) {
+    const int bb = bx * 256 + threadIdx.x;
     if (bb > nx + 1) return;
-    const int x = bb - 1, ap = blockIdx.y, zz = blockIdx.z;
+    const int x = bb - 1;
     store_group(out, apair_index(ap, zz, bb, nz, nx), voxel16(in, x, ap - 1, zz - 1, nx, ny, nz),
                 voxel16(in, x, ap - 1, zz, nx, ny, nz), voxel16(in, x, ap, zz - 1, nx, ny, nz),
                 voxel16(in, x, ap, zz, nx, ny, nz));
+}
+
+__global__ __launch_bounds__(256) void apair_x_kernel(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                                    int nx, int ny, int nz) {
+    __shared__ uint16_t tile[2][32][34];
+    apair_x_tile<uint16_t>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
+}
+
+__global__ __launch_bounds__(256) void apair_y_kernel(const uint16_t* __restrict__ in, uint16_t* __restrict__ out,
+                                                    int nx, int ny, int nz) {
+    apair_y_tile<uint16_t>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
 }
 
 // ---- shared launches of the gradient loop (FP32).  Element-wise tiles
@@ -388,15 +403,20 @@ __device__ __forceinline__ void batch_coords(const BatchItem& it, uint32_t tile,
     bz = (int)(rest / it.gy);
 }
 
-// z-pair copies of FP32 volumes: FP32 (bit copy) or BF16 (kItemBF16).
+// z-pair copies of FP32 volumes: FP32 (bit copy) or BF16 (kItemBF16), or
+// BF16 A-pair copies (kItemApair, the item's grid is then the A-pair one).
 __global__ __launch_bounds__(256) void zpair_x_batch_kernel(const BatchTable* __restrict__ t) {
     __shared__ uint32_t tile_lds[2][32][33];
     __shared__ uint16_t tile_lds16[2][32][33];
+    __shared__ uint16_t tile_ap[2][32][34];
     TOMO_BATCH_STRIDED(t) {
         const BatchItem& it = t->item[batch_find(t, tile)];
         int bx, by, bz;
         batch_coords(it, tile, bx, by, bz);
-        if (it.flags & kItemBF16)
+        if (it.flags & kItemApair)
+            apair_x_tile<float>((const float*)it.in, (uint16_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by, bz,
+                                tile_ap);
+        else if (it.flags & kItemBF16)
             zpair_x_tile<float, uint16_t>((const float*)it.in, (uint16_t*)it.out, it.g.nx, it.g.ny, it.g.nz, bx, by,
                                           bz, tile_lds16);
         else
@@ -417,7 +437,9 @@ __global__ __launch_bounds__(256) void zpair_y_batch_kernel(const BatchTable* __
         batch_coords(it, tile, bx, by, bz);
         const int nx = it.g.nx, ny = it.g.ny, nz = it.g.nz;
         for (int c = 0; c * 256 < nx + 2; ++c) {
-            if (it.flags & kItemBF16)
+            if (it.flags & kItemApair)  // by: the plane pair
+                apair_y_tile<float>((const float*)it.in, (uint16_t*)it.out, nx, ny, nz, c, by, bz);
+            else if (it.flags & kItemBF16)
                 zpair_y_tile<float, uint16_t>((const float*)it.in, (uint16_t*)it.out, nx, ny, nz, c, by, bz);
             else
                 zpair_y_tile<uint32_t, uint32_t>((const uint32_t*)it.in, (uint32_t*)it.out, nx, ny, nz, c, by, bz);

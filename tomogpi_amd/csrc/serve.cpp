@@ -129,6 +129,13 @@ namespace {
 uint32_t vec_tiles(uint64_t n) { return cdiv(n, kVecTile); }
 uint32_t lap_tiles(const DevGeom& d) { return cdiv(d.nx, 64) * cdiv(d.ny, 4) * (uint32_t)d.nz; }
 
+// Whether a request's operator copies are A-pair copies: BF16 copies (ces 2)
+// of an RS request whose A-pair copies take the buffer path (one load per
+// sample); otherwise z-pair copies (FP32 or BF16).
+bool apair_copies(const DevGeom& d, const tomo_gd_params& gd, size_t ces) {
+    return ces == 2 && gd.pair != TOMO_PAIR_SIDDONVI && apair_buffer_path(d.nx, d.ny, d.nz);
+}
+
 Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo, size_t ces) {
     const DevGeom& d = g.dev;
     const size_t nv = g.nvox(), ns = g.nsino();
@@ -143,8 +150,9 @@ Layout layout_of(const tomo_geom& g, const tomo_gd_params& gd, int algo, size_t 
     sizes[i++] = (lam || sirt) ? nv * 4 : 0;                             // 4 t1 (SIRT: volume weight C)
     sizes[i++] = lam ? nv * 4 : 0;                                       // 5 t2
     sizes[i++] = lam ? nv * 4 : 0;                                       // 6 t3
-    sizes[i++] = zpair_elems_x(d.nx, d.ny, d.nz) * ces;                  // 7 qx (FP32 or BF16)
-    sizes[i++] = zpair_elems_y(d.nx, d.ny, d.nz) * ces;                  // 8 qy
+    const bool ap = apair_copies(d, gd, ces);
+    sizes[i++] = (ap ? apair_elems_x(d.nx, d.ny, d.nz) : zpair_elems_x(d.nx, d.ny, d.nz)) * ces;  // 7 qx
+    sizes[i++] = (ap ? apair_elems_y(d.nx, d.ny, d.nz) : zpair_elems_y(d.nx, d.ny, d.nz)) * ces;  // 8 qy
     sizes[i++] = ns * 4;                                                 // 9 y
     sizes[i++] = ns * 4;                                                 // 10 px
     sizes[i++] = ns * 4;                                                 // 11 r
@@ -281,13 +289,15 @@ void build_tables(tomo_server* srv, int set, const std::vector<Request*>& runnin
         const uint32_t pgx = cdiv(d.nu, kProjTileU), pgz = cdiv(d.nv, kProjTileV);
         const bool sid = gd.pair == TOMO_PAIR_SIDDONVI;
         const int cube = (gd.flags & TOMO_FLAG_RS_FOV_CUBE) ? 1 : 0;
-        const int32_t cflag = q.bf16 ? kItemBF16 : 0;  // operator copies in BF16
+        const bool ap = apair_copies(d, gd, q.bf16 ? 2 : 4);
+        const int32_t cflag = (q.bf16 ? kItemBF16 : 0) | (ap ? kItemApair : 0);  // operator copies
         auto add_projection = [&](const float* vol, float* out, int opx, int opy, int ops, int opr) {
-            BatchItem& zx = tb[opx].add(d, cdiv(d.nx, 32), cdiv(d.ny + 2, 32), d.nz + 1);
+            BatchItem& zx = ap ? tb[opx].add(d, cdiv(d.nx + 1, 32), cdiv(d.ny + 2, 32), d.nz + 1)
+                               : tb[opx].add(d, cdiv(d.nx, 32), cdiv(d.ny + 2, 32), d.nz + 1);
             zx.in = vol;
             zx.out = q.qx;
             zx.flags = cflag;
-            BatchItem& zy = tb[opy].add(d, 1, d.ny, d.nz + 1);  // whole rows
+            BatchItem& zy = tb[opy].add(d, 1, ap ? d.ny + 1 : d.ny, d.nz + 1);  // whole rows
             zy.in = vol;
             zy.out = q.qy;
             zy.flags = cflag;

@@ -74,6 +74,16 @@ __host__ __device__ inline size_t zpair_elems_y(int nx, int ny, int nz) { return
 // * (nB+2) groups of four elements (zpair.hpp).
 __host__ __device__ inline size_t apair_elems_x(int nx, int ny, int nz) { return (size_t)(nx + 1) * (nz + 1) * (ny + 2) * 4; }
 __host__ __device__ inline size_t apair_elems_y(int nx, int ny, int nz) { return (size_t)(ny + 1) * (nz + 1) * (nx + 2) * 4; }
+// Whether RS reads the A-pair copies through 32-bit buffer offsets: each copy
+// below 2^32 - 16 bytes (the offset of a read that returns zeros lies past
+// it) and its plane stride below 2^24 bytes (24-bit multiplies); otherwise
+// 16-bit RS reads the z-pair copies.
+__host__ __device__ inline bool apair_buffer_path(int nx, int ny, int nz) {
+    const uint64_t plane_x = (uint64_t)(ny + 2) * 8u * (uint64_t)(nz + 1);
+    const uint64_t plane_y = (uint64_t)(nx + 2) * 8u * (uint64_t)(nz + 1);
+    return apair_elems_x(nx, ny, nz) * 2u < 0xfffffff0ull && apair_elems_y(nx, ny, nz) * 2u < 0xfffffff0ull &&
+           plane_x < (1ull << 24) && plane_y < (1ull << 24);
+}
 // Bytes of the largest copy any projector form reads for slabs along x (y)
 // from a volume of es-byte elements: what a caller that hands copies to every
 // projection (ProjExtra) allocates.
