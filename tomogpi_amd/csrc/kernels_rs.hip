@@ -198,6 +198,46 @@ __device__ __forceinline__ float trilinear_quads_buf(const CopyBuf& q, int nx, i
     return trilinear_taps<A>(u0, u1, u2, u3, w0, w1, w2, w3, fx, fy, fz);
 }
 
+// A-pair sampler through 64-bit element indices (16-bit volumes whose A-pair
+// copies are 4 GiB or more): one 16-byte load per sample, as below; a sample
+// outside the grid or whose block is returns 0, as trilinear_quads does.
+template <typename TV>
+__device__ __forceinline__ void apair_unpack(const uint32_t (&d)[4], float& u0, float& u1, float& u2, float& u3,
+                                             float& w0, float& w1, float& w2, float& w3) {
+    using U = Unpack16<TV>;
+    // d[0], d[1] = (a, b, z), (a, b, z+1), (a+1, b, z), (a+1, b, z+1); d[2], d[3]: the same at b + 1
+    u0 = U::lo(d[0]); u1 = U::hi(d[0]); w0 = U::lo(d[1]); w1 = U::hi(d[1]);
+    u2 = U::lo(d[2]); u3 = U::hi(d[2]); w2 = U::lo(d[3]); w3 = U::hi(d[3]);
+    if constexpr (std::is_same<TV, __half>::value) {
+        // Keep the FP16 taps as converted FP32 registers: otherwise the
+        // compiler folds their conversions into v_fma_mix_f32 operands, the
+        // same numbers at more issue time (RS at 512^3 / 512 views: 81.4 ms
+        // folded, 76.3 ms not).  An empty statement: it emits no instruction.
+        asm("" : "+v"(u0), "+v"(u1), "+v"(u2), "+v"(u3), "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3));
+    }
+}
+
+template <int A, typename TV>
+__device__ __forceinline__ float trilinear_apair(const TV* __restrict__ q, int nx, int ny, int nz, float x, float y,
+                                                 float z) {
+    static_assert(sizeof(TV) == 2, "A-pair copies hold 16-bit volumes");
+    const float xf = floorf(x), yf = floorf(y), zf = floorf(z);
+    if (!(xf >= -1.0f && xf <= (float)nx && yf >= -1.0f && yf <= (float)ny && zf >= -1.0f && zf <= (float)nz))
+        return 0.0f;
+    const int ix = (int)xf, iy = (int)yf, iz = (int)zf;
+    const float fx = x - xf, fy = y - yf, fz = z - zf;
+    const int ia = A == 0 ? ix : iy, ib = A == 0 ? iy : ix;
+    const int nA = A == 0 ? nx : ny, nB = A == 0 ? ny : nx;
+    if (!(ib < nB && iz < nz && ia < nA)) return 0.0f;  // every tap 0: the blend gives +0
+    const unsigned char* p =
+        (const unsigned char*)__builtin_assume_aligned(q + apair_index(ia + 1, iz + 1, ib + 1, nz, nB), 8);
+    uint32_t d[4];
+    __builtin_memcpy(d, p, 16);
+    float u0, u1, u2, u3, w0, w1, w2, w3;
+    apair_unpack<TV>(d, u0, u1, u2, u3, w0, w1, w2, w3);
+    return trilinear_taps<A>(u0, u1, u2, u3, w0, w1, w2, w3, fx, fy, fz);
+}
+
 // A-pair sampler (16-bit volumes, copies below 4 GiB): the eight taps are two
 // neighbouring groups of the A-pair copy (zpair.hpp), one 16-byte load where
 // the z-pair copy takes two.  Its planes -1 and nA are zeros of the copy, so
@@ -241,18 +281,10 @@ __device__ __forceinline__ float trilinear_apair_buf(const CopyBuf& q, int nx, i
     // samples and serialises their loads
     const uint32_t keep = 0u - (uint32_t)blk;
     const uint32_t o = ((__umul24(ap, q.sa) + __umul24(zq, q.sz) + bq * 8u) & keep) | (kOutside & ~keep);
-    const auto d = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o, 0, 0);
-    using U = Unpack16<TV>;
-    // d[0], d[1] = (a, b, z), (a, b, z+1), (a+1, b, z), (a+1, b, z+1); d[2], d[3]: the same at b + 1
-    float u0 = U::lo(d[0]), u1 = U::hi(d[0]), w0 = U::lo(d[1]), w1 = U::hi(d[1]);
-    float u2 = U::lo(d[2]), u3 = U::hi(d[2]), w2 = U::lo(d[3]), w3 = U::hi(d[3]);
-    if constexpr (std::is_same<TV, __half>::value) {
-        // Keep the FP16 taps as converted FP32 registers: otherwise the
-        // compiler folds their conversions into v_fma_mix_f32 operands, the
-        // same numbers at more issue time (RS at 512^3 / 512 views: 81.4 ms
-        // folded, 76.3 ms not).  An empty statement: it emits no instruction.
-        asm("" : "+v"(u0), "+v"(u1), "+v"(u2), "+v"(u3), "+v"(w0), "+v"(w1), "+v"(w2), "+v"(w3));
-    }
+    const auto v = __builtin_amdgcn_raw_buffer_load_b128(q.r, (int)o, 0, 0);
+    const uint32_t d[4] = {v[0], v[1], v[2], v[3]};
+    float u0, u1, u2, u3, w0, w1, w2, w3;
+    apair_unpack<TV>(d, u0, u1, u2, u3, w0, w1, w2, w3);
     return trilinear_taps<A>(u0, u1, u2, u3, w0, w1, w2, w3, fx, fy, fz);
 }
 
@@ -274,8 +306,8 @@ __device__ __forceinline__ float wave_minf(float v) {
 // s = k - k0 for k0 <= k < k0 + lr.  Same positions (fma(s, d, entry)) and
 // the same order of the sum as the per-ray loop.
 // q is the z-pair copy of A (FP32 or 16-bit, as stored); BUF reads it
-// through qb (see the buffer-path samplers), AP (16-bit, BUF) the A-pair copy
-// of A through qb instead.  The buffer path
+// through qb (see the buffer-path samplers); AP (16-bit volumes) reads the
+// A-pair copy of A instead, through qb or 64-bit indices.  The buffer path
 // has no branch: RS_UNROLL samples per iteration, their loads in flight
 // together, an inactive sample (s outside [0, lr)) adding +0 in its place,
 // which leaves the sum unchanged (it starts at +0 and is never -0).
@@ -307,7 +339,8 @@ __device__ __forceinline__ float rs_samples(const RsRay& r, const void* __restri
             if (s >= 0 && s < r.lr) {
                 const float sf = (float)s;
                 const float x = fmaf(sf, r.dx, r.ex), y = fmaf(sf, r.dy, r.ey), z = fmaf(sf, r.dz, r.ez);
-                acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
+                if constexpr (AP) acc += trilinear_apair<A, TV>((const TV*)q, nx, ny, nz, x, y, z);
+                else acc += trilinear_quads<A>((const TV*)q, nx, ny, nz, x, y, z, 1.0f);
             }
         }
     }
@@ -415,14 +448,18 @@ __global__ __launch_bounds__(BX* BY) void rs_kernel(DevGeom g, int cube, const v
 }
 
 // The gather form's launch: A-pair copies (16-bit volumes) when ap, else the
-// z-pair copies through the buffer path (buf) or 64-bit indices.
+// z-pair copies; through the buffer path when buf, else 64-bit indices.
 template <typename TV, typename TS>
 void launch_rs_gathers(bool ap, bool buf, dim3 grid, hipStream_t s, const DevGeom& d, int cube, const void* cx,
                        const void* cy, float inv, TS* sino, float sino_scale) {
     if constexpr (sizeof(TV) == 2) {
         if (ap) {
-            hipLaunchKernelGGL((rs_kernel<TV, TS, true, true>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv, sino,
-                               sino_scale);
+            if (buf)
+                hipLaunchKernelGGL((rs_kernel<TV, TS, true, true>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv,
+                                   sino, sino_scale);
+            else
+                hipLaunchKernelGGL((rs_kernel<TV, TS, false, true>), grid, dim3(BX, BY), 0, s, d, cube, cx, cy, inv,
+                                   sino, sino_scale);
             return;
         }
     }
@@ -816,15 +853,16 @@ int launch_rs(const tomo_geom* g, int cube, Buf vol, MutBuf sino, hipStream_t s,
     }
     const float inv = 1.0f / vol.scale;
     const uint32_t es = vol.dtype == TOMO_F32 ? 4u : 2u;
-    // 16-bit volumes whose A-pair copies fit the buffer path take one load per
-    // sample from them (the taps are the same numbers)
-    const bool ap = es == 2 && !x.index_reads && rs_apair_shape(d.nx, d.ny, d.nz).buf;
+    // 16-bit volumes take one load per sample from A-pair copies (the taps are
+    // the same numbers), through the buffer path where those fit it
+    const bool ap = es == 2;
     ZPairs zp;
     hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s, x, ap);
     const void* cx = zp.qx;
     const void* cy = zp.qy;
     if (le == hipSuccess) {
-        const bool buf = rs_copy_shape(d.nx, d.ny, d.nz, es).buf && !x.index_reads;
+        const bool buf = !x.index_reads && (ap ? apair_buffer_path(d.nx, d.ny, d.nz)
+                                               : rs_copy_shape(d.nx, d.ny, d.nz, es).buf);
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 launch_rs_gathers<TV, TS>(ap, buf, grid, s, d, cube, cx, cy, inv, (TS*)sino.p, sino.scale);
