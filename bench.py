@@ -527,7 +527,7 @@ def batch_leg(ctx, n_prob, iters=20, seed=7):
             "peak_active": st["peak_active"], "rounds": st["rounds"], "identical_to_one_at_a_time": same,
             "bf16_copies": {"batched_s": round(t_bf16, 3), "speedup_vs_one_at_a_time_fp32": round(t_seq / t_bf16, 3),
                             "max_rel_l2_vs_fp32_batched": float(f"{err16:.3e}"),
-                            "note": "BF16 z-pair copies rounded from the FP32 state; VI, sums and state FP32"}}
+                            "note": "BF16 operator copies (z-pair; A-pair for RS problems) rounded from the FP32 state; VI, sums and state FP32"}}
 
 
 def loop_leg(G, vol, sino, iters, upd, stream):
