@@ -421,19 +421,20 @@ def test_siddon_compat_bit_exact_coarse_detector(ctx):
 
 
 # ---- index readers (the form of copies of 4 GiB or more) ----------------
-@pytest.mark.parametrize("dt", ["f32", "bf16"])
+@pytest.mark.parametrize("dt", ["f32", "bf16", "f16"])
 @pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon"])
 def test_index_reads_bit_exact(setup, op, dt):
-    """TOMO_FLAG_INDEX_READS makes the projectors read their z-pair copies
-    through 64-bit element indices, as every 1024^3 volume does; the results
-    equal the oracle's (and the buffer readers') bit for bit."""
+    """TOMO_FLAG_INDEX_READS makes the projectors read their copies (z-pair;
+    A-pair for RS on 16-bit volumes) through 64-bit element indices, as every
+    1024^3 volume does; the results equal the oracle's (and the buffer
+    readers') bit for bit."""
     g, o, G = setup
     rng = np.random.default_rng(8)
     vol = ellipsoids(g.nx, g.ny, g.nz, count=12, seed=5, amin=2.0, amax=12.0)
     vol = vol + rng.random(g.vol_shape, dtype=np.float32)
     dv = dev(vol)
-    if dt == "bf16":
-        dv = dv.to(torch.bfloat16)
+    if dt != "f32":
+        dv = dv.to(torch.bfloat16 if dt == "bf16" else torch.float16)
         vol = dv.float().cpu().numpy()
     ref = o.siddon(vol) if op == "siddon" else o.rs(vol, cube=op == "rs_cube")
     flags = lib.FLAG_INDEX_READS | (lib.FLAG_RS_FOV_CUBE if op == "rs_cube" else 0)
