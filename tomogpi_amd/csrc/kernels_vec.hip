@@ -267,10 +267,35 @@ __device__ __forceinline__ void zpair_x_tile(const TI* __restrict__ in, T* __res
     }
 }
 
+// Single-problem z-pair copy for y slabs: a thread walks kZChunk z pairs of
+// one (y, bb) column, reading each input row once (a pair's upper voxel is
+// the next pair's lower one), where a tile of one pair reads both.  Grid
+// (ceil((nx+2)/256), ny, ceil((nz+1)/kZChunk)).
+constexpr int kZChunk = 32;
 template <typename T>
 __global__ __launch_bounds__(256) void zpair_y_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
                                                     int nz) {
-    zpair_y_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z);
+    const int bb = blockIdx.x * 256 + threadIdx.x;
+    if (bb > nx + 1) return;
+    const int x = bb - 1, y = blockIdx.y;
+    const int z0 = blockIdx.z * kZChunk, z1 = min(z0 + kZChunk, nz + 1);
+    const bool xin = x >= 0 && x < nx;
+    const T* col = in + (size_t)(xin ? x : 0) + (size_t)nx * y;  // (x, y, z) at col[nx ny z]
+    const size_t zs = (size_t)nx * ny;
+    // Clamped address (always in the volume), then a mask (the element types
+    // are integers): a conditional load, or a select the compiler turns into
+    // one, is a branch per pair that waits for each load before the next.
+    auto voxel = [&](int z) {
+        const T v = col[zs * (size_t)min(max(z, 0), nz - 1)];
+        return (T)(v & (T)(0u - (unsigned)(xin && z >= 0 && z < nz)));
+    };
+    T lo = voxel(z0 - 1);
+#pragma unroll 8
+    for (int zz = z0; zz < z1; ++zz) {
+        const T hi = voxel(zz);
+        store_pair(out, zpair_index(y, zz, bb, nz, nx), lo, hi);
+        lo = hi;
+    }
 }
 
 // out[y + nyp (x + nx z)] = in[x + nx (y + ny z)], 0 for ny <= y < nyp: the
@@ -303,11 +328,54 @@ __global__ __launch_bounds__(256) void row_pad_kernel(const float* __restrict__ 
     }
 }
 
+// Single-problem z-pair copy for x slabs: a workgroup takes one 32 x 32
+// (x, B) tile through kZChunk z pairs, each input plane of the tile read once
+// into LDS (a pair's upper plane is the next pair's lower one); the next
+// plane's loads are issued before the current pairs are written.  Grid
+// (ceil(nx/32), ceil((ny+2)/32), ceil((nz+1)/kZChunk)).
 template <typename T>
 __global__ __launch_bounds__(256) void zpair_x_kernel(const T* __restrict__ in, T* __restrict__ out, int nx, int ny,
                                                     int nz) {
-    __shared__ T tile[2][32][33];
-    zpair_x_tile<T, T>(in, out, nx, ny, nz, blockIdx.x, blockIdx.y, blockIdx.z, tile);
+    __shared__ T tile[2][32][33];  // [plane][B row][x]
+    const int x0 = blockIdx.x * 32, b0 = blockIdx.y * 32;
+    const int z0 = blockIdx.z * kZChunk, z1 = min(z0 + kZChunk, nz + 1);
+    const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;  // 32 x 8
+    const int x = x0 + tx;
+    // rows r = ty + 8 j of plane z, as in zpair_y_kernel: clamped addresses
+    // (the loads stay unconditional) and masks of the voxels in the grid,
+    // applied when the rows go to LDS so that nothing waits for the loads
+    // before then
+    auto fetch = [&](int z, T (&v)[4], T (&m)[4]) {
+        const int zc = min(max(z, 0), nz - 1);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            const int y = b0 + ty + 8 * j - 1;
+            const bool ok = x < nx && y >= 0 && y < ny && z >= 0 && z < nz;
+            const size_t o = (size_t)min(x, nx - 1) + (size_t)nx * ((size_t)min(max(y, 0), ny - 1) + (size_t)ny * zc);
+            v[j] = in[o];
+            m[j] = (T)(0u - (unsigned)ok);
+        }
+    };
+    auto put = [&](const T (&v)[4], const T (&m)[4], int c) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) tile[c][ty + 8 * j][tx] = (T)(v[j] & m[j]);
+    };
+    T v[4], m[4];
+    fetch(z0 - 1, v, m);
+    put(v, m, 0);
+    fetch(z0, v, m);
+    for (int zz = z0; zz < z1; ++zz) {
+        const int cur = (zz - z0 + 1) & 1, prev = cur ^ 1;  // planes zz and zz - 1
+        put(v, m, cur);
+        __syncthreads();
+        if (zz + 1 < z1) fetch(zz + 1, v, m);
+        for (int r = ty; r < 32; r += 8) {
+            const int xo = x0 + r, bb = b0 + tx;
+            if (xo < nx && bb <= ny + 1)
+                store_pair(out, zpair_index(xo, zz, bb, nz, ny), tile[prev][tx][r], tile[cur][tx][r]);
+        }
+        __syncthreads();  // every pair of this z is written before its lower plane is overwritten
+    }
 }
 
 // One A-pair group (four 16-bit elements at an index that is a multiple of
@@ -616,8 +684,8 @@ hipError_t launch_apair(const void* vol, void* qx, void* qy, int nx, int ny, int
 }
 
 hipError_t launch_zpair(const void* vol, void* qx, void* qy, size_t es, int nx, int ny, int nz, hipStream_t s) {
-    const dim3 gx(ceil_div(nx, 32), ceil_div((size_t)ny + 2, 32), nz + 1);
-    const dim3 gy(ceil_div((size_t)nx + 2, 256), ny, nz + 1);
+    const dim3 gx(ceil_div(nx, 32), ceil_div((size_t)ny + 2, 32), ceil_div((size_t)nz + 1, kZChunk));
+    const dim3 gy(ceil_div((size_t)nx + 2, 256), ny, ceil_div((size_t)nz + 1, kZChunk));
     if (es == 4) {
         hipLaunchKernelGGL((zpair_x_kernel<uint32_t>), gx, dim3(256), 0, s, (const uint32_t*)vol, (uint32_t*)qx, nx,
                            ny, nz);
