@@ -240,6 +240,9 @@ __device__ __forceinline__ void sid_step(SidState& s, const SidRay& r, const RD&
 // (copies below 4 GiB; the walk carries the offset in bytes) or a plain
 // global load by element index.
 template <typename TV, bool BUF> struct QuadReader;
+// A byte offset past every copy the buffer path reads (copies stay below it):
+// a read there returns zeros without touching memory.
+constexpr uint32_t kSidOutside = 0xfffffff0u;
 template <typename TV> struct QuadReader<TV, false> {
     static constexpr uint32_t kUnit = 1;  // element index
     static constexpr bool kAnyOffset = false;  // only a live lane's offset may be read
@@ -362,22 +365,26 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
     };
     if constexpr (QR::kAnyOffset && QR::kUnit == 2) {
         // 16-bit copies through the buffer path: the next slab's block is
-        // loaded before this slab's segments are summed, by every lane (a
-        // finished or waiting lane reads a value it does not use; past the
-        // copy the read gives zeros), into two alternating registers.  A load
-        // guarded per lane is a branch, at whose end the compiler waits for
-        // it, and so is a copy of the prefetched value.  At 512^3 / 512 views
-        // this takes FP16 Siddon from 46.0 to 42.2 ms; FP32, bound by the
-        // texture path's lines, gains nothing from it (49.6 against 49.3 ms),
-        // and keeps the direct form below.
-        float4 va = rd(e), vb = va;
+        // loaded before this slab's segments are summed, into two alternating
+        // registers.  Every lane issues the load (a load guarded per lane is a
+        // branch, at whose end the compiler waits for it, and so is a copy of
+        // the prefetched value); a lane that will not use it reads past the
+        // copy (zeros, no memory access), its offset formed with a mask.  At
+        // 512^3 / 512 views this takes FP16 Siddon from 46.0 to 42.6 ms; FP32,
+        // bound by the texture path's lines, gains nothing from it and keeps
+        // the direct form below.
+        auto next_off = [&](int outer_next) __attribute__((always_inline)) {
+            const uint32_t keep = 0u - (uint32_t)(live && outer_next >= lag);
+            return (e & keep) | (kSidOutside & ~keep);
+        };
+        float4 va = rd(next_off(0)), vb = va;
         for (int outer = 0; outer < trips; outer += 2) {
             Slab sl = advance(outer);
-            vb = rd(e);
+            vb = rd(next_off(outer + 1));
             accumulate(sl, va);
             if (outer + 1 < trips) {
                 sl = advance(outer + 1);
-                va = rd(e);
+                va = rd(next_off(outer + 2));
                 accumulate(sl, vb);
             }
         }
@@ -828,7 +835,7 @@ __global__ __launch_bounds__(BX* BY) void siddon_batch_kernel(const BatchTable* 
         const uint64_t es = bf ? 2ull : 4ull;
         const uint64_t qxb = zpair_elems_x(it.g.nx, it.g.ny, it.g.nz) * es;
         const uint64_t qyb = zpair_elems_y(it.g.nx, it.g.ny, it.g.nz) * es;
-        const bool buf = qxb < (1ull << 32) && qyb < (1ull << 32);
+        const bool buf = qxb < kSidOutside && qyb < kSidOutside;
         using BF = __hip_bfloat16;
         if (bf && buf)
             siddon_tile<BF, float, true>(it.g, (const BF*)it.in, (const BF*)it.in2, (unsigned)qxb, (unsigned)qyb, 1.0f,
@@ -999,7 +1006,7 @@ int launch_siddon(const tomo_geom* g, int compat, Buf vol, MutBuf sino, hipStrea
     ZPairs zp;
     hipError_t le = zp.alloc(vol.p, es, d.nx, d.ny, d.nz, s, x);
     if (le == hipSuccess) {
-        const bool buf = qx_bytes < (1ull << 32) && qy_bytes < (1ull << 32) && !x.index_reads;
+        const bool buf = qx_bytes < kSidOutside && qy_bytes < kSidOutside && !x.index_reads;
         TOMO_DISPATCH_STORAGE(vol.dtype, TV, {
             TOMO_DISPATCH_STORAGE(sino.dtype, TS, {
                 if (buf)
