@@ -363,16 +363,17 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
         acc = fmaf(sl.c2 - sl.c1, V1, acc);
         acc = fmaf(sl.c3 - sl.c2, V2, acc);
     };
-    if constexpr (QR::kAnyOffset && QR::kUnit == 2) {
-        // 16-bit copies through the buffer path: the next slab's block is
+    if constexpr (QR::kAnyOffset) {
+        // Copies read through the buffer path: the next slab's block is
         // loaded before this slab's segments are summed, into two alternating
         // registers.  Every lane issues the load (a load guarded per lane is a
         // branch, at whose end the compiler waits for it, and so is a copy of
         // the prefetched value); a lane that will not use it reads past the
         // copy (zeros, no memory access), its offset formed with a mask.  At
-        // 512^3 / 512 views this takes FP16 Siddon from 46.0 to 42.6 ms; FP32,
-        // bound by the texture path's lines, gains nothing from it and keeps
-        // the direct form below.
+        // 512^3 / 512 views: FP32 48.5 -> 46.5 ms, FP16 46.0 -> 42.7 ms; with
+        // every lane reading its stale offset instead, the idle lanes' lines
+        // cost FP32 what the prefetch gained.  The 64-bit index reads keep the
+        // direct form below.
         auto next_off = [&](int outer_next) __attribute__((always_inline)) {
             const uint32_t keep = 0u - (uint32_t)(live && outer_next >= lag);
             return (e & keep) | (kSidOutside & ~keep);
