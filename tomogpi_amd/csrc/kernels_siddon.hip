@@ -372,8 +372,10 @@ __device__ __forceinline__ void sid_walk_quads(const SidRay& r, const QR& rd, in
         // copy (zeros, no memory access), its offset formed with a mask.  At
         // 512^3 / 512 views: FP32 48.5 -> 46.5 ms, FP16 46.0 -> 42.7 ms; with
         // every lane reading its stale offset instead, the idle lanes' lines
-        // cost FP32 what the prefetch gained.  The 64-bit index reads keep the
-        // direct form below.
+        // cost FP32 what the prefetch gained.  The 64-bit index reads (copies
+        // of 4 GiB or more) keep the direct form below: at 1024^3 the walk is
+        // bound by the memory traffic, and the same prefetch (idle lanes
+        // reading the copy's first block) took FP32 from 482 to 520 ms.
         auto next_off = [&](int outer_next) __attribute__((always_inline)) {
             const uint32_t keep = 0u - (uint32_t)(live && outer_next >= lag);
             return (e & keep) | (kSidOutside & ~keep);
