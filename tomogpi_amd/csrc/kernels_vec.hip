@@ -393,11 +393,19 @@ __device__ __forceinline__ void store_group(uint16_t* __restrict__ out, size_t o
 
 // A voxel of an x-fastest volume as a 16-bit copy element (0 outside the
 // grid): a bit copy of a 16-bit volume, or an FP32 voxel rounded to BF16.
+// Read at the clamped address (always in the volume), then masked: a load
+// guarded by the bounds test is a branch, which waits for it before the
+// next.  The empty statement keeps the compiler from seeing that the mask is
+// 0 or ~0 (it would turn the AND back into a select, and that into the
+// branch); it emits no instruction.
 template <typename TI>
 __device__ __forceinline__ uint16_t voxel16(const TI* __restrict__ in, int x, int y, int z, int nx, int ny, int nz) {
-    return (x >= 0 && x < nx && y >= 0 && y < ny && z >= 0 && z < nz)
-               ? copy_cvt<uint16_t>(in[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)])
-               : (uint16_t)0;
+    const bool ok = (x >= 0) & (x < nx) & (y >= 0) & (y < ny) & (z >= 0) & (z < nz);
+    const size_t o = (size_t)min(max(x, 0), nx - 1) +
+                     (size_t)nx * ((size_t)min(max(y, 0), ny - 1) + (size_t)ny * min(max(z, 0), nz - 1));
+    uint32_t m = 0u - (uint32_t)ok;
+    asm("" : "+v"(m));
+    return (uint16_t)(copy_cvt<uint16_t>(in[o]) & m);
 }
 
 // A-pair copy for x slabs (A = x, B = y): a tile of 32 plane pairs x 32 B
