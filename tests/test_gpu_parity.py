@@ -325,6 +325,34 @@ def test_degenerate_shapes_bit_exact(ctx, name):
         G.close()
 
 
+@pytest.mark.parametrize("dt", ["f16", "bf16"])
+@pytest.mark.parametrize("name", list(EDGE_GEOMS))
+def test_degenerate_shapes_16bit_bit_exact(ctx, name, dt):
+    """The degenerate shapes with 16-bit volumes: RS reads A-pair copies (one
+    plane of groups more than the volume has along the slab axis, zero planes
+    at both ends), Siddon 16-bit z-pair copies, both through 32-bit buffer
+    offsets and through 64-bit element indices; every sum equals the FP32
+    oracle's on the rounded voxel values, bit for bit."""
+    g = EDGE_GEOMS[name]
+    print(f"[case] geometry={name} volume={dt}", flush=True)
+    G = lib.Geometry(ctx, to_desc(g))
+    try:
+        o = Oracle(g)
+        rng = np.random.default_rng(22)
+        v16 = dev(rng.random(g.vol_shape, dtype=np.float32)).to(torch.float16 if dt == "f16" else torch.bfloat16)
+        rounded = v16.float().cpu().numpy()
+        refs = {"rs": o.rs(rounded), "rs_cube": o.rs(rounded, cube=True), "siddon": o.siddon(rounded)}
+        out = torch.empty(g.sino_shape, dtype=torch.float32, device="cuda")
+        for op, ref in refs.items():
+            for index_reads in (False, True):
+                flags = (lib.FLAG_RS_FOV_CUBE if op == "rs_cube" else 0) | (lib.FLAG_INDEX_READS if index_reads else 0)
+                G.project(v16, out, lib.PROJ_SIDDON if op == "siddon" else lib.PROJ_RS, flags)
+                got = out.cpu().numpy()
+                assert np.array_equal(got, ref), (op, index_reads, rel_l2(got, ref), rel_linf(got, ref))
+    finally:
+        G.close()
+
+
 @pytest.mark.parametrize("op", ["rs", "rs_cube", "siddon", "vi"])
 def test_adjoint_dot(setup, ctx, op):
     """<A x, y> = <x, A^T y> with the GPU operator and its GPU transpose (rel 1e-5)."""
